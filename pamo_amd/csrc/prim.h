@@ -1,0 +1,213 @@
+// Device-wide primitives written for wave64 / LDS on gfx950:
+//   * exclusive prefix sum of int32 (reduce-then-scan, 2048 items per block);
+//   * stable LSD radix sort of (uint32 key, uint32 value) pairs, 8 bits per
+//     pass; within a tile, ranks of equal digits come from wave64 ballots
+//     (match of the 8 digit bits) and an LDS prefix across the 4 waves, so
+//     the order of equal keys is preserved exactly.
+#pragma once
+#include "hip_common.h"
+
+namespace pamo {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanIpt = 8;
+constexpr int kScanTile = kScanBlock * kScanIpt;
+
+// Block-wide exclusive scan of one value per thread (256 threads, 4 waves).
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds_wave /*[4]*/, int& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds_wave[w] = x;
+    __syncthreads();
+    int wbase = 0;
+    int tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int s = lds_wave[i];
+        if (i < w) wbase += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + x - v;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_tiles_k(const int* __restrict__ in, int* __restrict__ out,
+                                                           int* __restrict__ tile_sums, long long n) {
+    __shared__ int lds[4];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanIpt;
+    int v[kScanIpt];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanIpt; ++i) {
+        long long idx = base + i;
+        v[i] = idx < n ? in[idx] : 0;
+        s += v[i];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, lds, total);
+#pragma unroll
+    for (int i = 0; i < kScanIpt; ++i) {
+        long long idx = base + i;
+        if (idx < n) out[idx] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 0 && tile_sums) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_add_k(int* __restrict__ out, const int* __restrict__ tile_off,
+                                                         long long n) {
+    const long long base = (long long)blockIdx.x * kScanTile;
+    const int add = tile_off[blockIdx.x];
+    for (int i = threadIdx.x; i < kScanTile; i += kScanBlock) {
+        long long idx = base + i;
+        if (idx < n) out[idx] += add;
+    }
+}
+
+// Workspace (ints) needed by scan_exclusive for n elements.
+inline long long scan_workspace(long long n) {
+    long long w = 0;
+    while (n > kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        w += n + 16;
+    }
+    return w + 16;
+}
+
+// out[i] = sum_{j<i} in[j]; in may alias out.
+inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStream_t st) {
+    if (n <= 0) return 0;
+    long long nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 1) {
+        hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(kScanBlock), 0, st, in, out, (int*)nullptr, n);
+        PAMO_HIP(hipGetLastError());
+        return 0;
+    }
+    int* sums = ws;
+    hipLaunchKernelGGL(scan_tiles_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, sums, n);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(scan_exclusive(sums, sums, nb, ws + nb + 16, st));
+    hipLaunchKernelGGL(scan_add_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, out, (const int*)sums, n);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int kSortBlock = 256;
+constexpr int kSortIpt = 8;
+constexpr int kSortTile = kSortBlock * kSortIpt;
+
+__global__ void __launch_bounds__(kSortBlock) radix_hist_k(const uint32_t* __restrict__ keys, int n, int shift,
+                                                           int* __restrict__ counts, int nb) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int c = 0; c < kSortIpt; ++c) {
+        long long idx = base + c * kSortBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    counts[(long long)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortBlock) radix_scatter_k(const uint32_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout,
+                                                              uint32_t* __restrict__ vout, int n, int shift,
+                                                              const int* __restrict__ offs, int nb) {
+    __shared__ int base[256];
+    __shared__ int wcnt[4][256];
+    const int t = threadIdx.x, w = t >> 6;
+    base[t] = offs[(long long)t * nb + blockIdx.x];
+    const long long tile = (long long)blockIdx.x * kSortTile;
+    for (int c = 0; c < kSortIpt; ++c) {
+        const long long idx = tile + c * kSortBlock + t;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) wcnt[ww][t] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = lanes_below(peers);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        int run = 0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            int x = wcnt[ww][t];
+            wcnt[ww][t] = run;
+            run += x;
+        }
+        __syncthreads();
+        if (valid) {
+            const int pos = base[d] + wcnt[w][d] + rank;
+            kout[pos] = key;
+            vout[pos] = vin[idx];
+        }
+        __syncthreads();
+        base[t] += run;
+        __syncthreads();
+    }
+}
+
+inline long long radix_workspace(long long n) {
+    long long nb = (n + kSortTile - 1) / kSortTile;
+    return 256 * nb + scan_workspace(256 * nb) + 16;
+}
+
+// Stable sort of (keys, vals) on the low `bits` bits (multiple of 8).
+// Results end in (keys, vals); k2/v2 are n-element scratch.
+inline int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* k2, uint32_t* v2, int n, int bits, int* ws,
+                            hipStream_t st) {
+    if (n <= 1) return 0;
+    const int nb = (n + kSortTile - 1) / kSortTile;
+    int* counts = ws;
+    int* sws = ws + 256LL * nb;
+    uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_k, dim3(nb), dim3(kSortBlock), 0, st, ka, n, shift, counts, nb);
+        PAMO_HIP(hipGetLastError());
+        PAMO_TRY(scan_exclusive(counts, counts, 256LL * nb, sws, st));
+        hipLaunchKernelGGL(radix_scatter_k, dim3(nb), dim3(kSortBlock), 0, st, ka, va, kb, vb, n, shift,
+                           (const int*)counts, nb);
+        PAMO_HIP(hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        PAMO_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        PAMO_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    }
+    return 0;
+}
+
+template <typename T>
+__global__ void fill_k(T* __restrict__ p, T v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T>
+inline int fill(T* p, T v, long long n, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(fill_k<T>, dim3(grid_for(n, 256)), dim3(256), 0, st, p, v, n);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace pamo

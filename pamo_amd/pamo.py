@@ -1,0 +1,172 @@
+"""Drop-in host API: ``PaMO`` and the per-round operator ``CUDSP_Free``.
+
+Mirrors simp_cuda/pamo/__init__.py:18-205 (class PaMO) and
+simp_cuda/src/pybind.cpp:16-103 (class CUDSP_Free) of the reference, on top
+of libpamo_hip.so.  Tensors stay on the GPU and are handed to the library by
+pointer; the library runs on its own HIP stream, so the caller's stream is
+synchronised before each call.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _hip
+
+
+def _check_input(t: torch.Tensor, name: str):
+    # pybind.cpp:8-14 (AT_ASSERTM -> RuntimeError)
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+class CUDSP_Free:
+    """One collapse round per ``forward`` call (pybind.cpp:52-103).
+
+    State (persistent buffers, the stale-buffer semantics of the reference)
+    lives in the device context and carries over between calls, exactly as
+    one ``_C.CUDSP_Free()`` object does.
+    """
+
+    def __init__(self, device: int | None = None):
+        if device is None:
+            device = torch.cuda.current_device()
+        self.device = device
+        self.ctx = _hip.Context(device)
+
+    def forward(self, points, triangles, verts_undo, n_verts_undo, scale, threshold, is_stuck, init):
+        _check_input(points, "points")
+        _check_input(triangles, "triangles")
+        _check_input(verts_undo, "verts_undo")
+        if points.dtype != torch.float32:
+            raise RuntimeError("points type must match the pamo class")
+        if triangles.dtype != torch.int32:
+            raise RuntimeError("triangles type must match the pamo class")
+        if verts_undo.dtype != torch.int32:
+            raise RuntimeError("verts_undo must be int32")
+        n_pts, n_tris = int(points.shape[0]), int(triangles.shape[0])
+        n_undo = int(n_verts_undo)
+        dev = points.device
+        out_pts = torch.empty((n_pts, 3), dtype=torch.float32, device=dev)
+        out_tris = torch.empty((n_tris, 3), dtype=torch.int32, device=dev)
+        out_occ = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
+        out_map = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
+        cap = 4 * (n_pts + 1) + n_undo + 16
+        out_undo = torch.empty(cap, dtype=torch.int32, device=dev)
+        n_out = ctypes.c_int(0)
+        torch.cuda.current_stream(dev).synchronize()
+        _hip.check(_hip.lib().pamo_round(
+            self.ctx.ptr, points.data_ptr(), n_pts, triangles.data_ptr(), n_tris,
+            verts_undo.data_ptr() if n_undo > 0 else None, n_undo, float(scale), float(threshold),
+            int(bool(is_stuck)), int(bool(init)), out_pts.data_ptr(), out_tris.data_ptr(), out_occ.data_ptr(),
+            out_map.data_ptr(), out_undo.data_ptr(), cap, ctypes.byref(n_out)), "pamo_round")
+        return out_pts, out_tris, out_occ, out_map, out_undo[: n_out.value].clone()
+
+    __call__ = forward
+
+    def debug(self, what: int, dtype=torch.int32, width: int = 1, cap: int | None = None):
+        """Intermediate arrays of the last round (parity tests)."""
+        lib = _hip.lib()
+        n = ctypes.c_longlong(0)
+        if what in (_hip.PAMO_DBG_UNDO_ITERS, _hip.PAMO_DBG_N_REGULAR):
+            _hip.check(lib.pamo_round_debug(self.ctx.ptr, what, None, 0, ctypes.byref(n)), "pamo_round_debug")
+            return n.value
+        if cap is None:
+            cap = 1 << 26
+        buf = torch.empty(cap, dtype=dtype, device=torch.device("cuda", self.device))
+        _hip.check(lib.pamo_round_debug(self.ctx.ptr, what, buf.data_ptr(), cap, ctypes.byref(n)), "pamo_round_debug")
+        return buf[: n.value * (2 if what == _hip.PAMO_DBG_EDGES else 1)].clone()
+
+
+def face_centroid_mean(ctx: _hip.Context, points: torch.Tensor, triangles: torch.Tensor) -> np.ndarray:
+    out = (ctypes.c_float * 3)()
+    torch.cuda.current_stream(points.device).synchronize()
+    _hip.check(_hip.lib().pamo_centroid(ctx.ptr, points.data_ptr(), int(points.shape[0]), triangles.data_ptr(),
+                                        int(triangles.shape[0]), out), "pamo_centroid")
+    return np.array([out[0], out[1], out[2]], dtype=np.float32)
+
+
+def simplify_device(ctx: _hip.Context, verts: torch.Tensor, faces: torch.Tensor, ratio: float, tolerance: int = 4,
+                    threshold: float = 1e-3, iters: int = 1000000, min_verts: int = 10000000000):
+    """The stage-2 round loop of PaMO.run on device; verts must be centred."""
+    _check_input(verts, "points")
+    _check_input(faces, "triangles")
+    nv, nf = int(verts.shape[0]), int(faces.shape[0])
+    out_v = torch.empty_like(verts)
+    out_f = torch.empty_like(faces)
+    onv, onf, rounds = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    hist = np.zeros(1 << 16, dtype=np.int32)
+    torch.cuda.current_stream(verts.device).synchronize()
+    _hip.check(_hip.lib().pamo_simplify(
+        ctx.ptr, verts.data_ptr(), nv, faces.data_ptr(), nf, float(ratio), int(tolerance), float(threshold),
+        int(iters), int(min_verts), out_v.data_ptr(), out_f.data_ptr(), ctypes.byref(onv), ctypes.byref(onf),
+        hist.ctypes.data, len(hist), ctypes.byref(rounds)), "pamo_simplify")
+    r = rounds.value
+    return out_v[: onv.value], out_f[: onf.value], hist[: min(r, len(hist))].copy()
+
+
+class _Bounds:
+    def __init__(self, v):
+        v = np.asarray(v)
+        self.bounds = np.stack([v.min(0), v.max(0)])
+
+
+class PaMO(torch.nn.Module):
+    """``PaMO(input_mesh, use_stage1=True, use_stage3=True)`` (pamo/__init__.py:18-53).
+
+    ``input_mesh`` is a trimesh-like object (``vertices``, ``faces`` and,
+    optionally, ``bounding_box.bounds``) or a ``(vertices, faces)`` pair.
+    Stage 1 (SDF remeshing) is not part of this package: asking for it raises.
+    """
+
+    def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True):
+        super().__init__()
+        if use_stage1:
+            raise NotImplementedError(
+                "stage-1 remeshing (SDF + dual marching cubes) is not provided by pamo_amd; "
+                "construct PaMO(..., use_stage1=False)")
+        self.use_stage1 = use_stage1
+        self.use_stage3 = use_stage3
+        if isinstance(input_mesh, (tuple, list)):
+            V, F = input_mesh
+        else:
+            V, F = input_mesh.vertices, input_mesh.faces
+        self.gt_vertices = np.asarray(V, dtype=np.float64).copy()
+        self.gt_faces = np.asarray(F, dtype=np.int64).copy()
+        bb = getattr(input_mesh, "bounding_box", None)
+        self.bbox = bb.bounds if bb is not None else _Bounds(self.gt_vertices).bounds
+        self.ctx = _hip.Context(torch.cuda.current_device())
+        if use_stage3:
+            from . import stage3
+            self.config = stage3.Stage3Config()
+            self.system = stage3.Stage3System(self.config)
+        self.target_faces = None
+        self.timings = {}
+
+    def run(self, points, triangles, ratio, tolerance=4, threshold=1e-3, iter=1000000, min_verts=10000000000):
+        """pamo/__init__.py:122-205; returns numpy (vertices, faces)."""
+        self.target_faces = max(int(ratio * len(triangles)), min_verts)
+        t0 = time.perf_counter()
+        tris_mean = face_centroid_mean(self.ctx, points, triangles)
+        verts = points - torch.from_numpy(tris_mean).to(points.device)
+        v2, f2, hist = simplify_device(self.ctx, verts.contiguous(), triangles, ratio, tolerance, threshold, iter,
+                                       min_verts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        self.rounds = len(hist)
+        self.face_hist = hist
+        verts_np = v2.cpu().numpy() + tris_mean
+        faces_np = f2.cpu().numpy()
+        self.timings = {"stage2_s": t1 - t0}
+        if self.use_stage3:
+            from . import stage3
+            t2 = time.perf_counter()
+            verts_np, faces_np = stage3.process(self.gt_vertices, self.gt_faces, verts_np, faces_np, 5,
+                                                system=self.system, config=self.config)
+            self.timings["stage3_s"] = time.perf_counter() - t2
+        return verts_np, faces_np
