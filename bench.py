@@ -1,0 +1,148 @@
+"""Benchmark: 2M-face mesh -> 20k faces on one MI355X.
+
+Workload (synthetic): marching-tetrahedra contour of the seed-1 union of 16
+rotated ellipsoids and 4 boxes on [-1,1]^3 at grid 296 (2,006,528 faces,
+1,003,254 vertices, fp32 positions, int32 faces, watertight and manifold),
+simplified with ratio 0.01, tolerance 4, threshold 1e-3, iter 100000,
+min_verts 0, as PaMO(mesh, use_stage1=False, ...).run(...).
+
+One step = one full run from device-resident input arrays to device output
+arrays (centring, every stage-2 round with its guard and undo loop, and
+stage 3 when enabled).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
+    p.add_argument("--stage3", type=int, default=-1, help="1/0 force stage 3 on/off (default: on if available)")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
+              "build", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import face_centroid_mean, simplify_device
+
+    ratio, tol, thr, iters = 0.01, 4, 1e-3, 100000
+    V, F = synth.headline_mesh(1)
+    nF0, nV0 = len(F), len(V)
+    dev = torch.device("cuda", 0)
+    Vd = torch.from_numpy(V).to(dev)
+    Fd = torch.from_numpy(F).to(dev)
+    ctx = _hip.Context(0)
+    prof = _hip.Profiler(ctx)
+
+    def one_run():
+        mean = face_centroid_mean(ctx, Vd, Fd)
+        verts = (Vd - torch.from_numpy(mean).to(dev)).contiguous()
+        return simplify_device(ctx, verts, Fd, ratio, tol, thr, iters, 0)
+
+    for _ in range(args.warmup):
+        one_run()
+    torch.cuda.synchronize()
+    # untimed pass with every kernel bracketed by HIP events: per-kernel table
+    prof.reset()
+    prof.enable(True)
+    one_run()
+    torch.cuda.synchronize()
+    prof.enable(False)
+    stats = prof.stats()
+    dom_name = max(stats, key=lambda s: s["total_ms"])["name"]
+    # timed region: only the dominant kernel is bracketed
+    prof.reset()
+    prof.enable(True, kernels=[dom_name])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vo, fo, hist = one_run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof.enable(False)
+    per_step = (t1 - t0) / args.steps
+    face_rounds = int(nF0 + hist[:-1].sum())  # faces entering each round
+
+    # roofline of the dominant kernel (live HIP-event timing inside the library)
+    dom = [s for s in prof.stats() if s["name"] == dom_name][0]
+    achieved = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9 if dom["total_ms"] > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": 8000.0,
+                "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "avg_launch_us": round(1e3 * dom["total_ms"] / max(dom["launches"], 1), 2),
+                "launches": dom["launches"]}
+
+    # CPU baseline: the oracle on the first rounds of the same workload,
+    # scaled to the whole job by faces processed.
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        oracle.build()
+        mean = face_centroid_mean(ctx, Vd, Fd)
+        Vc = (V - mean).astype(np.float32)
+        r = oracle.Round()
+        budget = args.cpu_seconds
+        rounds_done, frs, t_cpu = 0, 0, 0.0
+        # one round at a time on the oracle until the budget is spent
+        hist_gpu = list(hist)
+        cur_rounds = 1
+        while True:
+            tc0 = time.perf_counter()
+            _, _, h = oracle.run(Vc, F, ratio, tol, thr, iters=cur_rounds, min_verts=0, ctx=oracle.Round())
+            dt = time.perf_counter() - tc0
+            rounds_done = len(h)
+            frs = int(nF0 + h[:-1].sum())
+            t_cpu = dt
+            if dt > budget / 3 or rounds_done >= len(hist_gpu):
+                break
+            cur_rounds *= 2
+        est = t_cpu * face_rounds / frs
+        cpu = {"value": round(est, 3), "unit": "s", "cores": oracle.lib().orc_num_threads(), "kind": "port",
+               "sample": f"oracle on the first {rounds_done} rounds ({frs} face-rounds of {face_rounds}) of the "
+                         f"same workload in {t_cpu:.2f}s, scaled by face-rounds"}
+
+    out = {
+        "metric": "2M->20k faces wall-time (stage 2)",
+        "value": round(per_step, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(per_step * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ellipsoid/box union, marching tetrahedra grid 296", "faces_in": nF0,
+                   "verts_in": nV0, "faces_out": int(len(fo)), "ratio": ratio, "tolerance": tol,
+                   "threshold": thr, "rounds": int(len(hist)), "stage3": False},
+        "faces_per_s": round(nF0 / per_step, 1),
+        "face_rounds": face_rounds,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": stats,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -50,6 +50,12 @@ int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tri
 #define PAMO_DBG_QUADRICS 6   /* float32 (10, n_pts) component-major */
 #define PAMO_DBG_UNDO_ITERS 7 /* count only                      */
 #define PAMO_DBG_N_REGULAR 8  /* count only                      */
+#define PAMO_DBG_BVH_N 9      /* count only: leaves of the last LBVH */
+#define PAMO_DBG_BVH_CODES 10 /* uint32 Morton codes, sorted      */
+#define PAMO_DBG_BVH_FACES 11 /* int32 face per sorted leaf       */
+#define PAMO_DBG_BVH_LEFT 12  /* int32 left child of internal node */
+#define PAMO_DBG_BVH_RIGHT 13 /* int32 right child                */
+#define PAMO_DBG_BVH_BOXES 14 /* float32 (2n-1, 8): lo xyz0, hi xyz0 */
 int pamo_round_debug(pamo_ctx* ctx, int what, void* d_out, long long cap, long long* n_out);
 
 /* The whole stage-2 loop of PaMO.run on already-centred vertices
@@ -62,6 +68,17 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
                   int tolerance, float threshold, long long iters, long long min_verts, float* d_out_verts,
                   int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
                   int* rounds_out);
+
+/* Launch profiler (no counterpart in the reference, which only prints host
+ * wall clock, simp_cuda/pamo/__init__.py:139-148,187-190): HIP events are
+ * recorded around launches of the kernels selected by kernel_mask (bit k =
+ * kernel id k); per kernel, pamo_prof_get returns the launch count, summed
+ * device time and summed algorithmic bytes (compulsory HBM traffic). */
+int pamo_prof_enable(pamo_ctx* ctx, unsigned long long kernel_mask);
+int pamo_prof_reset(pamo_ctx* ctx);
+int pamo_prof_count(void);
+int pamo_prof_get(pamo_ctx* ctx, int kernel, const char** name, long long* launches, double* total_ms,
+                  double* bytes);
 
 #ifdef __cplusplus
 }

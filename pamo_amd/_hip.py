@@ -21,6 +21,12 @@ PAMO_DBG_RING = 5
 PAMO_DBG_QUADRICS = 6
 PAMO_DBG_UNDO_ITERS = 7
 PAMO_DBG_N_REGULAR = 8
+PAMO_DBG_BVH_N = 9
+PAMO_DBG_BVH_CODES = 10
+PAMO_DBG_BVH_FACES = 11
+PAMO_DBG_BVH_LEFT = 12
+PAMO_DBG_BVH_RIGHT = 13
+PAMO_DBG_BVH_BOXES = 14
 
 # (name, restype, argtypes) for every symbol declared in include/pamo_hip.h
 SIGNATURES = [
@@ -33,6 +39,11 @@ SIGNATURES = [
     ("pamo_round_debug", c_int, [c_vp, c_int, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_simplify", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_float, c_ll, c_ll, c_vp, c_vp,
                               ctypes.POINTER(c_int), ctypes.POINTER(c_int), c_vp, c_int, ctypes.POINTER(c_int)]),
+    ("pamo_prof_enable", c_int, [c_vp, ctypes.c_ulonglong]),
+    ("pamo_prof_reset", c_int, [c_vp]),
+    ("pamo_prof_count", c_int, []),
+    ("pamo_prof_get", c_int, [c_vp, c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_ll),
+                              ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
 ]
 
 _lib = None
@@ -81,3 +92,47 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Profiler:
+    """HIP-event timing of selected kernels inside the library."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def enable(self, on: bool = True, kernels=None):
+        if not on:
+            mask = 0
+        elif kernels is None:
+            mask = (1 << lib().pamo_prof_count()) - 1
+        else:
+            names = self.names()
+            mask = 0
+            for k in kernels:
+                mask |= 1 << names.index(k)
+        check(lib().pamo_prof_enable(self.ctx.ptr, mask), "pamo_prof_enable")
+
+    def reset(self):
+        check(lib().pamo_prof_reset(self.ctx.ptr), "pamo_prof_reset")
+
+    def names(self):
+        out = []
+        for k in range(lib().pamo_prof_count()):
+            nm = ctypes.c_char_p()
+            check(lib().pamo_prof_get(self.ctx.ptr, k, ctypes.byref(nm), None, None, None), "pamo_prof_get")
+            out.append(nm.value.decode())
+        return out
+
+    def stats(self):
+        res = []
+        for k in range(lib().pamo_prof_count()):
+            nm = ctypes.c_char_p()
+            n = c_ll()
+            ms = c_double()
+            by = c_double()
+            check(lib().pamo_prof_get(self.ctx.ptr, k, ctypes.byref(nm), ctypes.byref(n), ctypes.byref(ms),
+                                      ctypes.byref(by)), "pamo_prof_get")
+            if n.value:
+                res.append({"name": nm.value.decode(), "launches": n.value, "total_ms": round(ms.value, 4),
+                            "bytes": by.value})
+        return res

@@ -50,83 +50,87 @@ __device__ inline bool ref_coplanar_d(const float A[3][3], const float B[3][3]) 
     return true;
 }
 
+struct V3f {
+    float x, y, z;
+};
+
+// One triangle's contribution to the overlap test along the plane-plane
+// line: the lone vertex (on its own side of the other plane) is V1, the two
+// others V0 / V2.  Register-only: the three vertex rotations are selected
+// with conditional moves, never by indexing.
+__device__ __forceinline__ void isect_range(V3f a, V3f b, V3f c, float s0, float s1, float s2, const double dir[3],
+                                            float& rmin, float& rmax, V3f& l0, V3f& l1) {
+    int rot;  // (o0, o1, o2) = (rot, rot + 1, rot + 2) mod 3
+    if ((s0 && s1) && (s0 < 0.0f) == (s1 < 0.0f)) rot = 1;
+    else if ((s1 && s2) && (s1 < 0.0f) == (s2 < 0.0f)) rot = 2;
+    else if (!s1 && (s2 < 0.0f) != (s0 < 0.0f)) rot = 1;
+    else rot = 0;
+    V3f P0 = rot == 0 ? a : (rot == 1 ? b : c);
+    V3f P1 = rot == 0 ? b : (rot == 1 ? c : a);
+    V3f P2 = rot == 0 ? c : (rot == 1 ? a : b);
+    float q0 = rot == 0 ? s0 : (rot == 1 ? s1 : s2);
+    float q1 = rot == 0 ? s1 : (rot == 1 ? s2 : s0);
+    float q2 = rot == 0 ? s2 : (rot == 1 ? s0 : s1);
+    const double db = dir[0] * (double)P1.x + dir[1] * (double)P1.y + dir[2] * (double)P1.z;
+    if (q1) {
+        const double da = dir[0] * (double)P0.x + dir[1] * (double)P0.y + dir[2] * (double)P0.z;
+        const double dc = dir[0] * (double)P2.x + dir[1] * (double)P2.y + dir[2] * (double)P2.z;
+        float f0 = q1 / (q1 - q0);
+        float f1 = q1 / (q1 - q2);
+        double off0 = f0 * (da - db);
+        double off1 = f1 * (dc - db);
+        if (off0 > off1) {
+            double td = off0; off0 = off1; off1 = td;
+            float tf = f0; f0 = f1; f1 = tf;
+            V3f tv = P0; P0 = P2; P2 = tv;
+        }
+        rmin = (float)(db + off0);
+        rmax = (float)(db + off1);
+        const float g0 = 1.0f - f0, g1 = 1.0f - f1;
+        l0 = {g0 * P1.x + f0 * P0.x, g0 * P1.y + f0 * P0.y, g0 * P1.z + f0 * P0.z};
+        l1 = {g1 * P1.x + f1 * P2.x, g1 * P1.y + f1 * P2.y, g1 * P1.z + f1 * P2.z};
+    } else {
+        rmin = rmax = (float)db;
+        l0 = P1;
+        l1 = P1;
+    }
+}
+
 __device__ inline bool ref_isect_d(const float A[3][3], const float B[3][3], float r1[3], float r2[3]) {
-    float side[2][3];
     double pa[4], pb[4];
     plane_of(A, pa);
-#pragma unroll
-    for (int v = 0; v < 3; ++v) side[1][v] = (float)plane_side_d(pa, B[v]);
-    if (!side[1][0] && !side[1][1] && !side[1][2]) return false;
-    if ((side[1][0] && side[1][1] && side[1][2]) && (side[1][0] < 0.0f) == (side[1][1] < 0.0f) &&
-        (side[1][0] < 0.0f) == (side[1][2] < 0.0f))
-        return false;
+    const float b0 = (float)plane_side_d(pa, B[0]), b1 = (float)plane_side_d(pa, B[1]),
+                b2 = (float)plane_side_d(pa, B[2]);
+    if (!b0 && !b1 && !b2) return false;
+    if ((b0 && b1 && b2) && (b0 < 0.0f) == (b1 < 0.0f) && (b0 < 0.0f) == (b2 < 0.0f)) return false;
     plane_of(B, pb);
-#pragma unroll
-    for (int v = 0; v < 3; ++v) side[0][v] = (float)plane_side_d(pb, A[v]);
-    if ((side[0][0] && side[0][1] && side[0][2]) && (side[0][0] < 0.0f) == (side[0][1] < 0.0f) &&
-        (side[0][0] < 0.0f) == (side[0][2] < 0.0f))
-        return false;
+    const float a0 = (float)plane_side_d(pb, A[0]), a1 = (float)plane_side_d(pb, A[1]),
+                a2 = (float)plane_side_d(pb, A[2]);
+    if ((a0 && a1 && a2) && (a0 < 0.0f) == (a1 < 0.0f) && (a0 < 0.0f) == (a2 < 0.0f)) return false;
     double dir[3];
     dir[0] = pa[1] * pb[2] - pa[2] * pb[1];
     dir[1] = pa[2] * pb[0] - pa[0] * pb[2];
     dir[2] = pa[0] * pb[1] - pa[1] * pb[0];
-    float rmin[2], rmax[2], loc[2][2][3];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const float(*T)[3] = t == 0 ? A : B;
-        const float* sd = side[t];
-        int o0, o1, o2;
-        if ((sd[0] && sd[1]) && (sd[0] < 0.0f) == (sd[1] < 0.0f)) {
-            o0 = 1; o1 = 2; o2 = 0;
-        } else if ((sd[1] && sd[2]) && (sd[1] < 0.0f) == (sd[2] < 0.0f)) {
-            o0 = 2; o1 = 0; o2 = 1;
-        } else if (!sd[1] && (sd[2] < 0.0f) != (sd[0] < 0.0f)) {
-            o0 = 1; o1 = 2; o2 = 0;
+    float min0, max0, min1, max1;
+    V3f l00, l01, l10, l11;
+    isect_range({A[0][0], A[0][1], A[0][2]}, {A[1][0], A[1][1], A[1][2]}, {A[2][0], A[2][1], A[2][2]}, a0, a1, a2,
+                dir, min0, max0, l00, l01);
+    isect_range({B[0][0], B[0][1], B[0][2]}, {B[1][0], B[1][1], B[1][2]}, {B[2][0], B[2][1], B[2][2]}, b0, b1, b2,
+                dir, min1, max1, l10, l11);
+    if ((max0 > min1) && (min0 < max1)) {
+        V3f x, y;
+        if (min0 >= min1) {
+            x = l00;
+            y = (max0 <= max1) ? l01 : l11;
+        } else if (max0 <= max1) {
+            x = l01;
+            y = l10;
         } else {
-            o0 = 0; o1 = 1; o2 = 2;
+            x = l10;
+            y = l11;
         }
-        const double db = dir[0] * (double)T[o1][0] + dir[1] * (double)T[o1][1] + dir[2] * (double)T[o1][2];
-        const float sc = sd[o1];
-        if (sc) {
-            const double da = dir[0] * (double)T[o0][0] + dir[1] * (double)T[o0][1] + dir[2] * (double)T[o0][2];
-            const double dc = dir[0] * (double)T[o2][0] + dir[1] * (double)T[o2][1] + dir[2] * (double)T[o2][2];
-            float f0 = sc / (sc - sd[o0]);
-            float f1 = sc / (sc - sd[o2]);
-            double off0 = f0 * (da - db);
-            double off1 = f1 * (dc - db);
-            if (off0 > off1) {
-                double td = off0; off0 = off1; off1 = td;
-                float tf = f0; f0 = f1; f1 = tf;
-                int ti = o0; o0 = o2; o2 = ti;
-            }
-            rmin[t] = (float)(db + off0);
-            rmax[t] = (float)(db + off1);
-            const float s0 = 1.0f - f0, s1 = 1.0f - f1;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                loc[t][0][d] = s0 * T[o1][d] + f0 * T[o0][d];
-                loc[t][1][d] = s1 * T[o1][d] + f1 * T[o2][d];
-            }
-        } else {
-            rmin[t] = rmax[t] = (float)db;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) loc[t][0][d] = loc[t][1][d] = T[o1][d];
-        }
-    }
-    if ((rmax[0] > rmin[1]) && (rmin[0] < rmax[1])) {
-        const float *a, *b;
-        if (rmin[0] >= rmin[1]) {
-            a = loc[0][0];
-            b = (rmax[0] <= rmax[1]) ? loc[0][1] : loc[1][1];
-        } else if (rmax[0] <= rmax[1]) {
-            a = loc[0][1];
-            b = loc[1][0];
-        } else {
-            a = loc[1][0];
-            b = loc[1][1];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) { r1[d] = a[d]; r2[d] = b[d]; }
+        r1[0] = x.x; r1[1] = x.y; r1[2] = x.z;
+        r2[0] = y.x; r2[1] = y.y; r2[2] = y.z;
         return true;
     }
     return false;

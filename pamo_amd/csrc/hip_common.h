@@ -39,6 +39,19 @@ const char* get_error();
 
 constexpr int kWave = 64;
 
+// Kernel ids of the launch profiler (HIP events around selected launches).
+enum KernelId {
+    K_RING_COUNT, K_RING_FILL, K_RING_SORT, K_EDGE_COUNT, K_EDGE_FILL, K_QUADRIC, K_EDGE_COST, K_PROPAGATE,
+    K_SELECT_COLLAPSE, K_LINE_FLAG, K_MARK_CHANGED, K_FACE_BOX, K_MORTON, K_RADIX_HIST, K_RADIX_SCATTER,
+    K_KARRAS, K_BVH_BOXES, K_GUARD_QUERY, K_UNDO_CAND, K_COMPACT_MESH, K_SCAN_TILES, K_SCAN_ADD, K_OTHER,
+    K_COUNT
+};
+
+// Called around a launch; no-ops unless the calling thread's active context
+// profiles this kernel id.
+void prof_begin(int id, double bytes, hipStream_t st);
+void prof_end(int id, hipStream_t st);
+
 inline unsigned grid_for(long long n, int block) {
     long long g = (n + block - 1) / block;
     return (unsigned)(g < 1 ? 1 : g);

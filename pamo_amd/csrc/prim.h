@@ -85,15 +85,21 @@ inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStre
     if (n <= 0) return 0;
     long long nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 1) {
+        prof_begin(K_SCAN_TILES, 8.0 * n, st);
         hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(kScanBlock), 0, st, in, out, (int*)nullptr, n);
+        prof_end(K_SCAN_TILES, st);
         PAMO_HIP(hipGetLastError());
         return 0;
     }
     int* sums = ws;
+    prof_begin(K_SCAN_TILES, 8.0 * n, st);
     hipLaunchKernelGGL(scan_tiles_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, sums, n);
+    prof_end(K_SCAN_TILES, st);
     PAMO_HIP(hipGetLastError());
     PAMO_TRY(scan_exclusive(sums, sums, nb, ws + nb + 16, st));
+    prof_begin(K_SCAN_ADD, 8.0 * n, st);
     hipLaunchKernelGGL(scan_add_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, out, (const int*)sums, n);
+    prof_end(K_SCAN_ADD, st);
     PAMO_HIP(hipGetLastError());
     return 0;
 }
@@ -180,11 +186,15 @@ inline int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* k2, uint32
     int* sws = ws + 256LL * nb;
     uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
     for (int shift = 0; shift < bits; shift += 8) {
+        prof_begin(K_RADIX_HIST, 4.0 * n + 1024.0 * nb, st);
         hipLaunchKernelGGL(radix_hist_k, dim3(nb), dim3(kSortBlock), 0, st, ka, n, shift, counts, nb);
+        prof_end(K_RADIX_HIST, st);
         PAMO_HIP(hipGetLastError());
         PAMO_TRY(scan_exclusive(counts, counts, 256LL * nb, sws, st));
+        prof_begin(K_RADIX_SCATTER, 16.0 * n + 1024.0 * nb, st);
         hipLaunchKernelGGL(radix_scatter_k, dim3(nb), dim3(kSortBlock), 0, st, ka, va, kb, vb, n, shift,
                            (const int*)counts, nb);
+        prof_end(K_RADIX_SCATTER, st);
         PAMO_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);

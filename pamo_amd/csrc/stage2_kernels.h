@@ -1,0 +1,1062 @@
+// Device kernels of the stage-2 round (see stage2.hip for the mapping to
+// the reference).  Layout in HBM:
+//   positions   float4 (x, y, z, 0) per vertex: every consumer gathers whole
+//               vertices, so one 16-byte load replaces three 4-byte loads;
+//   faces       SoA int arrays i[], j[], k[] (coalesced per-face passes);
+//   rings       ring_off[V+1] + ring[3F] (face ids, ascending per vertex) and
+//               rv[3F] = int4 (i, j, k, face) so one-ring passes read each
+//               ring face with one 16-byte load;
+//   quadrics    3 float4 per vertex (10 symmetric entries + 2 pad).
+#pragma once
+#include "guard_ref.h"
+#include "hip_common.h"
+
+namespace pamo {
+
+constexpr int kBlock = 256;
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ F3 f3sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 f3add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ float f3dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 f3cross(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float f3norm(F3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ F3 f3divs(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+// ternary max-then-min clamp of cusimp_free.cu:29-34 (NaN -> upper bound)
+__device__ __forceinline__ float tclamp(float x, float a, float b) {
+    float m = a > x ? a : x;
+    return m < b ? m : b;
+}
+// float -> uint32 with saturation (CUDA's conversion rule, cusimp_free.cu:584)
+__device__ __forceinline__ uint32_t sat_u32(float q) {
+    if (!(q < 4294967296.0f)) return 0xFFFFFFFFu;
+    if (q <= 0.0f) return 0u;
+    return (uint32_t)q;
+}
+
+struct Mesh {
+    float4* p;
+    int *i, *j, *k;
+};
+
+__device__ __forceinline__ F3 ld(const Mesh& m, int v) { return f3(m.p[v]); }
+__device__ __forceinline__ void st(const Mesh& m, int v, F3 q) { m.p[v] = make_float4(q.x, q.y, q.z, 0.0f); }
+__device__ __forceinline__ bool face_dead(int a, int b, int c) { return a == -1 && b == -1 && c == -1; }
+
+struct Feistel {
+    uint32_t lbits, rbits, lmask, rmask;
+    uint32_t key[24];
+};
+
+enum Counter {
+    C_LINE = 0,     // line-collapse count
+    C_NREG = 1,     // regular collapse records
+    C_NLIVE = 2,    // live faces (BVH leaves)
+    C_PUSHED = 3,   // undo candidates
+    C_NV = 4,       // next-round vertices
+    C_NF = 5,       // next-round faces
+    C_NEDGE = 6,
+    C_NQUERY = 7,   // guard queries
+    C_COUNT = 16
+};
+
+// ---------------------------------------------------------------- load / shuffle (A3)
+__global__ void load_points_k(const float* __restrict__ aos, float4* __restrict__ p, float4* __restrict__ op,
+                              int* __restrict__ occ, int n) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    if (v == n) {
+        occ[n] = 0;
+        return;
+    }
+    float4 q = make_float4(aos[3 * (long long)v], aos[3 * (long long)v + 1], aos[3 * (long long)v + 2], 0.0f);
+    p[v] = q;
+    op[v] = q;
+    occ[v] = 1;
+}
+
+__global__ void load_tris_k(const int* __restrict__ aos, int* ti, int* tj, int* tk, int n) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    ti[f] = aos[3 * (long long)f];
+    tj[f] = aos[3 * (long long)f + 1];
+    tk[f] = aos[3 * (long long)f + 2];
+}
+
+__device__ __forceinline__ uint32_t feistel_apply(const Feistel& fs, uint32_t v) {
+    uint32_t L = v >> fs.rbits, R = v & fs.rmask;
+    for (int r = 0; r < 24; ++r) {
+        uint64_t prod = 0xD2B74407B1CE6E93ull * (uint64_t)L;
+        uint32_t hi = (uint32_t)(prod >> 32), lo = (uint32_t)prod;
+        lo = (lo << (fs.rbits - fs.lbits)) | (R >> fs.lbits);
+        L = ((hi ^ fs.key[r]) ^ R) & fs.lmask;
+        R = lo & fs.rmask;
+    }
+    return (L << fs.rbits) | R;
+}
+
+// Feistel permutation of [0, 2^bits); images below m are kept in order.
+__global__ void shuffle_flags_k(Feistel fs, uint32_t domain, uint32_t m, int* __restrict__ flag,
+                                uint32_t* __restrict__ img) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx > domain) return;
+    if (idx == domain) {
+        flag[idx] = 0;
+        return;
+    }
+    uint32_t k = feistel_apply(fs, idx);
+    img[idx] = k;
+    flag[idx] = k < m ? 1 : 0;
+}
+
+__global__ void shuffle_scatter_k(uint32_t domain, uint32_t m, const int* __restrict__ pos,
+                                  const uint32_t* __restrict__ img, const int* __restrict__ si,
+                                  const int* __restrict__ sj, const int* __restrict__ sk, int* di, int* dj, int* dk) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= domain) return;
+    uint32_t k = img[idx];
+    if (k >= m) return;
+    int q = pos[idx];
+    di[q] = si[k];
+    dj[q] = sj[k];
+    dk[q] = sk[k];
+}
+
+// ---------------------------------------------------------------- undo-list prologue
+__global__ void stuck_prologue_k(const int* __restrict__ undo_in, int n, int* tmp_undo, int* undo_list,
+                                 const int* __restrict__ pts_map) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = undo_in[i];
+    tmp_undo[i] = v;
+    undo_list[i] = pts_map[v];
+}
+
+__global__ void invalid_set_k(const int* __restrict__ undo_in, int n, uint8_t* invalid, int cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = undo_in[i];
+    if (v >= 0 && v < cap) invalid[v] = 1;
+}
+
+// ---------------------------------------------------------------- rings and edges (A4, A5)
+__global__ void ring_count_k(Mesh m, int nT, int* __restrict__ cnt) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nT) return;
+    int a = m.i[f], b = m.j[f], c = m.k[f];
+    if (face_dead(a, b, c)) return;
+    atomicAdd(&cnt[a], 1);
+    atomicAdd(&cnt[b], 1);
+    atomicAdd(&cnt[c], 1);
+}
+
+__global__ void ring_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ fill,
+                            int* __restrict__ ring) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nT) return;
+    int a = m.i[f], b = m.j[f], c = m.k[f];
+    if (face_dead(a, b, c)) return;
+    ring[off[a] + atomicAdd(&fill[a], 1)] = f;
+    ring[off[b] + atomicAdd(&fill[b], 1)] = f;
+    ring[off[c] + atomicAdd(&fill[c], 1)] = f;
+}
+
+// Canonical ring order (ascending face index), then the packed int4 entries.
+__global__ void ring_sort_k(Mesh m, const int* __restrict__ off, int* __restrict__ ring, int4* __restrict__ rv,
+                            int nP) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nP) return;
+    int b = off[v], e = off[v + 1];
+    for (int q = b + 1; q < e; ++q) {
+        int x = ring[q];
+        int r = q - 1;
+        while (r >= b && ring[r] > x) {
+            ring[r + 1] = ring[r];
+            --r;
+        }
+        ring[r + 1] = x;
+    }
+    for (int q = b; q < e; ++q) {
+        int f = ring[q];
+        rv[q] = make_int4(m.i[f], m.j[f], m.k[f], f);
+    }
+}
+
+__global__ void edge_count_k(Mesh m, int nT, int* __restrict__ cnt) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f > nT) return;
+    if (f == nT) {
+        cnt[nT] = 0;
+        return;
+    }
+    int a = m.i[f], b = m.j[f], c = m.k[f];
+    cnt[f] = face_dead(a, b, c) ? 0 : (int)(a > b) + (int)(b > c) + (int)(c > a);
+}
+
+__global__ void edge_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ eu, int* __restrict__ ev) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nT) return;
+    int a = m.i[f], b = m.j[f], c = m.k[f];
+    if (face_dead(a, b, c)) return;
+    int q = off[f];
+    if (a > b) { eu[q] = b; ev[q] = a; ++q; }
+    if (b > c) { eu[q] = c; ev[q] = b; ++q; }
+    if (c > a) { eu[q] = a; ev[q] = c; ++q; }
+}
+
+// ---------------------------------------------------------------- quadrics and costs (A6, A7)
+__global__ void __launch_bounds__(kBlock) quadric_k(Mesh m, const int* __restrict__ off,
+                                                    const int4* __restrict__ rv, int nP, float4* __restrict__ Q) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nP) return;
+    float q[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) q[c] = 0.0f;
+    const int e = off[v + 1];
+    for (int r = off[v]; r < e; ++r) {
+        const int4 t = rv[r];
+        F3 a = ld(m, t.x), b = ld(m, t.y), d = ld(m, t.z);
+        F3 n = f3cross(f3sub(b, a), f3sub(d, a));
+        n = f3divs(n, f3norm(n));
+        float w = -f3dot(n, a);
+        float p[4] = {n.x, n.y, n.z, w};
+        int c = 0;
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+            for (int jj = ii; jj < 4; ++jj) q[c++] += p[ii] * p[jj];
+    }
+    Q[3 * (long long)v] = make_float4(q[0], q[1], q[2], q[3]);
+    Q[3 * (long long)v + 1] = make_float4(q[4], q[5], q[6], q[7]);
+    Q[3 * (long long)v + 2] = make_float4(q[8], q[9], 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ int succ_in(int4 t, int x) {
+    if (t.x == x) return t.y;
+    if (t.y == x) return t.z;
+    if (t.z == x) return t.x;
+    return -2;
+}
+
+__device__ __forceinline__ float heron(F3 a, F3 b, F3 d) {
+    float x = f3norm(f3sub(a, b)), y = f3norm(f3sub(b, d)), z = f3norm(f3sub(d, a));
+    float s = (x + y + z) / 2;
+    return sqrtf(s * (s - x) * (s - y) * (s - z));
+}
+
+struct CostParams {
+    float edge_s, tres;
+    int is_stuck;
+};
+
+// compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge.
+__global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
+                                                      const int4* __restrict__ rv, const int* __restrict__ eu,
+                                                      const int* __restrict__ ev, int nE,
+                                                      const uint8_t* __restrict__ invalid,
+                                                      const float4* __restrict__ Q, CostParams cp,
+                                                      uint32_t* __restrict__ cost) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const int u = eu[e], v = ev[e];
+    if (invalid[u] && invalid[v]) {
+        cost[e] = 0xFFFFFFFFu;
+        return;
+    }
+    if (cp.is_stuck) return;
+    const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
+    // link condition: successors of u and v in their rings (:405-444)
+    int dup = 0;
+    for (int a = ub; a < ue; ++a) {
+        const int su = succ_in(rv[a], u);
+        for (int b = vb; b < ve; ++b) {
+            dup += (su == succ_in(rv[b], v)) ? 1 : 0;
+            if (dup > 2) {
+                cost[e] = 0xFFFFFFFFu;
+                return;
+            }
+        }
+    }
+    if (dup != 2) return;
+    const F3 p0 = ld(m, u), p1 = ld(m, v);
+    float elen = 0.0f;
+    int nedge = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int b0 = side ? vb : ub, b1 = side ? ve : ue;
+        for (int a = b0; a < b1; ++a) {
+            const int4 t = rv[a];
+            const F3 pi = ld(m, t.x), pj = ld(m, t.y), pk = ld(m, t.z);
+            if (t.x > t.y) { elen += f3norm(f3sub(pi, pj)); ++nedge; }
+            if (t.y > t.z) { elen += f3norm(f3sub(pj, pk)); ++nedge; }
+            if (t.z > t.x) { elen += f3norm(f3sub(pk, pi)); ++nedge; }
+        }
+    }
+    elen = elen / (float)nedge / cp.edge_s * cp.tres;
+    elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
+    const F3 vm = f3divs(f3add(p0, p1), 2.0f);
+    const float k4s3 = 4.0f * sqrtf(3.0f);
+    float qa = 0.0f;
+    int ntri = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int w = side ? v : u, other = side ? u : v;
+        const int b0 = side ? vb : ub, b1 = side ? ve : ue;
+        for (int a = b0; a < b1; ++a) {
+            const int4 t = rv[a];
+            if (t.x == other || t.y == other || t.z == other) continue;
+            const F3 o0 = ld(m, t.x), o1 = ld(m, t.y), o2 = ld(m, t.z);
+            const F3 n0 = t.x == w ? vm : o0, n1 = t.y == w ? vm : o1, n2 = t.z == w ? vm : o2;
+            F3 on = f3cross(f3sub(o1, o0), f3sub(o2, o0));
+            on = f3divs(on, f3norm(on));
+            F3 nn = f3cross(f3sub(n1, n0), f3sub(n2, n0));
+            nn = f3divs(nn, f3norm(nn));
+            if (f3dot(on, nn) < 0) {
+                cost[e] = 0xFFFFFFFFu;
+                return;
+            }
+            float l01 = f3norm(f3sub(n0, n1)), l12 = f3norm(f3sub(n1, n2)), l20 = f3norm(f3sub(n2, n0));
+            float qv = k4s3 * heron(n0, n1, n2) / (l01 * l01) + l12 * l12 + l20 * l20 + 0.0000001f;
+            qa += 1.0f - tclamp(qv, 0.0f, 1.0f);
+            ++ntri;
+        }
+    }
+    qa = 5.0f * qa;
+    // M = Q_u + Q_v; storage 0:00 1:01 2:02 3:03 4:11 5:12 6:13 7:22 8:23 9:33
+    const float4 ua = Q[3 * (long long)u], ub4 = Q[3 * (long long)u + 1], uc = Q[3 * (long long)u + 2];
+    const float4 va = Q[3 * (long long)v], vb4 = Q[3 * (long long)v + 1], vc = Q[3 * (long long)v + 2];
+    const float M0 = ua.x + va.x, M1 = ua.y + va.y, M2 = ua.z + va.z, M3 = ua.w + va.w;
+    const float M4 = ub4.x + vb4.x, M5 = ub4.y + vb4.y, M6 = ub4.z + vb4.z, M7 = ub4.w + vb4.w;
+    const float M8 = uc.x + vc.x, M9 = uc.y + vc.y;
+    const float x0 = vm.x, x1 = vm.y, x2 = vm.z, x3 = 1.0f;
+    float r0 = M0 * x0 + M1 * x1 + M2 * x2 + M3 * x3;
+    float r1 = M1 * x0 + M4 * x1 + M5 * x2 + M6 * x3;
+    float r2 = M2 * x0 + M5 * x1 + M7 * x2 + M8 * x3;
+    float r3 = M3 * x0 + M6 * x1 + M8 * x2 + M9 * x3;
+    float qc = (r0 * x0 + r1 * x1 + r2 * x2 + r3 * x3) / (cp.edge_s * cp.edge_s);
+    float tot = qc + elen + qa / (float)ntri * cp.tres;
+    cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+}
+
+// ---------------------------------------------------------------- independent set + collapse (A8, A9)
+__device__ __forceinline__ uint64_t edge_key(uint32_t c, int e) { return ((uint64_t)c << 32) | (uint32_t)e; }
+
+// propagate_edge_cost_kernel (:587-610) keeps, per face t, the minimum key
+// over edges whose endpoint one-rings contain t.  A face lies in ring(x)
+// exactly when x is one of its vertices, so tri_min[t] = min over t's three
+// vertices of vmin[x], vmin[x] = min key over edges incident to x: two
+// atomics per edge instead of one per ring face.
+__global__ void __launch_bounds__(kBlock) propagate_k(const int* __restrict__ eu, const int* __restrict__ ev, int nE,
+                                                      const uint32_t* __restrict__ cost,
+                                                      unsigned long long* __restrict__ vmin) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const unsigned long long k = edge_key(cost[e], e);
+    atomicMin(&vmin[eu[e]], k);
+    atomicMin(&vmin[ev[e]], k);
+}
+
+__device__ __forceinline__ bool tri_has(int a, int b, int c, int x) { return a == x || b == x || c == x; }
+
+// Face rewrite shared by the regular and the line collapse (:692-715, :249-273).
+__device__ __forceinline__ void rewrite_rings(Mesh m, const int* __restrict__ off, const int* __restrict__ ring,
+                                              int u, int v) {
+    for (int a = off[u]; a < off[u + 1]; ++a) {
+        int f = ring[a];
+        if (tri_has(m.i[f], m.j[f], m.k[f], v)) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
+    }
+    for (int a = off[v]; a < off[v + 1]; ++a) {
+        int f = ring[a];
+        int ii = m.i[f], jj = m.j[f], kk = m.k[f];
+        if (tri_has(ii, jj, kk, u)) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
+        else if (ii == v) m.i[f] = u;
+        else if (jj == v) m.j[f] = u;
+        else if (kk == v) m.k[f] = u;
+    }
+}
+
+// is_skinny_triangle (:617-637): area below 1e-6 * (longest edge)^2, the
+// threshold formed in FP64 as written.
+__device__ __forceinline__ bool skinny(F3 p, F3 q, F3 r) {
+    F3 pq = f3sub(q, p), pr = f3sub(r, p), qr = f3sub(r, q);
+    float lpq = sqrtf(f3dot(pq, pq)), lpr = sqrtf(f3dot(pr, pr)), lqr = sqrtf(f3dot(qr, qr));
+    float ml = fmaxf(lpq, fmaxf(lpr, lqr));
+    F3 cr = {pq.y * pr.z - pq.z * pr.y, pq.z * pr.x - pq.x * pr.z, pq.x * pr.y - pq.y * pr.x};
+    float thr = (float)(1e-6 * (double)ml * (double)ml);
+    float area = 0.5f * sqrtf(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+    return area < thr;
+}
+
+struct Orig {
+    const float4* p;
+    const int *i, *j, *k;
+};
+
+__device__ __forceinline__ void restore_edge(Mesh m, Orig o, int* occ, const int* __restrict__ off,
+                                             const int* __restrict__ ring, int u, int v) {
+    occ[v] = 1;
+    m.p[u] = o.p[u];
+    m.p[v] = o.p[v];
+    for (int side = 0; side < 2; ++side) {
+        int w = side ? v : u;
+        for (int a = off[w]; a < off[w + 1]; ++a) {
+            int f = ring[a];
+            m.i[f] = o.i[f]; m.j[f] = o.j[f]; m.k[f] = o.k[f];
+        }
+    }
+}
+
+// Selection (:639-681) fused with the collapse (:683-773).  Selected edges
+// have pairwise disjoint one-ring face sets, and no selected edge moves a
+// vertex of another selected edge's rings, so the fused pass is race free
+// and its result does not depend on execution order.
+// Selected <=> cost <= collapse_t and every face t of ring(u) u ring(v) has
+// tri_min[t] == key(e), i.e. min(vmin of t's vertices) == key(e).
+__global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int* occ, const int* __restrict__ off,
+                                                            const int* __restrict__ ring, const int4* __restrict__ rv,
+                                                            const int* __restrict__ eu, const int* __restrict__ ev,
+                                                            int nE, const uint32_t* __restrict__ cost,
+                                                            const unsigned long long* __restrict__ vmin,
+                                                            uint32_t collapse_t, int* __restrict__ sel) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > nE) return;
+    if (e == nE) {
+        sel[nE] = 0;
+        return;
+    }
+    const uint32_t c = cost[e];
+    int s = 0;
+    if (c <= collapse_t) {
+        const unsigned long long key = edge_key(c, e);
+        const int u = eu[e], v = ev[e];
+        bool ok = true;
+        for (int side = 0; side < 2 && ok; ++side) {
+            const int w = side ? v : u;
+            for (int a = off[w]; a < off[w + 1] && ok; ++a) {
+                const int4 t = rv[a];
+                unsigned long long mn = vmin[t.x];
+                unsigned long long y = vmin[t.y], z = vmin[t.z];
+                mn = y < mn ? y : mn;
+                mn = z < mn ? z : mn;
+                ok = mn == key;
+            }
+        }
+        if (ok) {
+            s = 1;
+            F3 vm = f3divs(f3add(ld(m, u), ld(m, v)), 2.0f);
+            st(m, u, vm);
+            st(m, v, F3{0.0f, 0.0f, 0.0f});
+            occ[v] = 0;
+            rewrite_rings(m, off, ring, u, v);
+            bool sk = false;
+            for (int side = 0; side < 2 && !sk; ++side) {
+                int w = side ? v : u;
+                for (int a = off[w]; a < off[w + 1]; ++a) {
+                    int f = ring[a];
+                    int ii = m.i[f];
+                    if (ii == -1) continue;
+                    if (skinny(ld(m, ii), ld(m, m.j[f]), ld(m, m.k[f]))) { sk = true; break; }
+                }
+            }
+            if (sk) restore_edge(m, o, occ, off, ring, u, v);
+        }
+    }
+    sel[e] = s;
+}
+
+__global__ void compact_k(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out,
+                          int base) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    if (flag[e]) out[base + pos[e]] = e;
+}
+
+// ---------------------------------------------------------------- cleanup (A10)
+__global__ void remove_invalid_k(Mesh m, int nT) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nT) return;
+    int a = m.i[f], b = m.j[f], c = m.k[f];
+    if (face_dead(a, b, c)) return;
+    if (a == b || b == c || c == a) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
+}
+
+__global__ void line_flag_k(Mesh m, const int* __restrict__ eu, const int* __restrict__ ev, int nE,
+                            int* __restrict__ flag) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > nE) return;
+    if (e == nE) {
+        flag[nE] = 0;
+        return;
+    }
+    F3 p = ld(m, eu[e]), q = ld(m, ev[e]);
+    flag[e] = (p.x == q.x && p.y == q.y && p.z == q.z) ? 1 : 0;
+}
+
+// Line collapses are rare; applying them in edge order in one lane keeps
+// the result deterministic where the reference races on shared faces.
+__global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
+                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ flag,
+                             const int* __restrict__ pos, int nE, int* __restrict__ recs, int* cnt) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int base = cnt[C_NREG];
+    const int nl = pos[nE];
+    cnt[C_LINE] = nl;
+    if (nl == 0) return;
+    int w = 0;
+    for (int e = 0; e < nE && w < nl; ++e) {
+        if (!flag[e]) continue;
+        int u = eu[e], v = ev[e];
+        recs[base + w++] = e;
+        st(m, v, F3{0.0f, 0.0f, 0.0f});
+        occ[v] = 0;
+        rewrite_rings(m, off, ring, u, v);
+    }
+}
+
+__global__ void store_count_k(const int* __restrict__ pos, int n, int* cnt, int slot) { cnt[slot] = pos[n]; }
+
+__global__ void mark_changed_k(const int* __restrict__ recs, const int* __restrict__ cnt,
+                               const int* __restrict__ off, const int* __restrict__ ring,
+                               const int* __restrict__ eu, const int* __restrict__ ev,
+                               uint8_t* __restrict__ changed) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cnt[C_NREG] + cnt[C_LINE]) return;
+    int e = recs[r];
+    int u = eu[e], v = ev[e];
+    for (int a = off[u]; a < off[u + 1]; ++a) changed[ring[a]] = 1;
+    for (int a = off[v]; a < off[v + 1]; ++a) changed[ring[a]] = 1;
+}
+
+// ---------------------------------------------------------------- LBVH (A11)
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// 30-bit Morton code at resolution 1024 (morton_code.cuh:19-37)
+__device__ __forceinline__ uint32_t morton3(float x, float y, float z) {
+    bool sx = x < 0.0f, sy = y < 0.0f, sz = z < 0.0f;
+    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    y = fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+    z = fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+    uint32_t xx = expand_bits((uint32_t)x), yy = expand_bits((uint32_t)y), zz = expand_bits((uint32_t)z);
+    return (sx ? 0x80000000u : 0u) | (sy ? 0x40000000u : 0u) | (sz ? 0x20000000u : 0u) | (xx * 4 + yy * 2 + zz);
+}
+
+__device__ __forceinline__ void block_minmax6(float lo[3], float hi[3], float (*red)[kBlock]) {
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; ++d) {
+                red[d][threadIdx.x] = fminf(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+                red[3 + d][threadIdx.x] = fmaxf(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+}
+
+// Face boxes + per-block partial whole box + live count.
+__global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __restrict__ fbox,
+                                                     float* __restrict__ part, int* cnt) {
+    __shared__ float red[6][kBlock];
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    bool live = false;
+    if (f < nT) {
+        int a = m.i[f];
+        if (a != -1) {
+            live = true;
+            F3 p = ld(m, a), q = ld(m, m.j[f]), r = ld(m, m.k[f]);
+            lo[0] = fminf(p.x, fminf(q.x, r.x));
+            lo[1] = fminf(p.y, fminf(q.y, r.y));
+            lo[2] = fminf(p.z, fminf(q.z, r.z));
+            hi[0] = fmaxf(p.x, fmaxf(q.x, r.x));
+            hi[1] = fmaxf(p.y, fmaxf(q.y, r.y));
+            hi[2] = fmaxf(p.z, fmaxf(q.z, r.z));
+            fbox[2 * (long long)f] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+            fbox[2 * (long long)f + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        }
+    }
+    uint64_t lm = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&cnt[C_NLIVE], (int)__popcll(lm));
+    block_minmax6(lo, hi, red);
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Per-block partial bounding box of the vertices.
+__global__ void __launch_bounds__(kBlock) vert_box_k(const float4* __restrict__ p, int nP, float* __restrict__ part) {
+    __shared__ float red[6][kBlock];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    if (v < nP) {
+        float4 q = p[v];
+        lo[0] = hi[0] = q.x;
+        lo[1] = hi[1] = q.y;
+        lo[2] = hi[2] = q.z;
+    }
+    block_minmax6(lo, hi, red);
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kBlock) whole_box_k(float* __restrict__ part, int nparts) {
+    __shared__ float red[6][kBlock];
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int q = threadIdx.x; q < nparts; q += kBlock)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fminf(lo[d], part[6 * q + d]);
+            hi[d] = fmaxf(hi[d], part[6 * q + 3 + d]);
+        }
+    block_minmax6(lo, hi, red);
+    if (threadIdx.x < 6) part[6 * nparts + threadIdx.x] = red[threadIdx.x][0];  // whole box after the partials
+}
+
+__global__ void morton_k(Mesh m, int nT, const float4* __restrict__ fbox, const float* __restrict__ whole,
+                         uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nT) return;
+    val[f] = (uint32_t)f;
+    if (m.i[f] == -1) {
+        key[f] = 0xFFFFFFFFu;  // removed faces sort behind every live code (< 2^30)
+        return;
+    }
+    float4 lo = fbox[2 * (long long)f], hi = fbox[2 * (long long)f + 1];
+    float c[3] = {(hi.x + lo.x) * 0.5f, (hi.y + lo.y) * 0.5f, (hi.z + lo.z) * 0.5f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        c[d] -= whole[d];
+        c[d] /= (whole[3 + d] - whole[d]);
+    }
+    key[f] = morton3(c[0], c[1], c[2]);
+}
+
+__device__ __forceinline__ uint64_t node_key(const uint32_t* key, const uint32_t* val, int s) {
+    return ((uint64_t)key[s] << 32) | (uint64_t)val[s];
+}
+
+__device__ __forceinline__ int delta_at(const uint32_t* key, const uint32_t* val, int n, uint64_t ki, int j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(ki ^ node_key(key, val, j));
+}
+
+// Karras 2012 radix tree over the sorted keys (code << 32 | face); also
+// records each internal node's leaf range for the box pass.
+__global__ void karras_k(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val,
+                         const int* __restrict__ cnt, int* __restrict__ left, int* __restrict__ right,
+                         int* __restrict__ parent, int2* __restrict__ range) {
+    const int n = cnt[C_NLIVE];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint64_t ki = node_key(key, val, i);
+    int first, last;
+    if (i == 0) {
+        first = 0;
+        last = n - 1;
+    } else {
+        int dl = delta_at(key, val, n, ki, i - 1), dr = delta_at(key, val, n, ki, i + 1);
+        int d = dr > dl ? 1 : -1;
+        int dmin = min(dl, dr);
+        int lmax = 2;
+        while (delta_at(key, val, n, ki, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t > 0; t >>= 1)
+            if (delta_at(key, val, n, ki, i + (l + t) * d) > dmin) l += t;
+        int j = i + l * d;
+        first = min(i, j);
+        last = max(i, j);
+    }
+    const uint64_t kf = node_key(key, val, first), kl = node_key(key, val, last);
+    int split;
+    if (kf == kl) {
+        split = (first + last) >> 1;
+    } else {
+        int dn = __clzll(kf ^ kl);
+        split = first;
+        int stride = last - first;
+        do {
+            stride = (stride + 1) >> 1;
+            int mid = split + stride;
+            if (mid < last && __clzll(kf ^ node_key(key, val, mid)) > dn) split = mid;
+        } while (stride > 1);
+    }
+    const int l = (first == split) ? split + (n - 1) : split;
+    const int r = (last == split + 1) ? split + 1 + (n - 1) : split + 1;
+    left[i] = l;
+    right[i] = r;
+    parent[l] = i;
+    parent[r] = i;
+    range[i] = make_int2(first, last);
+}
+
+// Sorted leaf boxes, and prefix / suffix boxes within aligned blocks of 256
+// leaves (LDS Hillis-Steele scans).  blk[b] receives each block's box.
+// The same kernel runs over block boxes to get superblock prefix / suffix.
+// level 0: n = live faces, boxes gathered through `gather` and also stored
+// as tree leaves (node n-1+s of dst_leaf); level 1: n = number of blocks.
+__global__ void __launch_bounds__(kBlock) box_scan_k(const float4* __restrict__ src, const uint32_t* __restrict__ gather,
+                                                     const int* __restrict__ cnt, int level,
+                                                     float4* __restrict__ dst_leaf, float4* __restrict__ pre,
+                                                     float4* __restrict__ suf, float4* __restrict__ blk) {
+    __shared__ float lo[3][kBlock], hi[3][kBlock];
+    __shared__ float lo2[3][kBlock], hi2[3][kBlock];
+    const int nl = cnt[C_NLIVE];
+    const int n = level == 0 ? nl : (nl + kBlock - 1) / kBlock;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if ((long long)blockIdx.x * kBlock >= n) return;
+    const float inf = __builtin_huge_valf();
+    float4 a = make_float4(inf, inf, inf, 0.0f), b = make_float4(-inf, -inf, -inf, 0.0f);
+    if (s < n) {
+        long long g = gather ? (long long)gather[s] : s;
+        a = src[2 * g];
+        b = src[2 * g + 1];
+        if (dst_leaf) {
+            dst_leaf[2 * ((long long)nl - 1 + s)] = a;
+            dst_leaf[2 * ((long long)nl - 1 + s) + 1] = b;
+        }
+    }
+    const int t = threadIdx.x;
+    lo[0][t] = a.x; lo[1][t] = a.y; lo[2][t] = a.z;
+    hi[0][t] = b.x; hi[1][t] = b.y; hi[2][t] = b.z;
+    lo2[0][t] = a.x; lo2[1][t] = a.y; lo2[2][t] = a.z;
+    hi2[0][t] = b.x; hi2[1][t] = b.y; hi2[2][t] = b.z;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        float pl[3], ph[3], sl[3], sh[3];
+        for (int c = 0; c < 3; ++c) {
+            pl[c] = t >= d ? fminf(lo[c][t], lo[c][t - d]) : lo[c][t];
+            ph[c] = t >= d ? fmaxf(hi[c][t], hi[c][t - d]) : hi[c][t];
+            sl[c] = t + d < kBlock ? fminf(lo2[c][t], lo2[c][t + d]) : lo2[c][t];
+            sh[c] = t + d < kBlock ? fmaxf(hi2[c][t], hi2[c][t + d]) : hi2[c][t];
+        }
+        __syncthreads();
+        for (int c = 0; c < 3; ++c) {
+            lo[c][t] = pl[c]; hi[c][t] = ph[c];
+            lo2[c][t] = sl[c]; hi2[c][t] = sh[c];
+        }
+        __syncthreads();
+    }
+    if (s < n) {
+        pre[2 * (long long)s] = make_float4(lo[0][t], lo[1][t], lo[2][t], 0.0f);
+        pre[2 * (long long)s + 1] = make_float4(hi[0][t], hi[1][t], hi[2][t], 0.0f);
+        suf[2 * (long long)s] = make_float4(lo2[0][t], lo2[1][t], lo2[2][t], 0.0f);
+        suf[2 * (long long)s + 1] = make_float4(hi2[0][t], hi2[1][t], hi2[2][t], 0.0f);
+    }
+    if (t == 0) {
+        blk[2 * (long long)blockIdx.x] = make_float4(lo2[0][0], lo2[1][0], lo2[2][0], 0.0f);
+        blk[2 * (long long)blockIdx.x + 1] = make_float4(hi2[0][0], hi2[1][0], hi2[2][0], 0.0f);
+    }
+}
+
+__device__ __forceinline__ void grow(float4& lo, float4& hi, const float4* bx, long long q) {
+    float4 a = bx[2 * q], b = bx[2 * q + 1];
+    lo.x = fminf(lo.x, a.x); lo.y = fminf(lo.y, a.y); lo.z = fminf(lo.z, a.z);
+    hi.x = fmaxf(hi.x, b.x); hi.y = fmaxf(hi.y, b.y); hi.z = fmaxf(hi.z, b.z);
+}
+
+// Box of every internal node from its leaf range [a, b]: direct loop inside
+// one 256-leaf block, otherwise suffix(a) + prefix(b) + the full blocks in
+// between (again via block-level suffix / prefix and full superblocks).
+__global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ range, const int* __restrict__ cnt,
+                                                     const float4* __restrict__ leaf, const float4* __restrict__ pre,
+                                                     const float4* __restrict__ suf, const float4* __restrict__ blk,
+                                                     const float4* __restrict__ bpre,
+                                                     const float4* __restrict__ bsuf,
+                                                     const float4* __restrict__ sblk, float4* __restrict__ box) {
+    const int n = cnt[C_NLIVE];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 r = range[i];
+    const float inf = __builtin_huge_valf();
+    float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
+    const int A = r.x >> 8, B = r.y >> 8;
+    if (A == B) {
+        for (int s = r.x; s <= r.y; ++s) grow(lo, hi, leaf, (long long)(n - 1) + s);
+    } else {
+        grow(lo, hi, suf, r.x);
+        grow(lo, hi, pre, r.y);
+        const int A1 = A + 1, B1 = B - 1;
+        if (A1 <= B1) {
+            const int SA = A1 >> 8, SB = B1 >> 8;
+            if (SA == SB) {
+                for (int q = A1; q <= B1; ++q) grow(lo, hi, blk, q);
+            } else {
+                grow(lo, hi, bsuf, A1);
+                grow(lo, hi, bpre, B1);
+                for (int q = SA + 1; q <= SB - 1; ++q) grow(lo, hi, sblk, q);
+            }
+        }
+    }
+    box[2 * (long long)i] = lo;
+    box[2 * (long long)i + 1] = hi;
+}
+
+// ---------------------------------------------------------------- guard (A12, A13)
+__device__ __forceinline__ bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    if (ahi.x < blo.x || bhi.x < alo.x) return false;
+    if (ahi.y < blo.y || bhi.y < alo.y) return false;
+    if (ahi.z < blo.z || bhi.z < alo.z) return false;
+    return true;
+}
+
+__device__ __forceinline__ void load_tri9(Mesh m, int f, float T[3][3], int idx[3]) {
+    idx[0] = m.i[f]; idx[1] = m.j[f]; idx[2] = m.k[f];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        float4 q = m.p[idx[v]];
+        T[v][0] = q.x;
+        T[v][1] = q.y;
+        T[v][2] = q.z;
+    }
+}
+
+// Changed live faces in Morton order (neighbouring lanes query neighbouring
+// regions of the tree).
+__global__ void query_flag_k(const uint32_t* __restrict__ val, const int* __restrict__ cnt,
+                             const uint8_t* __restrict__ changed, int nT, int* __restrict__ flag) {
+    const int n = cnt[C_NLIVE];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nT) return;
+    flag[s] = (s < n && changed[val[s]]) ? 1 : 0;
+}
+
+__global__ void query_list_k(const uint32_t* __restrict__ val, const int* __restrict__ flag,
+                             const int* __restrict__ pos, int nT, int* __restrict__ list, int* cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) cnt[C_NQUERY] = pos[nT];
+    if (s >= nT) return;
+    if (flag[s]) list[pos[s]] = (int)val[s];
+}
+
+// Wave-level ordering point for LDS traffic between lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Both ordered reference predicates for one candidate pair; flags both faces
+// on a hit.
+__device__ __forceinline__ void eval_pair(Mesh m, int a, int b, float epsilon, uint8_t* __restrict__ flagged) {
+    float A[3][3], B[3][3];
+    int ia[3], ib[3];
+    load_tri9(m, a, A, ia);
+    load_tri9(m, b, B, ib);
+    if (ref_guard_d(B, ib, A, ia, epsilon) || ref_guard_d(A, ia, B, ib, epsilon)) {
+        flagged[a] = 1;
+        flagged[b] = 1;
+    }
+}
+
+constexpr int kQueue = 128;  // per-wave pair queue (< 64 pending + one append of <= 64)
+
+// Candidate pairs: every queried face a against every live face b != a whose
+// box overlaps a's box (closed intervals).  A pair of queried faces is taken
+// from the larger index only, and pairs sharing two vertex indices are
+// dropped before any arithmetic (the reference rejects them after its
+// coplanarity test in every outcome).
+//
+// Each lane walks the tree for its own query without a stack (parent links);
+// lanes that reach a candidate append it to a per-wave LDS queue, and the
+// whole wave evaluates the FP64 predicates 64 pairs at a time, so the long
+// predicate code runs with every lane busy instead of one lane at a time.
+__global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
+                                                        const int* __restrict__ cnt,
+                                                        const uint8_t* __restrict__ changed,
+                                                        const float4* __restrict__ fbox,
+                                                        const float4* __restrict__ box,
+                                                        const int* __restrict__ left, const int* __restrict__ right,
+                                                        const int* __restrict__ parent,
+                                                        const uint32_t* __restrict__ val, float epsilon,
+                                                        uint8_t* __restrict__ flagged) {
+    __shared__ int qa[kBlock / kWave][kQueue], qb[kBlock / kWave][kQueue];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nq = cnt[C_NQUERY];
+    const int n = cnt[C_NLIVE];
+    if (__ballot(q < nq) == 0 || n <= 1) return;  // uniform per wave
+    bool active = q < nq;
+    int a = 0, ia0 = 0, ia1 = 0, ia2 = 0;
+    float4 qlo = make_float4(0.f, 0.f, 0.f, 0.f), qhi = qlo;
+    if (active) {
+        a = list[q];
+        ia0 = m.i[a]; ia1 = m.j[a]; ia2 = m.k[a];
+        qlo = fbox[2 * (long long)a];
+        qhi = fbox[2 * (long long)a + 1];
+    }
+    int cur = 0, prev = -1;
+    int qn = 0;  // wave-uniform queue length
+    while (true) {
+        // advance to the next accepted candidate leaf (or finish)
+        int found = -1;
+        while (active && found < 0) {
+            if (cur < 0) {
+                active = false;
+                break;
+            }
+            const int par = cur == 0 ? -1 : parent[cur];
+            int next;
+            if (prev == par) {
+                if (!overlap(qlo, qhi, box[2 * (long long)cur], box[2 * (long long)cur + 1])) {
+                    next = par;
+                } else if (cur >= n - 1) {
+                    const int b = (int)val[cur - (n - 1)];
+                    next = par;
+                    if (b != a && !(changed[b] && b < a)) {
+                        const int b0 = m.i[b], b1 = m.j[b], b2 = m.k[b];
+                        const int sh = (ia0 == b0) + (ia0 == b1) + (ia0 == b2) + (ia1 == b0) + (ia1 == b1) +
+                                       (ia1 == b2) + (ia2 == b0) + (ia2 == b1) + (ia2 == b2);
+                        if (sh != 2) found = b;
+                    }
+                } else {
+                    next = left[cur];
+                }
+            } else if (prev == left[cur]) {
+                next = right[cur];
+            } else {
+                next = par;
+            }
+            prev = cur;
+            cur = next;
+        }
+        const uint64_t fm = __ballot(found >= 0);
+        if (found >= 0) {
+            const int slot = qn + lanes_below(fm);
+            qa[w][slot] = a;
+            qb[w][slot] = found;
+        }
+        qn += (int)__popcll(fm);
+        const bool any_active = __ballot(active) != 0;
+        wave_sync();
+        while (qn >= kWave || (!any_active && qn > 0)) {
+            const int take = qn < kWave ? qn : kWave;
+            if (lane < take) eval_pair(m, qa[w][qn - take + lane], qb[w][qn - take + lane], epsilon, flagged);
+            qn -= take;
+            wave_sync();
+        }
+        if (!any_active) break;
+    }
+}
+
+// ---------------------------------------------------------------- undo (A14)
+__global__ void undo_cand_k(const int* __restrict__ recs, int n_rec, const int* __restrict__ off,
+                            const int* __restrict__ ring, const int* __restrict__ eu, const int* __restrict__ ev,
+                            const uint8_t* __restrict__ flagged, int* __restrict__ pflag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n_rec) return;
+    if (r == n_rec) {
+        pflag[n_rec] = 0;
+        return;
+    }
+    int e = recs[r];
+    int u = eu[e], v = ev[e];
+    int hit = 0;
+    for (int a = off[u]; a < off[u + 1] && !hit; ++a) hit = flagged[ring[a]];
+    for (int a = off[v]; a < off[v + 1] && !hit; ++a) hit = flagged[ring[a]];
+    pflag[r] = hit ? 1 : 0;
+}
+
+__global__ void undo_compact_k(const int* __restrict__ recs, int n_rec, const int* __restrict__ pflag,
+                               const int* __restrict__ pos, int* __restrict__ pushed, int* cnt) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) cnt[C_PUSHED] = pos[n_rec];
+    if (r >= n_rec) return;
+    if (pflag[r]) pushed[pos[r]] = recs[r];
+}
+
+__global__ void undo_apply_k(Mesh m, Orig o, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
+                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ pushed,
+                             int npushed, int* __restrict__ tmp_undo) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= npushed) return;
+    int e = pushed[r];
+    int u = eu[e], v = ev[e];
+    tmp_undo[2 * r] = u;
+    tmp_undo[2 * r + 1] = v;
+    restore_edge(m, o, occ, off, ring, u, v);
+}
+
+__global__ void rearrange_k(const int* __restrict__ tmp_undo, int n, const int* __restrict__ pts_map,
+                            int* __restrict__ undo_list, int first) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    undo_list[first + i] = pts_map[tmp_undo[i]];
+}
+
+// ---------------------------------------------------------------- outputs / compaction (A15)
+__global__ void store_points_k(const float4* __restrict__ p, int n, float* __restrict__ aos) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float4 q = p[v];
+    aos[3 * (long long)v] = q.x;
+    aos[3 * (long long)v + 1] = q.y;
+    aos[3 * (long long)v + 2] = q.z;
+}
+
+__global__ void store_tris_k(Mesh m, int n, int* __restrict__ aos) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    aos[3 * (long long)f] = m.i[f];
+    aos[3 * (long long)f + 1] = m.j[f];
+    aos[3 * (long long)f + 2] = m.k[f];
+}
+
+__global__ void live_flag_k(Mesh m, int nT, int* __restrict__ flag) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f > nT) return;
+    flag[f] = (f < nT && m.i[f] >= 0) ? 1 : 0;
+}
+
+__global__ void compact_mesh_k(Mesh m, int nP, int nT, const int* __restrict__ occ,
+                               const int* __restrict__ pts_map, const int* __restrict__ fpos,
+                               float4* __restrict__ np, int* ni, int* nj, int* nk, int* cnt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        cnt[C_NV] = pts_map[nP];
+        cnt[C_NF] = fpos[nT];
+    }
+    if (t < nP && occ[t]) np[pts_map[t]] = m.p[t];
+    if (t < nT && m.i[t] >= 0) {
+        int d = fpos[t];
+        ni[d] = pts_map[m.i[t]];
+        nj[d] = pts_map[m.j[t]];
+        nk[d] = pts_map[m.k[t]];
+    }
+}
+
+// Mean over faces of the float32 face centroid ((a + b) + c) / 3, summed in
+// FP64 per block in a fixed tree (pamo/__init__.py:60-72 centring).
+__global__ void __launch_bounds__(kBlock) centroid_part_k(const float* __restrict__ pts, const int* __restrict__ tris,
+                                                          int nF, double* __restrict__ part) {
+    __shared__ double red[3][kBlock];
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+    if (f < nF) {
+        const int i0 = tris[3 * (long long)f], i1 = tris[3 * (long long)f + 1], i2 = tris[3 * (long long)f + 2];
+        for (int d = 0; d < 3; ++d) {
+            float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
+            a[d] = (double)(s / 3.0f);
+        }
+    }
+    for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = a[d];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace pamo

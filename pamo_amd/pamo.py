@@ -73,7 +73,7 @@ class CUDSP_Free:
         """Intermediate arrays of the last round (parity tests)."""
         lib = _hip.lib()
         n = ctypes.c_longlong(0)
-        if what in (_hip.PAMO_DBG_UNDO_ITERS, _hip.PAMO_DBG_N_REGULAR):
+        if what in (_hip.PAMO_DBG_UNDO_ITERS, _hip.PAMO_DBG_N_REGULAR, _hip.PAMO_DBG_BVH_N):
             _hip.check(lib.pamo_round_debug(self.ctx.ptr, what, None, 0, ctypes.byref(n)), "pamo_round_debug")
             return n.value
         if cap is None:

@@ -129,3 +129,37 @@ def test_fused_loop_reuses_context(torch_cuda, oracle_lib):
         vo, fo, ho = oracle_lib.run(Vc, F, 0.05, min_verts=0, ctx=octx)
         assert np.array_equal(hg, ho)
         assert np.array_equal(fg.cpu().numpy(), fo)
+
+
+def test_lbvh_bit_exact(torch_cuda, oracle_lib):
+    """Morton codes, stable sort order and Karras children of the LBVH built
+    by the last guard of a round equal the oracle's; every internal box is
+    exactly the union of its children's boxes."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import CUDSP_Free
+    torch = torch_cuda
+    V, F = synth.ellipsoid_box_union(1, 48)
+    Vc = centred(V, F)
+    print(f"case: LBVH after one round, {len(F)} faces", flush=True)
+    op = CUDSP_Free(0)
+    out = op.forward(torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(),
+                     torch.zeros(0, dtype=torch.int32, device="cuda"), 0, float((Vc.max(0) - Vc.min(0)).max()),
+                     1e-3, False, True)
+    pts, tris = out[0].cpu().numpy(), out[1].cpu().numpy()
+    n = op.debug(_hip.PAMO_DBG_BVH_N)
+    codes_o, faces_o, left_o, right_o = oracle_lib.lbvh(pts, tris)
+    assert n == len(codes_o)
+    codes = op.debug(_hip.PAMO_DBG_BVH_CODES).cpu().numpy().view(np.uint32)
+    faces = op.debug(_hip.PAMO_DBG_BVH_FACES).cpu().numpy()
+    assert np.array_equal(codes, codes_o)
+    assert np.array_equal(faces, faces_o)
+    assert np.array_equal(op.debug(_hip.PAMO_DBG_BVH_LEFT).cpu().numpy(), left_o)
+    assert np.array_equal(op.debug(_hip.PAMO_DBG_BVH_RIGHT).cpu().numpy(), right_o)
+    boxes = op.debug(_hip.PAMO_DBG_BVH_BOXES, dtype=torch.float32).cpu().numpy().reshape(2 * n - 1, 8)
+    lo, hi = boxes[:, 0:3], boxes[:, 4:7]
+    # leaves: exact face boxes
+    T = pts[tris[faces]]
+    assert np.array_equal(lo[n - 1:], T.min(1)) and np.array_equal(hi[n - 1:], T.max(1))
+    # internal: exact union of children
+    assert np.array_equal(lo[: n - 1], np.minimum(lo[left_o], lo[right_o]))
+    assert np.array_equal(hi[: n - 1], np.maximum(hi[left_o], hi[right_o]))
