@@ -33,6 +33,10 @@ def lib():
         L.orc_ctx_destroy.argtypes = [c_vp]
         L.orc_num_threads.restype = c_int
         L.orc_shuffle_perm.argtypes = [c_ll, c_vp]
+        L.orc_minstd.restype = ctypes.c_uint
+        L.orc_minstd.argtypes = [c_ll]
+        L.orc_morton3.restype = ctypes.c_uint
+        L.orc_morton3.argtypes = [c_float, c_float, c_float]
         L.orc_feistel.restype = ctypes.c_ulonglong
         L.orc_feistel.argtypes = [c_ll, ctypes.c_ulonglong]
         L.orc_forward.restype = c_int
@@ -76,6 +80,14 @@ def shuffle_perm(m: int) -> np.ndarray:
     out = np.empty(m, dtype=np.int64)
     lib().orc_shuffle_perm(m, out.ctypes.data)
     return out
+
+
+def minstd(n: int) -> int:
+    return int(lib().orc_minstd(n))
+
+
+def morton3(x: float, y: float, z: float) -> int:
+    return int(lib().orc_morton3(x, y, z))
 
 
 def feistel(m: int, idx: int) -> int:

@@ -969,6 +969,15 @@ void orc_shuffle_perm(long long m, long long* perm) {
     for (long long i = 0; i < m; ++i) perm[i] = p[i];
 }
 
+// n-th output (1-based) of minstd_rand seeded with 1 (the Feistel key stream).
+unsigned int orc_minstd(long long n) {
+    unsigned long long s = 1;
+    for (long long i = 0; i < n; ++i) s = (s * 48271ull) % 2147483647ull;
+    return (unsigned int)s;
+}
+
+unsigned int orc_morton3(float x, float y, float z) { return orc::morton3(x, y, z); }
+
 unsigned long long orc_feistel(long long m, unsigned long long idx) {
     orc::Feistel f((uint64_t)m);
     return f(idx);

