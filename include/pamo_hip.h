@@ -80,6 +80,104 @@ int pamo_prof_count(void);
 int pamo_prof_get(pamo_ctx* ctx, int kernel, const char** name, long long* launches, double* total_ms,
                   double* bytes);
 
+/* ---------------------------------------------------------------- stage 3
+ * Barrier-energy safe projection (reference
+ * simp_cuda/safe_project/src/pamo_safe_project/system.py:20-470).  All
+ * positions live in the normalised frame x' = x * scale + t, with scale and t
+ * computed by the caller as processing.py:15-52 does. */
+typedef struct pamo_s3 pamo_s3;
+
+/* Energy terms, Stage3Config.energy_calcs (reference config.py:29-41). */
+#define PAMO_S3_E_M2GT 1       /* Mesh2GTDistanceBvhEnergyCalculator (energy.py:346-468) */
+#define PAMO_S3_E_ELASTIC 2    /* ElasticEnergyCalculator (energy.py:470-588)            */
+#define PAMO_S3_E_HINGE 4      /* HingeEnergyCalculator (energy.py:590-706)              */
+#define PAMO_S3_E_COLLISION 8  /* CollisionBvhEnergyCalculator (energy.py:708-948)       */
+#define PAMO_S3_E_DENSE 16     /* DummyEnergyCalculator (energy.py:62-111), tests only   */
+#define PAMO_S3_E_DEFAULT 15
+
+/* Fields of Stage3Config (reference config.py:6-80) that the path reads. */
+typedef struct pamo_s3_params {
+    int energy_mask;         /* PAMO_S3_E_* bits; PAMO_S3_E_DEFAULT */
+    int n_newton_iters;      /* 10  */
+    int n_cg_iters;          /* 40  */
+    int n_line_search_iters; /* 10  */
+    int max_blocks;          /* 1 << 25: contact-pair cap before the radius halves */
+    int ccd_max_iters;       /* 100 */
+    double d_hat;            /* 1e-3 */
+    double coll_stiffness;   /* 1e2  */
+    double contact_detection_radius; /* 1e-2 */
+    double ee_classify_thres;        /* 1e-3 */
+    double ccd_slackness;            /* 0.7  */
+    double ccd_thickness;            /* 1e-6 */
+    double elas_young_modulus;       /* 1e-1 */
+    double elas_poisson_ratio;       /* 0.0  */
+    double hinge_stiffness;          /* 1e-2 */
+    double mesh2gt_dist_stiffness;   /* 1e3  */
+} pamo_s3_params;
+
+/* Replaces Stage3System.__init__ (system.py:20-70). */
+int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* params);
+int pamo_s3_destroy(pamo_s3* s3);
+
+/* Input ("ground truth") surface: device float32 (n_gt_v,3), int32
+ * (n_gt_f,3), normalised in FP64 with scale / t3; builds its LBVH.
+ * Replaces the input-surface half of Stage3System.register_mesh
+ * (system.py:175-205,266-269: GT arrays and the wp.Mesh over them). */
+int pamo_s3_set_target(pamo_s3* s3, const float* d_gt_v, int n_gt_v, const int* d_gt_f, int n_gt_f, double scale,
+                       const double* t3);
+
+/* Simplified mesh to project: device float32 (n_v,3) / int32 (n_f,3); host
+ * int32 unique edges (n_e,2), hinges (n_h,4: opposite, edge a, edge b,
+ * opposite) and per-vertex Voronoi areas of the rest shape (n_v floats).
+ * Rest-shape terms (membrane Dm^-1 and areas, hinge rest angles and edge
+ * lengths) are computed on the device.  Counts of terms that the energy mask
+ * leaves out may be 0.  Replaces the rest of Stage3System.register_mesh
+ * (system.py:167-272). */
+int pamo_s3_set_mesh(pamo_s3* s3, const float* d_v, int n_v, const int* d_f, int n_f, double scale, const double* t3,
+                     const int* h_edges, int n_e, const int* h_hinges, int n_h, const float* h_voronoi,
+                     int do_nothing);
+
+/* n_steps calls of Stage3System.step (system.py:411-470): closest-point
+ * targets, then n_newton_iters of {contacts, gradient, Jacobi PCG, clamp,
+ * ACCD, halving line search}. */
+int pamo_s3_step(pamo_s3* s3, int n_steps);
+
+/* Current positions mapped back to the input frame, (x' - t) / scale, into a
+ * device float32 (n_v,3) buffer (processing.py:96-104). */
+int pamo_s3_get_vertices(pamo_s3* s3, float* d_out, double scale, const double* t3);
+
+/* Current normalised positions as host float64 (n_v,3) (tests). */
+int pamo_s3_get_normalized(pamo_s3* s3, double* h_out);
+
+/* Dense quadratic energy 0.5 x^T A x + b^T x + c over the 3 * n_v
+ * coordinates (A row-major FP64, 3n_v x 3n_v, n_v <= 1024): the reference's
+ * DummyEnergyCalculator (energy.py:62-111), used by its CG test
+ * (simp_cuda/safe_project/tests/test_dummy_energy.py). */
+int pamo_s3_set_dense(pamo_s3* s3, const double* h_A, const double* h_b, double c, int n_v);
+
+/* Test access to single parts of a step (no counterpart in the reference;
+ * each mirrors one Stage3System method).  h_in / h_out are host FP64. */
+#define PAMO_S3_DBG_SET_Q 1    /* in: q (n_v,3) normalised positions                  */
+#define PAMO_S3_DBG_TARGET 2   /* _update_vertex_target; out: target (n_v,3)          */
+#define PAMO_S3_DBG_CONTACTS 3 /* _detect_contact; out: (n_c,5) i0 i1 i2 i3 type      */
+#define PAMO_S3_DBG_ENERGY 4   /* detect + _compute_energy; out: 6 = per-term (M2GT,
+                                  elastic, hinge, collision, dense) and the total      */
+#define PAMO_S3_DBG_DIFF 5     /* detect + _compute_diff; out: grad (n_v,3), diag (n_v,3) */
+#define PAMO_S3_DBG_HVP 6      /* detect + diff + _compute_hess_dx; in: dx (n_v,3);
+                                  out: H dx (n_v,3)                                    */
+#define PAMO_S3_DBG_SOLVE 7    /* detect + diff + CGSolver.solve; out: p (n_v,3)     */
+#define PAMO_S3_DBG_CCD 8      /* detect + _clamp_p + _ccd on in: p (n_v,3);
+                                  out: clamped p (n_v,3) then ccd_step                 */
+#define PAMO_S3_DBG_HINGE 9    /* after DIFF: projected hinge blocks (n_h,144)        */
+int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, double* h_out, long long cap,
+                  long long* n_out);
+
+/* Statistics (no counterpart in the reference): [0] last total energy,
+ * [1] contact pairs summed over Newton iterations, [2] Newton iterations,
+ * [3] CG iterations, [4] line-search evaluations, [5] current detection
+ * radius, [6] do_nothing flag, [7] reserved. */
+int pamo_s3_stats(pamo_s3* s3, double* out8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,35 @@ PAMO_DBG_BVH_LEFT = 12
 PAMO_DBG_BVH_RIGHT = 13
 PAMO_DBG_BVH_BOXES = 14
 
+PAMO_S3_E_M2GT = 1
+PAMO_S3_E_ELASTIC = 2
+PAMO_S3_E_HINGE = 4
+PAMO_S3_E_COLLISION = 8
+PAMO_S3_E_DENSE = 16
+PAMO_S3_E_DEFAULT = 15
+
+PAMO_S3_DBG_SET_Q = 1
+PAMO_S3_DBG_TARGET = 2
+PAMO_S3_DBG_CONTACTS = 3
+PAMO_S3_DBG_ENERGY = 4
+PAMO_S3_DBG_DIFF = 5
+PAMO_S3_DBG_HVP = 6
+PAMO_S3_DBG_SOLVE = 7
+PAMO_S3_DBG_CCD = 8
+PAMO_S3_DBG_HINGE = 9
+
+
+class S3Params(ctypes.Structure):
+    """pamo_s3_params (include/pamo_hip.h)."""
+    _fields_ = [
+        ("energy_mask", c_int), ("n_newton_iters", c_int), ("n_cg_iters", c_int), ("n_line_search_iters", c_int),
+        ("max_blocks", c_int), ("ccd_max_iters", c_int), ("d_hat", c_double), ("coll_stiffness", c_double),
+        ("contact_detection_radius", c_double), ("ee_classify_thres", c_double), ("ccd_slackness", c_double),
+        ("ccd_thickness", c_double), ("elas_young_modulus", c_double), ("elas_poisson_ratio", c_double),
+        ("hinge_stiffness", c_double), ("mesh2gt_dist_stiffness", c_double),
+    ]
+
+
 # (name, restype, argtypes) for every symbol declared in include/pamo_hip.h
 SIGNATURES = [
     ("pamo_ctx_create", c_int, [ctypes.POINTER(c_vp), c_int]),
@@ -44,6 +73,17 @@ SIGNATURES = [
     ("pamo_prof_count", c_int, []),
     ("pamo_prof_get", c_int, [c_vp, c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_ll),
                               ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    ("pamo_s3_create", c_int, [ctypes.POINTER(c_vp), c_int, ctypes.POINTER(S3Params)]),
+    ("pamo_s3_destroy", c_int, [c_vp]),
+    ("pamo_s3_set_target", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double)]),
+    ("pamo_s3_set_mesh", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double), c_vp, c_int,
+                                 c_vp, c_int, c_vp, c_int]),
+    ("pamo_s3_step", c_int, [c_vp, c_int]),
+    ("pamo_s3_get_vertices", c_int, [c_vp, c_vp, c_double, ctypes.POINTER(c_double)]),
+    ("pamo_s3_get_normalized", c_int, [c_vp, c_vp]),
+    ("pamo_s3_set_dense", c_int, [c_vp, c_vp, c_vp, c_double, c_int]),
+    ("pamo_s3_debug", c_int, [c_vp, c_int, c_vp, c_ll, c_vp, c_ll, ctypes.POINTER(c_ll)]),
+    ("pamo_s3_stats", c_int, [c_vp, c_vp]),
 ]
 
 _lib = None

@@ -11,6 +11,7 @@ namespace pamo {
 // Records the first failing HIP call in a thread-local string and returns a
 // nonzero status from the enclosing function.
 void set_error(const std::string& msg);
+void clear_error();
 const char* get_error();
 
 #define PAMO_HIP(call)                                                                        \

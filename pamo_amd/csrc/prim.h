@@ -8,6 +8,9 @@
 #include "hip_common.h"
 
 namespace pamo {
+// Internal linkage: every translation unit that includes this header gets
+// its own copy of the kernels.
+namespace {
 
 constexpr int kScanBlock = 256;
 constexpr int kScanIpt = 8;
@@ -220,4 +223,5 @@ inline int fill(T* p, T v, long long n, hipStream_t st) {
     return 0;
 }
 
+}  // namespace
 }  // namespace pamo

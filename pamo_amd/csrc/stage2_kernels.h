@@ -12,6 +12,9 @@
 #include "hip_common.h"
 
 namespace pamo {
+// Internal linkage: every translation unit that includes this header gets
+// its own copy of the kernels.
+namespace {
 
 constexpr int kBlock = 256;
 
@@ -1059,4 +1062,5 @@ __global__ void __launch_bounds__(kBlock) centroid_part_k(const float* __restric
     if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+}  // namespace
 }  // namespace pamo

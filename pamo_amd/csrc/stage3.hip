@@ -1,0 +1,1601 @@
+// Stage 3 of PaMO on gfx950: barrier-energy safe projection of the
+// simplified mesh back onto the input surface, behind the C ABI of
+// include/pamo_hip.h (pamo_s3_*).
+//
+// Reference behaviour followed (simp_cuda/safe_project/src/pamo_safe_project/):
+//   process / normalisation ... processing.py:15-108
+//   register_mesh ............. system.py:167-272
+//   step (10 Newton) .......... system.py:411-470
+//   closest point on GT ....... kernels/geometry_kernels.py:82-94
+//   contact detection ......... energy.py:896-948, energy_kernels/contact_detection.py:119-216
+//   energies .................. energy_kernels/{distance,elastic,hinge,collision}_energy.py
+//   SPD projection ............ kernels/utils_kernels.py:21-39
+//   Jacobi PCG ................ cg_solver.py:96-158, kernels/solver_kernels.py:4-58
+//   clamp / ACCD / line search  solver_kernels.py:61-94, ccd_kernels.py:8-110, system.py:354-409
+//
+// State is FP32 (as the reference); the total energy and every CG dot
+// product are reduced in FP64 in a fixed two-level order (per-block partial
+// sums, then one block), so energies, step sizes and line-search decisions
+// do not depend on scheduling.  Gradient / Hessian-vector scatters use FP32
+// atomics like the reference.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pamo_hip.h"
+#include "prim.h"
+#include "stage2_kernels.h"
+#include "stage3_math.h"
+
+namespace pamo {
+namespace s3 {
+
+constexpr int kB = 256;
+
+// ------------------------------------------------------------ generic LBVH over boxes
+struct Tree {
+    int n = 0, cap = 0;
+    float4* item = nullptr;  // 2 per item (lo, hi)
+    uint32_t *key = nullptr, *val = nullptr, *key2 = nullptr, *val2 = nullptr;
+    int *left = nullptr, *right = nullptr, *parent = nullptr;
+    int2* range = nullptr;
+    float4 *box = nullptr, *pre = nullptr, *suf = nullptr, *blk = nullptr, *blkpre = nullptr, *blksuf = nullptr,
+           *sblk = nullptr;
+    float* part = nullptr;
+    int* cnt = nullptr;  // [C_COUNT]; cnt[C_NLIVE] = n
+};
+
+template <typename T>
+static int dalloc(T** q, size_t n) {
+    if (*q) PAMO_HIP(hipFree(*q));
+    *q = nullptr;
+    PAMO_HIP(hipMalloc((void**)q, sizeof(T) * std::max<size_t>(n, 1)));
+    PAMO_HIP(hipMemset(*q, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    return 0;
+}
+
+static int tree_alloc(Tree& t, int n) {
+    if (n <= t.cap) return 0;
+    size_t a = (size_t)n + n / 4 + 16;
+    t.cap = (int)a;
+    PAMO_TRY(dalloc(&t.item, 2 * a));
+    PAMO_TRY(dalloc(&t.key, a)); PAMO_TRY(dalloc(&t.val, a));
+    PAMO_TRY(dalloc(&t.key2, a)); PAMO_TRY(dalloc(&t.val2, a));
+    PAMO_TRY(dalloc(&t.left, a)); PAMO_TRY(dalloc(&t.right, a)); PAMO_TRY(dalloc(&t.parent, 2 * a));
+    PAMO_TRY(dalloc(&t.range, a));
+    PAMO_TRY(dalloc(&t.box, 4 * a));
+    PAMO_TRY(dalloc(&t.pre, 2 * a)); PAMO_TRY(dalloc(&t.suf, 2 * a));
+    size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
+    PAMO_TRY(dalloc(&t.blk, 2 * nb)); PAMO_TRY(dalloc(&t.blkpre, 2 * nb)); PAMO_TRY(dalloc(&t.blksuf, 2 * nb));
+    PAMO_TRY(dalloc(&t.sblk, 2 * ns));
+    PAMO_TRY(dalloc(&t.part, 6 * (a / kBlock + 4)));
+    if (!t.cnt) PAMO_TRY(dalloc(&t.cnt, C_COUNT));
+    return 0;
+}
+
+static void tree_free(Tree& t) {
+    void* ptrs[] = {t.item, t.key, t.val, t.key2, t.val2, t.left, t.right, t.parent, t.range, t.box,
+                    t.pre,  t.suf, t.blk, t.blkpre, t.blksuf, t.sblk, t.part, t.cnt};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    t = Tree{};
+}
+
+__global__ void items_box_part_k(const float4* __restrict__ item, int n, float* __restrict__ part) {
+    __shared__ float red[6][kBlock];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    if (i < n) {
+        float4 a = item[2 * (long long)i], b = item[2 * (long long)i + 1];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+    }
+    block_minmax6(lo, hi, red);
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void items_morton_k(const float4* __restrict__ item, int n, const float* __restrict__ whole,
+                               uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 lo = item[2 * (long long)i], hi = item[2 * (long long)i + 1];
+    float c[3] = {(hi.x + lo.x) * 0.5f, (hi.y + lo.y) * 0.5f, (hi.z + lo.z) * 0.5f};
+    for (int d = 0; d < 3; ++d) {
+        float ext = whole[3 + d] - whole[d];
+        c[d] = ext > 0.0f ? (c[d] - whole[d]) / ext : 0.0f;
+    }
+    key[i] = morton3(c[0], c[1], c[2]);
+    val[i] = (uint32_t)i;
+}
+
+// Builds the tree over t.item[0..n) (boxes already written).
+static int tree_build(Tree& t, int n, int* ws, hipStream_t st) {
+    t.n = n;
+    PAMO_TRY(fill(t.cnt + C_NLIVE, n, 1, st));
+    if (n <= 0) return 0;
+    const unsigned nb = grid_for(n, kBlock), nb1 = grid_for(nb, kBlock);
+    hipLaunchKernelGGL(items_box_part_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, n, t.part);
+    hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, st, t.part, (int)nb);
+    hipLaunchKernelGGL(items_morton_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, n,
+                       (const float*)(t.part + 6 * nb), t.key, t.val);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(radix_sort_pairs(t.key, t.val, t.key2, t.val2, n, 32, ws, st));
+    hipLaunchKernelGGL(karras_k, dim3(nb), dim3(kBlock), 0, st, (const uint32_t*)t.key, (const uint32_t*)t.val,
+                       (const int*)t.cnt, t.left, t.right, t.parent, t.range);
+    hipLaunchKernelGGL(box_scan_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, (const uint32_t*)t.val,
+                       (const int*)t.cnt, 0, t.box, t.pre, t.suf, t.blk);
+    hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, st, (const float4*)t.blk, (const uint32_t*)nullptr,
+                       (const int*)t.cnt, 1, (float4*)nullptr, t.blkpre, t.blksuf, t.sblk);
+    hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
+                       (const float4*)t.box, (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blk,
+                       (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+struct TreeView {
+    int n;
+    const float4* box;
+    const int *left, *right, *parent;
+    const uint32_t* val;
+};
+static TreeView view(const Tree& t) { return TreeView{t.n, t.box, t.left, t.right, t.parent, t.val}; }
+
+__device__ __forceinline__ bool box_hit(V3 lo, V3 hi, float4 blo, float4 bhi) {
+    return !(hi.x < blo.x || bhi.x < lo.x || hi.y < blo.y || bhi.y < lo.y || hi.z < blo.z || bhi.z < lo.z);
+}
+
+// Stackless walk of every leaf whose box overlaps [lo, hi]; f(item) per hit.
+template <typename F>
+__device__ __forceinline__ void tree_query(const TreeView& T, V3 lo, V3 hi, F f) {
+    const int n = T.n;
+    if (n <= 0) return;
+    if (n == 1) {
+        if (box_hit(lo, hi, T.box[0], T.box[1])) f((int)T.val[0]);
+        return;
+    }
+    int cur = 0, prev = -1;
+    while (cur >= 0) {
+        const int par = cur == 0 ? -1 : T.parent[cur];
+        int next;
+        if (prev == par) {
+            if (!box_hit(lo, hi, T.box[2 * (long long)cur], T.box[2 * (long long)cur + 1])) next = par;
+            else if (cur >= n - 1) {
+                f((int)T.val[cur - (n - 1)]);
+                next = par;
+            } else next = T.left[cur];
+        } else if (prev == T.left[cur]) next = T.right[cur];
+        else next = par;
+        prev = cur;
+        cur = next;
+    }
+}
+
+__device__ __forceinline__ float box_dist2(V3 p, float4 lo, float4 hi) {
+    float dx = fmaxf(fmaxf(lo.x - p.x, 0.0f), p.x - hi.x);
+    float dy = fmaxf(fmaxf(lo.y - p.y, 0.0f), p.y - hi.y);
+    float dz = fmaxf(fmaxf(lo.z - p.z, 0.0f), p.z - hi.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// ------------------------------------------------------------ parameters / context
+struct Params {
+    int n_newton, n_cg, n_ls, max_blocks, ccd_max_iters, mask;
+    float d_hat, kappa, radius0, ee_thres, ccd_slack, ccd_thick, mu, la, k_hinge, k_m2gt;
+};
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    Params P{};
+    int nV = 0, nF = 0, nE = 0, nH = 0, nGV = 0, nGF = 0;
+    int capV = 0, capF = 0, capE = 0, capH = 0, capG = 0;
+    bool do_nothing = false;
+    float radius = 0.0f;
+    // vertices (float4)
+    float4 *q = nullptr, *qn = nullptr, *qd = nullptr, *qrest = nullptr, *p = nullptr, *target = nullptr;
+    float4 *grad = nullptr, *diag = nullptr, *r = nullptr, *v = nullptr, *Av = nullptr, *z = nullptr;
+    float* vor = nullptr;
+    // topology
+    int4* tri = nullptr;
+    int2* edge = nullptr;
+    int4* hinge = nullptr;
+    float *area = nullptr, *rest_angle = nullptr, *rest_elen = nullptr;
+    float4* inv_dm = nullptr;
+    float* hblock = nullptr;  // 144 per hinge (4x4 blocks of 3x3, row-major)
+    // GT
+    float4* gt_v = nullptr;
+    int4* gt_f = nullptr;
+    Tree gt_tree, tri_tree, edge_tree;
+    // contacts
+    int capC = 0;
+    int4* cidx = nullptr;
+    int *ctype1 = nullptr, *ctype2 = nullptr;
+    float* cd = nullptr;
+    float4* cdd = nullptr;  // 4 per contact
+    int* ccount = nullptr;
+    int* hcount = nullptr;
+    // scalars (device): 0 energy, 1 energy_prev, 2 vAv, 3 zr, 4 zr_last
+    double* sc = nullptr;
+    double* hsc = nullptr;
+    float* ccd_step = nullptr;
+    double* part = nullptr;
+    int cap_part = 0;
+    // dense quadratic test energy
+    double *dense_A = nullptr, *dense_b = nullptr;
+    double dense_c = 0.0;
+    int* ws = nullptr;
+    long long cap_ws = 0;
+    // statistics
+    long long stat_contacts = 0, stat_newton = 0, stat_cg = 0, stat_ls = 0;
+    double last_energy = 0.0;
+};
+
+enum { SC_E = 0, SC_EPREV = 1, SC_VAV = 2, SC_ZR = 3, SC_ZRLAST = 4, SC_N = 8 };
+
+static void ctx_free(Ctx& c) {
+    void* ptrs[] = {c.q, c.qn, c.qd, c.qrest, c.p, c.target, c.grad, c.diag, c.r, c.v, c.Av, c.z, c.vor, c.tri,
+                    c.edge, c.hinge, c.area, c.rest_angle, c.rest_elen, c.inv_dm, c.hblock, c.gt_v, c.gt_f,
+                    c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
+                    c.dense_A, c.dense_b};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c.hcount) (void)hipHostFree(c.hcount);
+    if (c.hsc) (void)hipHostFree(c.hsc);
+    tree_free(c.gt_tree);
+    tree_free(c.tri_tree);
+    tree_free(c.edge_tree);
+}
+
+static int ensure_ws(Ctx& c, long long n) {
+    long long need = std::max(radix_workspace(n), scan_workspace(n + 1)) + 256;
+    if (need > c.cap_ws) {
+        c.cap_ws = need + need / 4;
+        PAMO_TRY(dalloc(&c.ws, (size_t)c.cap_ws));
+    }
+    return 0;
+}
+
+static int ensure_part(Ctx& c, long long items) {
+    int need = (int)(items / kB + 8);
+    if (need > c.cap_part) {
+        c.cap_part = need + need / 4;
+        PAMO_TRY(dalloc(&c.part, (size_t)c.cap_part));
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ deterministic FP64 reductions
+__device__ __forceinline__ double block_sum(double x, double* red) {
+    red[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// out[slot] = sum of part[0..n) in a fixed order (one block).
+__global__ void __launch_bounds__(kB) finish_sum_k(const double* __restrict__ part, int n, double* out, int slot) {
+    __shared__ double red[kB];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kB) acc += part[i];
+    double s = block_sum(acc, red);
+    if (threadIdx.x == 0) out[slot] = s;
+}
+
+__global__ void dot_part_k(const float4* __restrict__ a, const float4* __restrict__ b, int n,
+                           double* __restrict__ part) {
+    __shared__ double red[kB];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.0;
+    if (i < n) {
+        float4 u = a[i], w = b[i];
+        x = (double)u.x * (double)w.x + (double)u.y * (double)w.y + (double)u.z * (double)w.z;
+    }
+    double s = block_sum(x, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ------------------------------------------------------------ preprocessing kernels
+// elastic_preprocess_kernel (elastic_energy.py:8-38)
+__global__ void elastic_pre_k(const float4* __restrict__ x, const int4* __restrict__ tri, int nF, float* area,
+                              float4* inv_dm) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nF) return;
+    int4 f = tri[t];
+    V3 x0 = v3(x[f.x]), x1 = v3(x[f.y]), x2 = v3(x[f.z]);
+    V3 x01 = x1 - x0, x02 = x2 - x0;
+    float W = 0.5f * len(cross(x01, x02));
+    V3 b1 = normalize(x01);
+    V3 b2 = normalize(x02 - dot(x02, b1) * b1);
+    float a = dot(b1, x01), b = dot(b1, x02), cc = dot(b2, x01), d = dot(b2, x02);
+    float det = a * d - b * cc;
+    area[t] = W;
+    inv_dm[t] = make_float4(d / det, -b / det, -cc / det, a / det);
+}
+
+__global__ void hinge_rest_k(const float4* __restrict__ x, const int4* __restrict__ h, int nH, float* angle,
+                             float* elen) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nH) return;
+    int4 k = h[i];
+    V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+    angle[i] = hinge_angle(x0, x1, x2, x3);
+    elen[i] = len(x1 - x2);
+}
+
+__global__ void normalize_points_k(const float* __restrict__ src, int n, double scale, double t0, double t1,
+                                   double t2, float4* __restrict__ dst, float4* __restrict__ dst2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 o = make_float4((float)((double)src[3 * (long long)i] * scale + t0),
+                           (float)((double)src[3 * (long long)i + 1] * scale + t1),
+                           (float)((double)src[3 * (long long)i + 2] * scale + t2), 0.0f);
+    dst[i] = o;
+    if (dst2) dst2[i] = o;
+}
+
+__global__ void load_faces4_k(const int* __restrict__ src, int n, int4* __restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = make_int4(src[3 * (long long)i], src[3 * (long long)i + 1], src[3 * (long long)i + 2], 0);
+}
+
+__global__ void tri_items_k(const float4* __restrict__ x, const int4* __restrict__ f, int n, float inflate,
+                            float4* __restrict__ item) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int4 t = f[i];
+    V3 a = v3(x[t.x]), b = v3(x[t.y]), c = v3(x[t.z]);
+    V3 lo = vmin(a, vmin(b, c)), hi = vmax(a, vmax(b, c));
+    item[2 * (long long)i] = make_float4(lo.x - inflate, lo.y - inflate, lo.z - inflate, 0.0f);
+    item[2 * (long long)i + 1] = make_float4(hi.x + inflate, hi.y + inflate, hi.z + inflate, 0.0f);
+}
+
+__global__ void edge_items_k(const float4* __restrict__ x, const int2* __restrict__ e, int n,
+                             float4* __restrict__ item) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int2 t = e[i];
+    V3 a = v3(x[t.x]), b = v3(x[t.y]);
+    V3 lo = vmin(a, b), hi = vmax(a, b);
+    item[2 * (long long)i] = f4(lo);
+    item[2 * (long long)i + 1] = f4(hi);
+}
+
+// ------------------------------------------------------------ closest point on the input surface (B4)
+// Nearest triangle by branch and bound over the GT tree: nearer child first,
+// subtrees whose box is no closer than the best distance so far skipped.
+__global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__ x, int nV, TreeView T,
+                                                     const float4* __restrict__ gv, const int4* __restrict__ gf,
+                                                     float4* __restrict__ target) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nV) return;
+    const V3 p = v3(x[i]);
+    float best = 1e18f;  // squared; query radius 1e9
+    V3 bestp = mk(1e9f, 1e9f, 1e9f);
+    const int n = T.n;
+    auto leaf = [&](int f) {
+        int4 t = gf[f];
+        V3 c = closest_on_tri(p, v3(gv[t.x]), v3(gv[t.y]), v3(gv[t.z]));
+        float d2 = len_sq(c - p);
+        if (d2 < best) {
+            best = d2;
+            bestp = c;
+        }
+    };
+    if (n == 1) {
+        leaf((int)T.val[0]);
+    } else if (n > 1) {
+        int cur = 0, prev = -1;
+        while (cur >= 0) {
+            const int par = cur == 0 ? -1 : T.parent[cur];
+            int next;
+            if (prev == par) {
+                if (box_dist2(p, T.box[2 * (long long)cur], T.box[2 * (long long)cur + 1]) >= best) next = par;
+                else if (cur >= n - 1) {
+                    leaf((int)T.val[cur - (n - 1)]);
+                    next = par;
+                } else {
+                    int l = T.left[cur], r = T.right[cur];
+                    float dl = box_dist2(p, T.box[2 * (long long)l], T.box[2 * (long long)l + 1]);
+                    float dr = box_dist2(p, T.box[2 * (long long)r], T.box[2 * (long long)r + 1]);
+                    next = dl <= dr ? l : r;
+                }
+            } else {
+                int l = T.left[cur], r = T.right[cur];
+                float dl = box_dist2(p, T.box[2 * (long long)l], T.box[2 * (long long)l + 1]);
+                float dr = box_dist2(p, T.box[2 * (long long)r], T.box[2 * (long long)r + 1]);
+                int first = dl <= dr ? l : r, second = dl <= dr ? r : l;
+                next = prev == first ? second : par;
+            }
+            prev = cur;
+            cur = next;
+        }
+    }
+    target[i] = f4(bestp);
+}
+
+// ------------------------------------------------------------ contact detection (B5)
+__global__ void __launch_bounds__(kB) detect_pt_k(const float4* __restrict__ x, int nV, const int4* __restrict__ tri,
+                                                  TreeView T, float d_hat, float radius, int* counter, int cap,
+                                                  int4* __restrict__ cidx, int* __restrict__ ctype1) {
+    int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 >= nV) return;
+    const V3 x0 = v3(x[i0]);
+    const float thr = 2.0f * radius + d_hat;
+    const V3 lo = x0 - mk(thr, thr, thr), hi = x0 + mk(thr, thr, thr);
+    tree_query(T, lo, hi, [&](int f) {
+        int4 t = tri[f];
+        if (i0 == t.x || i0 == t.y || i0 == t.z) return;
+        V3 x1 = v3(x[t.x]), x2 = v3(x[t.y]), x3 = v3(x[t.z]);
+        int ty = pt_classify(x0, x1, x2, x3);
+        float d = pt_distance(x0, x1, x2, x3, ty);
+        if (d <= thr) {
+            int bid = atomicAdd(counter, 1);
+            if (bid < cap) {
+                cidx[bid] = make_int4(i0, t.x, t.y, t.z);
+                ctype1[bid] = BLOCK_PT;
+            }
+        }
+    });
+}
+
+__global__ void __launch_bounds__(kB) detect_ee_k(const float4* __restrict__ x, int nE, const int2* __restrict__ edge,
+                                                  TreeView T, float d_hat, float radius, float ee_thres,
+                                                  int* counter, int cap, int4* __restrict__ cidx,
+                                                  int* __restrict__ ctype1) {
+    int e0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e0 >= nE) return;
+    const int2 a = edge[e0];
+    const V3 x0 = v3(x[a.x]), x1 = v3(x[a.y]);
+    const float thr = 2.0f * radius + d_hat;
+    const V3 lo = vmin(x0, x1) - mk(thr, thr, thr), hi = vmax(x0, x1) + mk(thr, thr, thr);
+    tree_query(T, lo, hi, [&](int e1) {
+        if (!(e0 < e1)) return;
+        int2 b = edge[e1];
+        if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
+        V3 x2 = v3(x[b.x]), x3 = v3(x[b.y]);
+        int ty = ee_classify(x0, x1, x2, x3, ee_thres);
+        float d = ee_distance(x0, x1, x2, x3, ty);
+        if (d <= thr) {
+            int bid = atomicAdd(counter, 1);
+            if (bid < cap) {
+                cidx[bid] = make_int4(a.x, a.y, b.x, b.y);
+                ctype1[bid] = BLOCK_EE;
+            }
+        }
+    });
+}
+
+// ------------------------------------------------------------ energies (B7)
+struct Topo {
+    const int4* tri;
+    const float* area;
+    const float4* inv_dm;
+    const int4* hinge;
+    const float *rest_angle, *rest_elen;
+    const float* vor;
+    const float4* target;
+    const int4* cidx;
+    const int* ctype1;
+};
+
+__device__ __forceinline__ void mat22_mul(const float A[4], const float B[4], float C[4]) {
+    C[0] = A[0] * B[0] + A[1] * B[2];
+    C[1] = A[0] * B[1] + A[1] * B[3];
+    C[2] = A[2] * B[0] + A[3] * B[2];
+    C[3] = A[2] * B[1] + A[3] * B[3];
+}
+
+struct Membrane {
+    V3 b1, b2;
+    float F[4], E[4], trE;
+};
+
+// Local 2D frame, deformation gradient and Green strain (elastic_energy.py:64-82)
+__device__ __forceinline__ Membrane membrane(V3 x0, V3 x1, V3 x2, float4 idm) {
+    Membrane m;
+    V3 x01 = x1 - x0, x02 = x2 - x0;
+    m.b1 = normalize(x01);
+    m.b2 = normalize(x02 - dot(x02, m.b1) * m.b1);
+    float Ds[4] = {dot(m.b1, x01), dot(m.b1, x02), dot(m.b2, x01), dot(m.b2, x02)};
+    float Dm[4] = {idm.x, idm.y, idm.z, idm.w};
+    mat22_mul(Ds, Dm, m.F);
+    // E = 0.5 (F^T F - I)
+    float FtF[4] = {m.F[0] * m.F[0] + m.F[2] * m.F[2], m.F[0] * m.F[1] + m.F[2] * m.F[3],
+                    m.F[1] * m.F[0] + m.F[3] * m.F[2], m.F[1] * m.F[1] + m.F[3] * m.F[3]};
+    m.E[0] = 0.5f * (FtF[0] - 1.0f);
+    m.E[1] = 0.5f * FtF[1];
+    m.E[2] = 0.5f * FtF[2];
+    m.E[3] = 0.5f * (FtF[3] - 1.0f);
+    m.trE = m.E[0] + m.E[3];
+    return m;
+}
+
+// Per-item energy: items [0, nV) M2GT, [nV, +nF) elastic, [+nH) hinge,
+// [+nC) collision (also stores the distance and its classification).
+__global__ void __launch_bounds__(kB) energy_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF,
+                                               int nH, int nC, float* __restrict__ cd, int* __restrict__ ctype2,
+                                               double* __restrict__ part) {
+    __shared__ double red[kB];
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < nV) {
+        V3 d = v3(x[i]) - v3(tp.target[i]);
+        float w = tp.vor[i] * P.k_m2gt;
+        e = (double)(0.5f * len_sq(d) * w);
+    } else if ((i -= nV) < nF) {
+        int4 f = tp.tri[i];
+        Membrane m = membrane(v3(x[f.x]), v3(x[f.y]), v3(x[f.z]), tp.inv_dm[i]);
+        // tr(E E^T) = sum of squares
+        float tEEt = m.E[0] * m.E[0] + m.E[1] * m.E[1] + m.E[2] * m.E[2] + m.E[3] * m.E[3];
+        e = (double)(tp.area[i] * (P.mu * tEEt + 0.5f * P.la * m.trE * m.trE));
+    } else if ((i -= nF) < nH) {
+        int4 k = tp.hinge[i];
+        float th = hinge_angle(v3(x[k.x]), v3(x[k.y]), v3(x[k.z]), v3(x[k.w]));
+        float dt = th - tp.rest_angle[i];
+        e = (double)(0.5f * P.k_hinge * dt * dt * tp.rest_elen[i]);
+    } else if ((i -= nH) < nC) {
+        int4 k = tp.cidx[i];
+        V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+        float d;
+        int t2;
+        if (tp.ctype1[i] == BLOCK_PT) {
+            t2 = pt_classify(x0, x1, x2, x3);
+            d = pt_distance(x0, x1, x2, x3, t2);
+        } else {
+            t2 = ee_classify(x0, x1, x2, x3, P.ee_thres);
+            d = ee_distance(x0, x1, x2, x3, t2);
+        }
+        ctype2[i] = t2;
+        cd[i] = d;
+        if (d < P.d_hat) e = (double)(barrier(d, P.d_hat) * P.kappa);
+    }
+    double s = block_sum(e, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__device__ __forceinline__ void atomic_add3(float4* a, int i, V3 v) {
+    atomicAdd(&a[i].x, v.x);
+    atomicAdd(&a[i].y, v.y);
+    atomicAdd(&a[i].z, v.z);
+}
+
+// Symmetric eigen-clamp of the 9x9 leading block (cyclic Jacobi), then rows /
+// columns of vertex 3 rebuilt by translation invariance (sum over a block
+// row / column vanishes), as block_spd_project_kernel (utils_kernels.py:21-39).
+__device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row-major 3x3 */) {
+    float A[9][9], Vm[9][9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            A[r][c] = B[9 * (4 * (r / 3) + (c / 3)) + 3 * (r % 3) + (c % 3)];
+            Vm[r][c] = r == c ? 1.0f : 0.0f;
+        }
+    // symmetrise
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int c = r + 1; c < 9; ++c) {
+            float s = 0.5f * (A[r][c] + A[c][r]);
+            A[r][c] = s;
+            A[c][r] = s;
+        }
+    for (int sweep = 0; sweep < 8; ++sweep) {
+#pragma unroll
+        for (int pp = 0; pp < 8; ++pp)
+#pragma unroll
+            for (int qq = pp + 1; qq < 9; ++qq) {
+                float apq = A[pp][qq];
+                if (fabsf(apq) > 1e-30f) {
+                    float theta = (A[qq][qq] - A[pp][pp]) / (2.0f * apq);
+                    float t = (theta >= 0.0f ? 1.0f : -1.0f) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+                    float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        float akp = A[k][pp], akq = A[k][qq];
+                        A[k][pp] = cs * akp - sn * akq;
+                        A[k][qq] = sn * akp + cs * akq;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        float apk = A[pp][k], aqk = A[qq][k];
+                        A[pp][k] = cs * apk - sn * aqk;
+                        A[qq][k] = sn * apk + cs * aqk;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        float vkp = Vm[k][pp], vkq = Vm[k][qq];
+                        Vm[k][pp] = cs * vkp - sn * vkq;
+                        Vm[k][qq] = sn * vkp + cs * vkq;
+                    }
+                }
+            }
+    }
+    float lam[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lam[k] = fmaxf(A[k][k], 0.0f);
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s += Vm[r][k] * lam[k] * Vm[c][k];
+            B[9 * (4 * (r / 3) + (c / 3)) + 3 * (r % 3) + (c % 3)] = s;
+        }
+    // row 3: block[3][j] = -(block[0][j] + block[1][j] + block[2][j]) for j < 3
+    for (int j = 0; j < 3; ++j)
+        for (int e = 0; e < 9; ++e)
+            B[9 * (4 * 3 + j) + e] = -(B[9 * (4 * 0 + j) + e] + B[9 * (4 * 1 + j) + e] + B[9 * (4 * 2 + j) + e]);
+    // column 3: block[j][3] = block[3][j]^T
+    for (int j = 0; j < 3; ++j)
+        for (int rr = 0; rr < 3; ++rr)
+            for (int cc = 0; cc < 3; ++cc) B[9 * (4 * j + 3) + 3 * rr + cc] = B[9 * (4 * 3 + j) + 3 * cc + rr];
+    // block[3][3] = -(block[0][3] + block[1][3] + block[2][3])
+    for (int e = 0; e < 9; ++e)
+        B[9 * (4 * 3 + 3) + e] = -(B[9 * (4 * 0 + 3) + e] + B[9 * (4 * 1 + 3) + e] + B[9 * (4 * 2 + 3) + e]);
+}
+
+struct M3 {
+    float m[9];
+};
+__device__ __forceinline__ M3 outer(V3 a, V3 b) {
+    return {{a.x * b.x, a.x * b.y, a.x * b.z, a.y * b.x, a.y * b.y, a.y * b.z, a.z * b.x, a.z * b.y, a.z * b.z}};
+}
+__device__ __forceinline__ M3 tr(M3 a) { return {{a.m[0], a.m[3], a.m[6], a.m[1], a.m[4], a.m[7], a.m[2], a.m[5], a.m[8]}}; }
+__device__ __forceinline__ M3 add(M3 a, M3 b) { M3 r; for (int k = 0; k < 9; ++k) r.m[k] = a.m[k] + b.m[k]; return r; }
+__device__ __forceinline__ M3 sub(M3 a, M3 b) { M3 r; for (int k = 0; k < 9; ++k) r.m[k] = a.m[k] - b.m[k]; return r; }
+__device__ __forceinline__ M3 scl(M3 a, float s) { M3 r; for (int k = 0; k < 9; ++k) r.m[k] = a.m[k] * s; return r; }
+__device__ __forceinline__ M3 sym(M3 a) { return add(a, tr(a)); }
+__device__ __forceinline__ void put(float* B, int a, int b, M3 x) { for (int k = 0; k < 9; ++k) B[9 * (4 * a + b) + k] = x.m[k]; }
+
+// Gradient (coefficient `coeff`), Jacobi diagonal and, for hinges, the
+// projected 12x12 Hessian blocks; for contacts the distance gradient.
+// Items as in energy_k.
+__global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF, int nH,
+                                             int nC, float coeff, const float* __restrict__ cd,
+                                             const int* __restrict__ ctype2, float4* __restrict__ grad,
+                                             float4* __restrict__ diag, float* __restrict__ hblock,
+                                             float4* __restrict__ cdd) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nV) {
+        float w = tp.vor[i] * P.k_m2gt;
+        V3 g = (v3(x[i]) - v3(tp.target[i])) * w;
+        atomic_add3(grad, (int)i, g * coeff);
+        atomic_add3(diag, (int)i, mk(w, w, w));
+    } else if ((i -= nV) < nF) {
+        int4 f = tp.tri[i];
+        Membrane m = membrane(v3(x[f.x]), v3(x[f.y]), v3(x[f.z]), tp.inv_dm[i]);
+        // P = F (2 mu E + la tr(E) I); H = coeff W P inv_Dm^T
+        float S[4] = {2.0f * P.mu * m.E[0] + P.la * m.trE, 2.0f * P.mu * m.E[1], 2.0f * P.mu * m.E[2],
+                      2.0f * P.mu * m.E[3] + P.la * m.trE};
+        float Pm[4];
+        mat22_mul(m.F, S, Pm);
+        float4 idm = tp.inv_dm[i];
+        float IdT[4] = {idm.x, idm.z, idm.y, idm.w};
+        float H[4];
+        mat22_mul(Pm, IdT, H);
+        float cw = coeff * tp.area[i];
+        for (int k = 0; k < 4; ++k) H[k] *= cw;
+        V3 f1 = H[0] * m.b1 + H[2] * m.b2;
+        V3 f2 = H[1] * m.b1 + H[3] * m.b2;
+        V3 f0 = -f1 - f2;
+        atomic_add3(grad, f.x, f0);
+        atomic_add3(grad, f.y, f1);
+        atomic_add3(grad, f.z, f2);
+    } else if ((i -= nF) < nH) {
+        int4 k = tp.hinge[i];
+        V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+        V3 e0 = x2 - x1, e1 = x0 - x2, e2 = x0 - x1, e1_ = x3 - x2, e2_ = x3 - x1;
+        float e0_lsq = len_sq(e0);
+        float e0_len = sqrtf(e0_lsq), e1_len = sqrtf(len_sq(e1)), e2_len = sqrtf(len_sq(e2));
+        float e1_len_ = sqrtf(len_sq(e1_)), e2_len_ = sqrtf(len_sq(e2_));
+        V3 e0u = normalize(e0), e1u = normalize(e1), e2u = normalize(e2), e1u_ = normalize(e1_), e2u_ = normalize(e2_);
+        float ca1 = dot(e2u, e0u), ca2 = -dot(e1u, e0u), ca1_ = dot(e2u_, e0u), ca2_ = -dot(e1u_, e0u);
+        float h0 = len(cross(e1, e2)) / e0_len, h1 = len(cross(e0, e2)) / e1_len, h2 = len(cross(e0, e1)) / e2_len;
+        float h0_ = len(cross(e1_, e2_)) / e0_len, h1_ = len(cross(e0, e2_)) / e1_len_,
+              h2_ = len(cross(e0, e1_)) / e2_len_;
+        V3 n = normalize(cross(e0, e2)), n_ = normalize(cross(e2_, e0));
+        float cth = dot(n, n_), sth = dot(cross(n, n_), e0) / e0_len;
+        float th = atan2f(sth, cth);
+        float dth = th - tp.rest_angle[i];
+        float dE = P.k_hinge * dth * tp.rest_elen[i];
+        float d2E = P.k_hinge * tp.rest_elen[i];
+        V3 g1 = n * (ca2 / h1) + n_ * (ca2_ / h1_);
+        V3 g2 = n * (ca1 / h2) + n_ * (ca1_ / h2_);
+        V3 g0 = n * (-1.0f / h0);
+        V3 g3 = n_ * (-1.0f / h0_);
+        atomic_add3(grad, k.x, dE * g0 * coeff);
+        atomic_add3(grad, k.y, dE * g1 * coeff);
+        atomic_add3(grad, k.z, dE * g2 * coeff);
+        atomic_add3(grad, k.w, dE * g3 * coeff);
+        V3 m0 = normalize(cross(e0, n)), m1 = normalize(cross(e1, n)), m2 = normalize(cross(n, e2));
+        V3 m0_ = normalize(cross(n_, e0)), m1_ = normalize(cross(n_, e1_)), m2_ = normalize(cross(e2_, n_));
+        M3 M0 = outer(n, m0), M1 = outer(n, m1), M2 = outer(n, m2);
+        M3 M0_ = outer(n_, m0_), M1_ = outer(n_, m1_), M2_ = outer(n_, m2_);
+        M3 N0 = scl(M0, 1.0f / e0_lsq), N0_ = scl(M0_, 1.0f / e0_lsq);
+        M3 P10 = scl(tr(M0), ca1 / (h1 * h0)), P11 = scl(tr(M1), ca1 / (h1 * h1)), P12 = scl(tr(M2), ca1 / (h1 * h2));
+        M3 P20 = scl(tr(M0), ca2 / (h2 * h0)), P21 = scl(tr(M1), ca2 / (h2 * h1)), P22 = scl(tr(M2), ca2 / (h2 * h2));
+        M3 P10_ = scl(tr(M0_), ca1_ / (h1_ * h0_)), P11_ = scl(tr(M1_), ca1_ / (h1_ * h1_)),
+           P12_ = scl(tr(M2_), ca1_ / (h1_ * h2_));
+        M3 P20_ = scl(tr(M0_), ca2_ / (h2_ * h0_)), P21_ = scl(tr(M1_), ca2_ / (h2_ * h1_)),
+           P22_ = scl(tr(M2_), ca2_ / (h2_ * h2_));
+        M3 Q0 = scl(M0, 1.0f / (h0 * h0)), Q1 = scl(M1, 1.0f / (h0 * h1)), Q2 = scl(M2, 1.0f / (h0 * h2));
+        M3 Q0_ = scl(M0_, 1.0f / (h0_ * h0_)), Q1_ = scl(M1_, 1.0f / (h0_ * h1_)), Q2_ = scl(M2_, 1.0f / (h0_ * h2_));
+        float* B = hblock + 144 * (long long)i;
+        put(B, 0, 0, add(scl(sym(Q0), -dE), scl(outer(g0, g0), d2E)));
+        put(B, 3, 3, add(scl(sym(Q0_), -dE), scl(outer(g3, g3), d2E)));
+        put(B, 1, 1, add(scl(sub(add(sub(sym(P11), N0), sym(P11_)), N0_), dE), scl(outer(g1, g1), d2E)));
+        put(B, 2, 2, add(scl(sub(add(sub(sym(P22), N0), sym(P22_)), N0_), dE), scl(outer(g2, g2), d2E)));
+        M3 b10 = add(scl(sub(P10, Q1), dE), scl(outer(g1, g0), d2E));
+        M3 b20 = add(scl(sub(P20, Q2), dE), scl(outer(g2, g0), d2E));
+        M3 b13 = add(scl(sub(P10_, Q1_), dE), scl(outer(g1, g3), d2E));
+        M3 b23 = add(scl(sub(P20_, Q2_), dE), scl(outer(g2, g3), d2E));
+        M3 b12 = add(scl(add(add(add(add(add(P12, tr(P21)), N0), P12_), tr(P21_)), N0_), dE), scl(outer(g1, g2), d2E));
+        M3 b03 = scl(outer(g0, g3), d2E);
+        put(B, 1, 0, b10); put(B, 0, 1, tr(b10));
+        put(B, 2, 0, b20); put(B, 0, 2, tr(b20));
+        put(B, 1, 3, b13); put(B, 3, 1, tr(b13));
+        put(B, 2, 3, b23); put(B, 3, 2, tr(b23));
+        put(B, 1, 2, b12); put(B, 2, 1, tr(b12));
+        put(B, 0, 3, b03); put(B, 3, 0, tr(b03));
+        // the Jacobi diagonal takes the blocks before projection (hinge_energy.py:255-256)
+        const int ids[4] = {k.x, k.y, k.z, k.w};
+        for (int a = 0; a < 4; ++a) {
+            const float* D = B + 9 * (4 * a + a);
+            atomic_add3(diag, ids[a], mk(D[0], D[4], D[8]));
+        }
+        spd_project_hinge(B);
+    } else if ((i -= nH) < nC) {
+        float d = cd[i];
+        if (d >= P.d_hat) return;
+        int4 k = tp.cidx[i];
+        V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+        V3 g[4];
+        if (tp.ctype1[i] == BLOCK_PT) pt_pair_grad(x0, x1, x2, x3, ctype2[i], g);
+        else ee_pair_grad(x0, x1, x2, x3, ctype2[i], g);
+        float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
+        float c2 = barrier_d2(d, P.d_hat) * P.kappa;
+        const int ids[4] = {k.x, k.y, k.z, k.w};
+        for (int a = 0; a < 4; ++a) {
+            cdd[4 * i + a] = f4(g[a]);
+            atomic_add3(grad, ids[a], g[a] * c1);
+            atomic_add3(diag, ids[a], c2 * mk(g[a].x * g[a].x, g[a].y * g[a].y, g[a].z * g[a].z));
+        }
+    }
+}
+
+// Hessian-vector product H dx (Gauss-Newton for contacts).
+__global__ void __launch_bounds__(kB) hvp_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF, int nH,
+                                            int nC, const float* __restrict__ cd, const float* __restrict__ hblock,
+                                            const float4* __restrict__ cdd, const float4* __restrict__ dx,
+                                            float4* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nV) {
+        float w = tp.vor[i] * P.k_m2gt;
+        atomic_add3(out, (int)i, v3(dx[i]) * w);
+    } else if ((i -= nV) < nF) {
+        int4 f = tp.tri[i];
+        V3 x0 = v3(x[f.x]), x1 = v3(x[f.y]), x2 = v3(x[f.z]);
+        V3 d0 = v3(dx[f.x]), d1 = v3(dx[f.y]), d2 = v3(dx[f.z]);
+        float4 idm = tp.inv_dm[i];
+        Membrane m = membrane(x0, x1, x2, idm);
+        V3 dx01 = d1 - d0, dx02 = d2 - d0;
+        float dDs[4] = {dot(m.b1, dx01), dot(m.b1, dx02), dot(m.b2, dx01), dot(m.b2, dx02)};
+        float Dm[4] = {idm.x, idm.y, idm.z, idm.w};
+        float dF[4];
+        mat22_mul(dDs, Dm, dF);
+        // dE = 0.5 (dF^T F + F^T dF)
+        float a[4] = {dF[0] * m.F[0] + dF[2] * m.F[2], dF[0] * m.F[1] + dF[2] * m.F[3],
+                      dF[1] * m.F[0] + dF[3] * m.F[2], dF[1] * m.F[1] + dF[3] * m.F[3]};
+        float b[4] = {m.F[0] * dF[0] + m.F[2] * dF[2], m.F[0] * dF[1] + m.F[2] * dF[3],
+                      m.F[1] * dF[0] + m.F[3] * dF[2], m.F[1] * dF[1] + m.F[3] * dF[3]};
+        float dE[4];
+        for (int k = 0; k < 4; ++k) dE[k] = 0.5f * (a[k] + b[k]);
+        float trdE = dE[0] + dE[3];
+        float S[4] = {2.0f * P.mu * m.E[0] + P.la * m.trE, 2.0f * P.mu * m.E[1], 2.0f * P.mu * m.E[2],
+                      2.0f * P.mu * m.E[3] + P.la * m.trE};
+        float dS[4] = {2.0f * P.mu * dE[0] + P.la * trdE, 2.0f * P.mu * dE[1], 2.0f * P.mu * dE[2],
+                       2.0f * P.mu * dE[3] + P.la * trdE};
+        float t1[4], t2[4];
+        mat22_mul(dF, S, t1);
+        mat22_mul(m.F, dS, t2);
+        float dP[4] = {t1[0] + t2[0], t1[1] + t2[1], t1[2] + t2[2], t1[3] + t2[3]};
+        float IdT[4] = {idm.x, idm.z, idm.y, idm.w};
+        float dH[4];
+        mat22_mul(dP, IdT, dH);
+        float W = tp.area[i];
+        for (int k = 0; k < 4; ++k) dH[k] *= W;
+        V3 df1 = dH[0] * m.b1 + dH[2] * m.b2;
+        V3 df2 = dH[1] * m.b1 + dH[3] * m.b2;
+        V3 df0 = -df1 - df2;
+        atomic_add3(out, f.x, df0);
+        atomic_add3(out, f.y, df1);
+        atomic_add3(out, f.z, df2);
+    } else if ((i -= nF) < nH) {
+        int4 k = tp.hinge[i];
+        const int ids[4] = {k.x, k.y, k.z, k.w};
+        V3 xs[4];
+        for (int a = 0; a < 4; ++a) xs[a] = v3(dx[ids[a]]);
+        const float* B = hblock + 144 * (long long)i;
+        for (int a = 0; a < 4; ++a) {
+            V3 acc = mk(0.0f, 0.0f, 0.0f);
+            for (int bb = 0; bb < 4; ++bb) {
+                const float* M = B + 9 * (4 * a + bb);
+                V3 w = xs[bb];
+                acc = acc + mk(M[0] * w.x + M[1] * w.y + M[2] * w.z, M[3] * w.x + M[4] * w.y + M[5] * w.z,
+                               M[6] * w.x + M[7] * w.y + M[8] * w.z);
+            }
+            atomic_add3(out, ids[a], acc);
+        }
+    } else if ((i -= nH) < nC) {
+        float d = cd[i];
+        if (d >= P.d_hat) return;
+        int4 k = tp.cidx[i];
+        const int ids[4] = {k.x, k.y, k.z, k.w};
+        float c2 = barrier_d2(d, P.d_hat) * P.kappa;
+        V3 g[4];
+        float s = 0.0f;
+        for (int a = 0; a < 4; ++a) {
+            g[a] = v3(cdd[4 * i + a]);
+            s += dot(g[a], v3(dx[ids[a]]));
+        }
+        for (int a = 0; a < 4; ++a) atomic_add3(out, ids[a], g[a] * (s * c2));
+    }
+}
+
+// ------------------------------------------------------------ PCG (B8)
+// z0 = r / (diag + 1e-6); v = z; zr partials (cg_solver.py:120-142)
+__global__ void __launch_bounds__(kB) cg_init_k(const float4* __restrict__ grad, const float4* __restrict__ diag,
+                                                int n, float4* __restrict__ r, float4* __restrict__ z,
+                                                float4* __restrict__ v, float4* __restrict__ p,
+                                                double* __restrict__ part) {
+    __shared__ double red[kB];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.0;
+    if (i < n) {
+        V3 rr = v3(grad[i]), dg = v3(diag[i]);
+        V3 zz = mk(rr.x / (dg.x + 1e-6f), rr.y / (dg.y + 1e-6f), rr.z / (dg.z + 1e-6f));
+        r[i] = f4(rr);
+        z[i] = f4(zz);
+        v[i] = f4(zz);
+        p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        x = (double)zz.x * rr.x + (double)zz.y * rr.y + (double)zz.z * rr.z;
+    }
+    double s = block_sum(x, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// alpha = zr_last / vAv (if vAv > 1e-16); p += alpha v; r -= alpha Av;
+// z = r / diag; zr partials (solver_kernels.py:4-23)
+__global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, const float4* __restrict__ Av,
+                                                  const float4* __restrict__ diag, const double* __restrict__ sc,
+                                                  int n, float4* __restrict__ p, float4* __restrict__ r,
+                                                  float4* __restrict__ z, double* __restrict__ part) {
+    __shared__ double red[kB];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.0;
+    if (i < n) {
+        const double vAv = sc[SC_VAV];
+        V3 rr = v3(r[i]);
+        if (vAv > 1e-16) {
+            float alpha = (float)(sc[SC_ZRLAST] / vAv);
+            p[i] = f4(v3(p[i]) + alpha * v3(v[i]));
+            rr = rr - alpha * v3(Av[i]);
+            r[i] = f4(rr);
+        }
+        V3 dg = v3(diag[i]);
+        V3 zz = mk(rr.x / dg.x, rr.y / dg.y, rr.z / dg.z);
+        z[i] = f4(zz);
+        x = (double)zz.x * rr.x + (double)zz.y * rr.y + (double)zz.z * rr.z;
+    }
+    double s = block_sum(x, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// v = z + (zr / zr_last) v; also clears Av for the next product.
+__global__ void cg_dir_k(const float4* __restrict__ z, const double* __restrict__ sc, int n, float4* __restrict__ v,
+                         float4* __restrict__ Av) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = (float)(sc[SC_ZR] / sc[SC_ZRLAST]);
+    v[i] = f4(v3(z[i]) + s * v3(v[i]));
+    Av[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void copy_slot_k(double* sc, int from, int to) { sc[to] = sc[from]; }
+
+// ------------------------------------------------------------ clamp, CCD, line search (B9, B10)
+__global__ void clamp_k(float4* __restrict__ p, const float4* __restrict__ qn, const float4* __restrict__ qd, int n,
+                        float radius) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 pi = v3(p[i]), qni = v3(qn[i]), qdi = v3(qd[i]);
+    V3 dq = qni + pi - qdi;
+    if (len(dq) > radius) p[i] = f4(normalize(dq) * radius + qdi - qni);
+}
+
+// Additive CCD (Li et al. 2021), ccd_kernels.py:8-110; min step by an
+// integer atomic on the non-negative float's bits.
+__global__ void __launch_bounds__(kB) accd_k(const float4* __restrict__ x, const float4* __restrict__ vel, Topo tp,
+                                             Params P, int nC, float* __restrict__ ccd_step) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nC) return;
+    int4 k = tp.cidx[b];
+    V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+    V3 v0 = v3(vel[k.x]), v1 = v3(vel[k.y]), v2 = v3(vel[k.z]), v3_ = v3(vel[k.w]);
+    const bool is_pt = tp.ctype1[b] == BLOCK_PT;
+    V3 va = (v0 + v1 + v2 + v3_) / 4.0f;
+    v0 = v0 - va; v1 = v1 - va; v2 = v2 - va; v3_ = v3_ - va;
+    float lv = is_pt ? len(v0) + fmaxf(len(v1), fmaxf(len(v2), len(v3_)))
+                     : fmaxf(len(v0), len(v1)) + fmaxf(len(v2), len(v3_));
+    if (lv < 1e-12f) return;
+    auto dist = [&]() {
+        if (is_pt) {
+            int t = pt_classify(x0, x1, x2, x3);
+            return pt_distance(x0, x1, x2, x3, t);
+        }
+        int t = ee_classify(x0, x1, x2, x3, P.ee_thres);
+        return ee_distance(x0, x1, x2, x3, t);
+    };
+    float d = dist();
+    float dsq = d * d;
+    float dmt = fmaxf((dsq - P.ccd_thick * P.ccd_thick) / (sqrtf(dsq) + P.ccd_thick), 0.0f);
+    const float g = (1.0f - P.ccd_slack) * dmt;
+    float t = 0.0f;
+    float tl = P.ccd_slack * dmt / lv;
+    int it = 0;
+    while (true) {
+        x0 = x0 + tl * v0; x1 = x1 + tl * v1; x2 = x2 + tl * v2; x3 = x3 + tl * v3_;
+        d = dist();
+        dsq = d * d;
+        dmt = fmaxf((dsq - P.ccd_thick * P.ccd_thick) / (sqrtf(dsq) + P.ccd_thick), 0.0f);
+        if ((it > 0 && dmt <= g) || it >= P.ccd_max_iters) {
+            atomicMin((int*)ccd_step, __float_as_int(t));
+            return;
+        }
+        t = t + tl;
+        if (t > 1.0f) return;
+        tl = 0.8f * dmt / lv;
+        ++it;
+    }
+}
+
+// q = q_n + alpha 0.5^n p unless n > 0 and the energy already decreased
+// (solver_kernels.py:61-74)
+__global__ void line_search_k(const float4* __restrict__ qn, const float4* __restrict__ p,
+                              const float* __restrict__ alpha, int nh, const double* __restrict__ sc, int n,
+                              float4* __restrict__ q) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (nh > 0 && sc[SC_E] < sc[SC_EPREV]) return;
+    float f = alpha[0] * powf(0.5f, (float)nh);
+    q[i] = f4(v3(qn[i]) + f * v3(p[i]));
+}
+
+// ------------------------------------------------------------ dense quadratic test energy
+// E = 0.5 x^T A x + b^T x + c over the first n3 = 3 * nV coordinates
+// (DummyEnergyCalculator, energy.py:62-111); A row-major FP64, n3 <= 3072.
+__global__ void __launch_bounds__(kB) dense_energy_k(const float4* __restrict__ x, int nV, const double* __restrict__ A,
+                                                     const double* __restrict__ b, double c0, double* sc, int slot) {
+    __shared__ double red[kB];
+    const int n3 = 3 * nV;
+    double acc = 0.0;
+    for (int r = threadIdx.x; r < n3; r += kB) {
+        double ax = 0.0;
+        for (int k = 0; k < n3; ++k) ax += A[(long long)r * n3 + k] * (double)((const float*)&x[k / 3])[k % 3];
+        const double xr = (double)((const float*)&x[r / 3])[r % 3];
+        acc += 0.5 * xr * ax + b[r] * xr;
+    }
+    double s = block_sum(acc, red);
+    if (threadIdx.x == 0) sc[slot] += s + c0;
+}
+
+__global__ void __launch_bounds__(kB) dense_diff_k(const float4* __restrict__ x, int nV, const double* __restrict__ A,
+                                                   const double* __restrict__ b, float coeff, float4* __restrict__ grad,
+                                                   float4* __restrict__ diag) {
+    const int n3 = 3 * nV;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n3) return;
+    double ax = b[r];
+    for (int k = 0; k < n3; ++k) ax += A[(long long)r * n3 + k] * (double)((const float*)&x[k / 3])[k % 3];
+    atomicAdd(&((float*)&grad[r / 3])[r % 3], (float)ax * coeff);
+    atomicAdd(&((float*)&diag[r / 3])[r % 3], (float)A[(long long)r * n3 + r]);
+}
+
+__global__ void __launch_bounds__(kB) dense_hvp_k(int nV, const double* __restrict__ A, const float4* __restrict__ dx,
+                                                  float4* __restrict__ out) {
+    const int n3 = 3 * nV;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n3) return;
+    double ax = 0.0;
+    for (int k = 0; k < n3; ++k) ax += A[(long long)r * n3 + k] * (double)((const float*)&dx[k / 3])[k % 3];
+    atomicAdd(&((float*)&out[r / 3])[r % 3], (float)ax);
+}
+
+// ------------------------------------------------------------ host orchestration
+#define L3(kern, n, ...)                                                                                  \
+    do {                                                                                                  \
+        hipLaunchKernelGGL(kern, dim3(grid_for((long long)(n), kB)), dim3(kB), 0, c.stream, __VA_ARGS__); \
+        PAMO_HIP(hipGetLastError());                                                                      \
+    } while (0)
+
+static Topo topo(Ctx& c) {
+    return Topo{c.tri, c.area, c.inv_dm, c.hinge, c.rest_angle, c.rest_elen, c.vor, c.target, c.cidx, c.ctype1};
+}
+
+// Item counts of the enabled energy terms (items of a disabled term are
+// simply not launched).
+struct Counts {
+    int v, f, h, c;
+    long long total() const { return (long long)v + f + h + c; }
+};
+static Counts counts(const Ctx& c, int nC, int mask) {
+    return Counts{(mask & PAMO_S3_E_M2GT) ? c.nV : 0, (mask & PAMO_S3_E_ELASTIC) ? c.nF : 0,
+                  (mask & PAMO_S3_E_HINGE) ? c.nH : 0, (mask & PAMO_S3_E_COLLISION) ? nC : 0};
+}
+
+static int compute_energy(Ctx& c, int nC, int slot, int mask) {
+    const Counts k = counts(c, nC, mask);
+    PAMO_TRY(ensure_part(c, k.total()));
+    const unsigned nb = grid_for(k.total(), kB);
+    hipLaunchKernelGGL(energy_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c,
+                       c.cd, c.ctype2, c.part);
+    hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, slot);
+    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
+        hipLaunchKernelGGL(dense_energy_k, dim3(1), dim3(kB), 0, c.stream, (const float4*)c.q, c.nV,
+                           (const double*)c.dense_A, (const double*)c.dense_b, c.dense_c, c.sc, slot);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+// grad (coefficient -1) and Jacobi diagonal; hinge blocks and contact
+// distance gradients are stored for the Hessian-vector products.
+static int compute_diff(Ctx& c, int nC, int mask) {
+    const Counts k = counts(c, nC, mask);
+    PAMO_TRY(fill(c.grad, make_float4(0, 0, 0, 0), c.nV, c.stream));
+    PAMO_TRY(fill(c.diag, make_float4(0, 0, 0, 0), c.nV, c.stream));
+    if (k.total() > 0)
+        L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
+           (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.cdd);
+    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
+        L3(dense_diff_k, 3 * c.nV, (const float4*)c.q, c.nV, (const double*)c.dense_A, (const double*)c.dense_b, -1.0f,
+           c.grad, c.diag);
+    return 0;
+}
+
+// out += H dx (out must be cleared by the caller)
+static int compute_hvp(Ctx& c, int nC, int mask, const float4* dx, float4* out) {
+    const Counts k = counts(c, nC, mask);
+    if (k.total() > 0)
+        L3(hvp_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, (const float*)c.cd,
+           (const float*)c.hblock, (const float4*)c.cdd, dx, out);
+    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
+        L3(dense_hvp_k, 3 * c.nV, c.nV, (const double*)c.dense_A, dx, out);
+    return 0;
+}
+
+static int dot_into(Ctx& c, const float4* a, const float4* b, int slot) {
+    const unsigned nb = grid_for(c.nV, kB);
+    hipLaunchKernelGGL(dot_part_k, dim3(nb), dim3(kB), 0, c.stream, a, b, c.nV, c.part);
+    hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, slot);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+static int ensure_contacts(Ctx& c, int n) {
+    if (n <= c.capC) return 0;
+    int a = n + n / 4 + 1024;
+    c.capC = a;
+    PAMO_TRY(dalloc(&c.cidx, a));
+    PAMO_TRY(dalloc(&c.ctype1, a));
+    PAMO_TRY(dalloc(&c.ctype2, a));
+    PAMO_TRY(dalloc(&c.cd, a));
+    PAMO_TRY(dalloc(&c.cdd, 4 * (size_t)a));
+    return 0;
+}
+
+// _detect_contact (system.py:304-318, energy.py:896-948): refit the edge and
+// triangle trees, gather PT and EE pairs within 2r + d_hat, halve r and
+// retry while the count exceeds max_blocks; q_prev_detection <- q.
+static int detect_contacts(Ctx& c, int& nC) {
+    nC = 0;
+    if (c.P.mask & PAMO_S3_E_COLLISION) {
+        PAMO_TRY(tree_alloc(c.tri_tree, c.nF));
+        PAMO_TRY(tree_alloc(c.edge_tree, c.nE));
+        L3(tri_items_k, c.nF, (const float4*)c.q, (const int4*)c.tri, c.nF, 0.0f, c.tri_tree.item);
+        PAMO_TRY(tree_build(c.tri_tree, c.nF, c.ws, c.stream));
+        L3(edge_items_k, c.nE, (const float4*)c.q, (const int2*)c.edge, c.nE, c.edge_tree.item);
+        PAMO_TRY(tree_build(c.edge_tree, c.nE, c.ws, c.stream));
+        while (true) {
+            PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
+            const int cap = std::min(c.capC, c.P.max_blocks);
+            L3(detect_pt_k, c.nV, (const float4*)c.q, c.nV, (const int4*)c.tri, view(c.tri_tree), c.P.d_hat,
+               c.radius, c.ccount, cap, c.cidx, c.ctype1);
+            L3(detect_ee_k, c.nE, (const float4*)c.q, c.nE, (const int2*)c.edge, view(c.edge_tree), c.P.d_hat,
+               c.radius, c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1);
+            PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            PAMO_HIP(hipStreamSynchronize(c.stream));
+            const int n = c.hcount[0];
+            if (n > c.P.max_blocks) {
+                PAMO_CHECK(c.radius > 1e-30f, "contact detection: more than max_blocks pairs within d_hat");
+                c.radius /= 2.0f;
+                continue;
+            }
+            if (n > c.capC) {
+                PAMO_TRY(ensure_contacts(c, n));
+                continue;
+            }
+            nC = n;
+            break;
+        }
+    }
+    PAMO_HIP(hipMemcpyAsync(c.qd, c.q, sizeof(float4) * c.nV, hipMemcpyDeviceToDevice, c.stream));
+    c.stat_contacts += nC;
+    return 0;
+}
+
+static int update_target(Ctx& c) {
+    if (c.P.mask & PAMO_S3_E_M2GT)
+        L3(closest_point_k, c.nV, (const float4*)c.q, c.nV, view(c.gt_tree), (const float4*)c.gt_v,
+           (const int4*)c.gt_f, c.target);
+    return 0;
+}
+
+// CGSolver.solve (cg_solver.py:96-158): Jacobi PCG with n_cg fixed
+// iterations, p = H^-1 grad.
+static int cg_solve(Ctx& c, int nC) {
+    const int nV = c.nV, mask = c.P.mask;
+    const unsigned nb = grid_for(nV, kB);
+    hipLaunchKernelGGL(cg_init_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.grad, (const float4*)c.diag, nV,
+                       c.r, c.z, c.v, c.p, c.part);
+    hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, (int)SC_ZR);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(fill(c.Av, make_float4(0, 0, 0, 0), nV, c.stream));
+    for (int k = 0; k < c.P.n_cg; ++k) {
+        ++c.stat_cg;
+        PAMO_TRY(compute_hvp(c, nC, mask, c.v, c.Av));
+        PAMO_TRY(dot_into(c, c.v, c.Av, SC_VAV));
+        hipLaunchKernelGGL(copy_slot_k, dim3(1), dim3(1), 0, c.stream, c.sc, (int)SC_ZR, (int)SC_ZRLAST);
+        hipLaunchKernelGGL(cg_update_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.v, (const float4*)c.Av,
+                           (const float4*)c.diag, (const double*)c.sc, nV, c.p, c.r, c.z, c.part);
+        hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc,
+                           (int)SC_ZR);
+        hipLaunchKernelGGL(cg_dir_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.z, (const double*)c.sc, nV, c.v,
+                           c.Av);
+        PAMO_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// _clamp_p + _ccd (system.py:354-391)
+static int clamp_and_ccd(Ctx& c, int nC) {
+    L3(clamp_k, c.nV, c.p, (const float4*)c.qn, (const float4*)c.qd, c.nV, c.P.radius0);
+    PAMO_TRY(fill(c.ccd_step, 1.0f, 1, c.stream));
+    if (nC > 0) L3(accd_k, nC, (const float4*)c.q, (const float4*)c.p, topo(c), c.P, nC, c.ccd_step);
+    return 0;
+}
+
+static int check_ready(const Ctx& c, const char* who) {
+    PAMO_CHECK(c.nV > 0, std::string(who) + ": no mesh registered");
+    PAMO_CHECK(!(c.P.mask & PAMO_S3_E_M2GT) || c.gt_tree.n > 0, std::string(who) + ": no target surface registered");
+    PAMO_CHECK(!(c.P.mask & (PAMO_S3_E_ELASTIC | PAMO_S3_E_COLLISION)) || c.nF > 0,
+               std::string(who) + ": the enabled energies need faces");
+    PAMO_CHECK(!(c.P.mask & PAMO_S3_E_COLLISION) || c.nE > 0, std::string(who) + ": contacts need edges");
+    PAMO_CHECK(!(c.P.mask & PAMO_S3_E_DENSE) || c.dense_A, std::string(who) + ": dense energy not set");
+    return 0;
+}
+
+// One step(): target update, then n_newton Newton iterations (system.py:411-470).
+static int step_once(Ctx& c) {
+    if (c.do_nothing) return 0;
+    const int nV = c.nV, mask = c.P.mask;
+    PAMO_TRY(update_target(c));
+    for (int it = 0; it < c.P.n_newton; ++it) {
+        ++c.stat_newton;
+        PAMO_HIP(hipMemcpyAsync(c.qn, c.q, sizeof(float4) * nV, hipMemcpyDeviceToDevice, c.stream));
+        int nC = 0;
+        PAMO_TRY(detect_contacts(c, nC));
+        PAMO_TRY(compute_energy(c, nC, SC_E, mask));
+        PAMO_TRY(compute_diff(c, nC, mask));
+        PAMO_TRY(cg_solve(c, nC));
+        PAMO_TRY(clamp_and_ccd(c, nC));
+        // _line_search (system.py:393-409)
+        hipLaunchKernelGGL(copy_slot_k, dim3(1), dim3(1), 0, c.stream, c.sc, (int)SC_E, (int)SC_EPREV);
+        for (int h = 0; h < c.P.n_ls; ++h) {
+            ++c.stat_ls;
+            L3(line_search_k, nV, (const float4*)c.qn, (const float4*)c.p, (const float*)c.ccd_step, h,
+               (const double*)c.sc, nV, c.q);
+            PAMO_TRY(compute_energy(c, nC, SC_E, mask));
+        }
+    }
+    PAMO_HIP(hipMemcpyAsync(c.hsc, c.sc, sizeof(double) * SC_N, hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    c.last_energy = c.hsc[SC_E];
+    return 0;
+}
+
+}  // namespace s3
+}  // namespace pamo
+
+// ================================================================ C ABI
+
+using pamo::s3::Ctx;
+
+extern "C" {
+
+int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
+    pamo::clear_error();
+    PAMO_CHECK(out && prm, "pamo_s3_create: null argument");
+    Ctx* c = new Ctx();
+    c->device = device;
+    pamo::s3::Params& P = c->P;
+    P.n_newton = prm->n_newton_iters;
+    P.n_cg = prm->n_cg_iters;
+    P.n_ls = prm->n_line_search_iters;
+    P.max_blocks = prm->max_blocks;
+    P.ccd_max_iters = prm->ccd_max_iters;
+    P.mask = prm->energy_mask;
+    P.d_hat = (float)prm->d_hat;
+    P.kappa = (float)prm->coll_stiffness;
+    P.radius0 = (float)prm->contact_detection_radius;
+    P.ee_thres = (float)prm->ee_classify_thres;
+    P.ccd_slack = (float)prm->ccd_slackness;
+    P.ccd_thick = (float)prm->ccd_thickness;
+    const double E = prm->elas_young_modulus, nu = prm->elas_poisson_ratio;
+    P.mu = (float)(E / (2.0 * (1.0 + nu)));
+    P.la = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    P.k_hinge = (float)prm->hinge_stiffness;
+    P.k_m2gt = (float)prm->mesh2gt_dist_stiffness;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->ccount, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->sc, sizeof(double) * pamo::s3::SC_N);
+    if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(double) * pamo::s3::SC_N);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, sizeof(float));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcount, sizeof(int), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hsc, sizeof(double) * pamo::s3::SC_N, 0);
+    if (e != hipSuccess) {
+        pamo::set_error(std::string("pamo_s3_create: ") + hipGetErrorString(e));
+        pamo::s3::ctx_free(*c);
+        delete c;
+        return 1;
+    }
+    *out = reinterpret_cast<pamo_s3*>(c);
+    return 0;
+}
+
+int pamo_s3_destroy(pamo_s3* s) {
+    if (!s) return 0;
+    Ctx* c = reinterpret_cast<Ctx*>(s);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pamo::s3::ctx_free(*c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int pamo_s3_set_target(pamo_s3* s, const float* d_gt_v, int n_gt_v, const int* d_gt_f, int n_gt_f, double scale,
+                       const double* t3) {
+    pamo::clear_error();
+    PAMO_CHECK(s && d_gt_v && d_gt_f && t3, "pamo_s3_set_target: null argument");
+    PAMO_CHECK(n_gt_v > 0 && n_gt_f > 0, "pamo_s3_set_target: empty mesh");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    if (n_gt_v > c.capG || n_gt_f > c.capG) {
+        c.capG = std::max(n_gt_v, n_gt_f) + 1024;
+        PAMO_TRY(pamo::s3::dalloc(&c.gt_v, c.capG));
+        PAMO_TRY(pamo::s3::dalloc(&c.gt_f, c.capG));
+    }
+    c.nGV = n_gt_v;
+    c.nGF = n_gt_f;
+    PAMO_TRY(pamo::s3::ensure_ws(c, n_gt_f + 64));
+    hipLaunchKernelGGL(pamo::s3::normalize_points_k, dim3(pamo::grid_for(n_gt_v, 256)), dim3(256), 0, c.stream,
+                       d_gt_v, n_gt_v, scale, t3[0], t3[1], t3[2], c.gt_v, (float4*)nullptr);
+    hipLaunchKernelGGL(pamo::s3::load_faces4_k, dim3(pamo::grid_for(n_gt_f, 256)), dim3(256), 0, c.stream, d_gt_f,
+                       n_gt_f, c.gt_f);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(pamo::s3::tree_alloc(c.gt_tree, n_gt_f));
+    hipLaunchKernelGGL(pamo::s3::tri_items_k, dim3(pamo::grid_for(n_gt_f, 256)), dim3(256), 0, c.stream,
+                       (const float4*)c.gt_v, (const int4*)c.gt_f, n_gt_f, 0.0f, c.gt_tree.item);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(pamo::s3::tree_build(c.gt_tree, n_gt_f, c.ws, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int n_f, double scale, const double* t3,
+                     const int* h_edges, int n_e, const int* h_hinges, int n_h, const float* h_voronoi,
+                     int do_nothing) {
+    pamo::clear_error();
+    PAMO_CHECK(s && d_v && t3 && h_voronoi, "pamo_s3_set_mesh: null argument");
+    PAMO_CHECK(n_v > 0 && n_f >= 0 && n_e >= 0 && n_h >= 0, "pamo_s3_set_mesh: bad sizes");
+    PAMO_CHECK((n_f == 0 || d_f) && (n_e == 0 || h_edges) && (n_h == 0 || h_hinges),
+               "pamo_s3_set_mesh: null topology array");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    using pamo::s3::dalloc;
+    if (n_v > c.capV) {
+        c.capV = n_v + n_v / 4 + 64;
+        float4** vv[] = {&c.q, &c.qn, &c.qd, &c.qrest, &c.p, &c.target, &c.grad, &c.diag, &c.r, &c.v, &c.Av, &c.z};
+        for (float4** q : vv) PAMO_TRY(dalloc(q, c.capV));
+        PAMO_TRY(dalloc(&c.vor, c.capV));
+    }
+    if (n_f > c.capF) {
+        c.capF = n_f + n_f / 4 + 64;
+        PAMO_TRY(dalloc(&c.tri, c.capF));
+        PAMO_TRY(dalloc(&c.area, c.capF));
+        PAMO_TRY(dalloc(&c.inv_dm, c.capF));
+    }
+    if (n_e > c.capE) {
+        c.capE = n_e + n_e / 4 + 64;
+        PAMO_TRY(dalloc(&c.edge, c.capE));
+    }
+    if (n_h > c.capH) {
+        c.capH = n_h + n_h / 4 + 64;
+        PAMO_TRY(dalloc(&c.hinge, c.capH));
+        PAMO_TRY(dalloc(&c.rest_angle, c.capH));
+        PAMO_TRY(dalloc(&c.rest_elen, c.capH));
+        PAMO_TRY(dalloc(&c.hblock, 144 * (size_t)c.capH));
+    }
+    c.nV = n_v;
+    c.nF = n_f;
+    c.nE = n_e;
+    c.nH = n_h;
+    c.do_nothing = do_nothing != 0;
+    c.radius = c.P.radius0;
+    c.stat_contacts = c.stat_newton = c.stat_cg = c.stat_ls = 0;
+    PAMO_TRY(pamo::s3::ensure_ws(c, std::max(n_f, n_e) + 64));
+    PAMO_TRY(pamo::s3::ensure_part(c, (long long)n_v + n_f + n_h + 1024));
+    PAMO_TRY(pamo::s3::ensure_contacts(c, 4 * n_v + 1024));
+    hipLaunchKernelGGL(pamo::s3::normalize_points_k, dim3(pamo::grid_for(n_v, 256)), dim3(256), 0, c.stream, d_v,
+                       n_v, scale, t3[0], t3[1], t3[2], c.q, c.qrest);
+    hipLaunchKernelGGL(pamo::s3::load_faces4_k, dim3(pamo::grid_for(n_f, 256)), dim3(256), 0, c.stream, d_f, n_f,
+                       c.tri);
+    PAMO_HIP(hipGetLastError());
+    if (n_e > 0)
+        PAMO_HIP(hipMemcpyAsync(c.edge, h_edges, sizeof(int) * 2 * (size_t)n_e, hipMemcpyHostToDevice, c.stream));
+    if (n_h > 0)
+        PAMO_HIP(hipMemcpyAsync(c.hinge, h_hinges, sizeof(int) * 4 * (size_t)n_h, hipMemcpyHostToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.vor, h_voronoi, sizeof(float) * (size_t)n_v, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(pamo::s3::elastic_pre_k, dim3(pamo::grid_for(n_f, 256)), dim3(256), 0, c.stream,
+                       (const float4*)c.qrest, (const int4*)c.tri, n_f, c.area, c.inv_dm);
+    if (n_h > 0)
+        hipLaunchKernelGGL(pamo::s3::hinge_rest_k, dim3(pamo::grid_for(n_h, 256)), dim3(256), 0, c.stream,
+                           (const float4*)c.qrest, (const int4*)c.hinge, n_h, c.rest_angle, c.rest_elen);
+    PAMO_HIP(hipGetLastError());
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int pamo_s3_step(pamo_s3* s, int n_steps) {
+    pamo::clear_error();
+    PAMO_CHECK(s != nullptr, "pamo_s3_step: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_TRY(pamo::s3::check_ready(c, "pamo_s3_step"));
+    PAMO_HIP(hipSetDevice(c.device));
+    for (int k = 0; k < n_steps; ++k) PAMO_TRY(pamo::s3::step_once(c));
+    return 0;
+}
+
+int pamo_s3_get_vertices(pamo_s3* s, float* d_out, double scale, const double* t3) {
+    pamo::clear_error();
+    PAMO_CHECK(s && d_out, "pamo_s3_get_vertices: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    // normalised -> input frame in FP64: (q - t) / scale
+    std::vector<float4> h(c.nV);
+    PAMO_HIP(hipMemcpyAsync(h.data(), c.q, sizeof(float4) * c.nV, hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    std::vector<float> o(3 * (size_t)c.nV);
+    for (int i = 0; i < c.nV; ++i) {
+        o[3 * (size_t)i] = (float)(((double)h[i].x - t3[0]) / scale);
+        o[3 * (size_t)i + 1] = (float)(((double)h[i].y - t3[1]) / scale);
+        o[3 * (size_t)i + 2] = (float)(((double)h[i].z - t3[2]) / scale);
+    }
+    PAMO_HIP(hipMemcpy(d_out, o.data(), sizeof(float) * o.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pamo_s3_get_normalized(pamo_s3* s, double* h_out) {
+    pamo::clear_error();
+    PAMO_CHECK(s && h_out, "pamo_s3_get_normalized: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    std::vector<float4> h(c.nV);
+    PAMO_HIP(hipMemcpyAsync(h.data(), c.q, sizeof(float4) * c.nV, hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < c.nV; ++i) {
+        h_out[3 * (size_t)i] = h[i].x;
+        h_out[3 * (size_t)i + 1] = h[i].y;
+        h_out[3 * (size_t)i + 2] = h[i].z;
+    }
+    return 0;
+}
+
+int pamo_s3_set_dense(pamo_s3* s, const double* h_A, const double* h_b, double c0, int n_v) {
+    pamo::clear_error();
+    PAMO_CHECK(s && h_A && h_b, "pamo_s3_set_dense: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_CHECK(n_v == c.nV && n_v > 0 && n_v <= 1024, "pamo_s3_set_dense: n_v must equal the mesh's (<= 1024)");
+    PAMO_HIP(hipSetDevice(c.device));
+    const size_t n3 = 3 * (size_t)n_v;
+    PAMO_TRY(pamo::s3::dalloc(&c.dense_A, n3 * n3));
+    PAMO_TRY(pamo::s3::dalloc(&c.dense_b, n3));
+    PAMO_HIP(hipMemcpy(c.dense_A, h_A, sizeof(double) * n3 * n3, hipMemcpyHostToDevice));
+    PAMO_HIP(hipMemcpy(c.dense_b, h_b, sizeof(double) * n3, hipMemcpyHostToDevice));
+    c.dense_c = c0;
+    return 0;
+}
+
+namespace {
+int read_vec(Ctx& c, const float4* d, double* out) {
+    std::vector<float4> h(c.nV);
+    PAMO_HIP(hipMemcpyAsync(h.data(), d, sizeof(float4) * c.nV, hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < c.nV; ++i) {
+        out[3 * (size_t)i] = h[i].x;
+        out[3 * (size_t)i + 1] = h[i].y;
+        out[3 * (size_t)i + 2] = h[i].z;
+    }
+    return 0;
+}
+int write_vec(Ctx& c, const double* in, float4* d) {
+    std::vector<float4> h(c.nV);
+    for (int i = 0; i < c.nV; ++i)
+        h[i] = make_float4((float)in[3 * (size_t)i], (float)in[3 * (size_t)i + 1], (float)in[3 * (size_t)i + 2], 0.0f);
+    PAMO_HIP(hipMemcpyAsync(d, h.data(), sizeof(float4) * c.nV, hipMemcpyHostToDevice, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+double read_slot(Ctx& c, int slot) {
+    double v = 0.0;
+    if (hipMemcpyAsync(&v, c.sc + slot, sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return NAN;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return NAN;
+    return v;
+}
+}  // namespace
+
+int pamo_s3_debug(pamo_s3* s, int what, const double* h_in, long long n_in, double* h_out, long long cap,
+                  long long* n_out) {
+    pamo::clear_error();
+    PAMO_CHECK(s && n_out, "pamo_s3_debug: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_CHECK(c.nV > 0, "pamo_s3_debug: no mesh registered");
+    PAMO_HIP(hipSetDevice(c.device));
+    using namespace pamo::s3;
+    const long long n3 = 3LL * c.nV;
+    *n_out = 0;
+    auto need_in = [&](long long n) { return h_in != nullptr && n_in >= n; };
+    auto need_out = [&](long long n) { return h_out != nullptr && cap >= n; };
+    int nC = 0;
+    switch (what) {
+    case PAMO_S3_DBG_SET_Q:
+        PAMO_CHECK(need_in(n3), "pamo_s3_debug: SET_Q needs 3 n_v inputs");
+        PAMO_TRY(write_vec(c, h_in, c.q));
+        return 0;
+    case PAMO_S3_DBG_TARGET:
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        PAMO_CHECK(need_out(n3), "pamo_s3_debug: output too small");
+        PAMO_TRY(update_target(c));
+        PAMO_TRY(read_vec(c, c.target, h_out));
+        *n_out = n3;
+        return 0;
+    case PAMO_S3_DBG_CONTACTS: {
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        PAMO_TRY(detect_contacts(c, nC));
+        PAMO_CHECK(need_out(5LL * nC), "pamo_s3_debug: output too small");
+        std::vector<int4> idx(nC);
+        std::vector<int> ty(nC);
+        if (nC > 0) {
+            PAMO_HIP(hipMemcpyAsync(idx.data(), c.cidx, sizeof(int4) * nC, hipMemcpyDeviceToHost, c.stream));
+            PAMO_HIP(hipMemcpyAsync(ty.data(), c.ctype1, sizeof(int) * nC, hipMemcpyDeviceToHost, c.stream));
+        }
+        PAMO_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < nC; ++i) {
+            double* o = h_out + 5LL * i;
+            o[0] = idx[i].x; o[1] = idx[i].y; o[2] = idx[i].z; o[3] = idx[i].w; o[4] = ty[i];
+        }
+        *n_out = nC;
+        return 0;
+    }
+    case PAMO_S3_DBG_ENERGY: {
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        PAMO_CHECK(need_out(6), "pamo_s3_debug: output too small");
+        PAMO_TRY(detect_contacts(c, nC));
+        const int bits[5] = {PAMO_S3_E_M2GT, PAMO_S3_E_ELASTIC, PAMO_S3_E_HINGE, PAMO_S3_E_COLLISION, PAMO_S3_E_DENSE};
+        for (int k = 0; k < 5; ++k) {
+            if (c.P.mask & bits[k]) {
+                PAMO_TRY(compute_energy(c, nC, SC_E, bits[k]));
+                h_out[k] = read_slot(c, SC_E);
+            } else {
+                h_out[k] = 0.0;
+            }
+        }
+        PAMO_TRY(compute_energy(c, nC, SC_E, c.P.mask));
+        h_out[5] = read_slot(c, SC_E);
+        *n_out = 6;
+        return 0;
+    }
+    case PAMO_S3_DBG_DIFF:
+    case PAMO_S3_DBG_HVP:
+    case PAMO_S3_DBG_SOLVE:
+    case PAMO_S3_DBG_HINGE: {
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        const long long want = what == PAMO_S3_DBG_DIFF ? 2 * n3 : what == PAMO_S3_DBG_HINGE ? 144LL * c.nH : n3;
+        PAMO_CHECK(need_out(want), "pamo_s3_debug: output too small");
+        PAMO_CHECK(what != PAMO_S3_DBG_HVP || need_in(n3), "pamo_s3_debug: HVP needs 3 n_v inputs");
+        PAMO_TRY(detect_contacts(c, nC));
+        PAMO_TRY(compute_energy(c, nC, SC_E, c.P.mask));
+        PAMO_TRY(compute_diff(c, nC, c.P.mask));
+        if (what == PAMO_S3_DBG_DIFF) {
+            PAMO_TRY(read_vec(c, c.grad, h_out));
+            PAMO_TRY(read_vec(c, c.diag, h_out + n3));
+        } else if (what == PAMO_S3_DBG_HVP) {
+            PAMO_TRY(write_vec(c, h_in, c.v));
+            PAMO_TRY(pamo::fill(c.Av, make_float4(0, 0, 0, 0), c.nV, c.stream));
+            PAMO_TRY(compute_hvp(c, nC, c.P.mask, c.v, c.Av));
+            PAMO_TRY(read_vec(c, c.Av, h_out));
+        } else if (what == PAMO_S3_DBG_SOLVE) {
+            PAMO_TRY(cg_solve(c, nC));
+            PAMO_TRY(read_vec(c, c.p, h_out));
+        } else {
+            std::vector<float> hb(144 * (size_t)c.nH);
+            if (c.nH > 0)
+                PAMO_HIP(hipMemcpyAsync(hb.data(), c.hblock, sizeof(float) * hb.size(), hipMemcpyDeviceToHost,
+                                        c.stream));
+            PAMO_HIP(hipStreamSynchronize(c.stream));
+            for (size_t k = 0; k < hb.size(); ++k) h_out[k] = hb[k];
+        }
+        *n_out = want;
+        return 0;
+    }
+    case PAMO_S3_DBG_CCD: {
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        PAMO_CHECK(need_in(n3) && need_out(n3 + 1), "pamo_s3_debug: CCD needs 3 n_v inputs and 3 n_v + 1 outputs");
+        PAMO_HIP(hipMemcpyAsync(c.qn, c.q, sizeof(float4) * c.nV, hipMemcpyDeviceToDevice, c.stream));
+        PAMO_TRY(detect_contacts(c, nC));
+        PAMO_TRY(compute_energy(c, nC, SC_E, c.P.mask));
+        PAMO_TRY(write_vec(c, h_in, c.p));
+        PAMO_TRY(clamp_and_ccd(c, nC));
+        PAMO_TRY(read_vec(c, c.p, h_out));
+        float st = 0.0f;
+        PAMO_HIP(hipMemcpyAsync(&st, c.ccd_step, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        PAMO_HIP(hipStreamSynchronize(c.stream));
+        h_out[n3] = st;
+        *n_out = n3 + 1;
+        return 0;
+    }
+    default:
+        pamo::set_error("pamo_s3_debug: unknown item");
+        return 1;
+    }
+}
+
+int pamo_s3_stats(pamo_s3* s, double* out8) {
+    pamo::clear_error();
+    PAMO_CHECK(s && out8, "pamo_s3_stats: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    out8[0] = c.last_energy;
+    out8[1] = (double)c.stat_contacts;
+    out8[2] = (double)c.stat_newton;
+    out8[3] = (double)c.stat_cg;
+    out8[4] = (double)c.stat_ls;
+    out8[5] = (double)c.radius;
+    out8[6] = c.do_nothing ? 1.0 : 0.0;
+    out8[7] = 0.0;
+    return 0;
+}
+
+}  // extern "C"

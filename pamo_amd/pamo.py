@@ -136,8 +136,8 @@ class PaMO(torch.nn.Module):
             V, F = input_mesh
         else:
             V, F = input_mesh.vertices, input_mesh.faces
-        self.gt_vertices = np.asarray(V, dtype=np.float64).copy()
-        self.gt_faces = np.asarray(F, dtype=np.int64).copy()
+        self.gt_vertices = np.asarray(V.cpu() if isinstance(V, torch.Tensor) else V, dtype=np.float64).copy()
+        self.gt_faces = np.asarray(F.cpu() if isinstance(F, torch.Tensor) else F, dtype=np.int64).copy()
         bb = getattr(input_mesh, "bounding_box", None)
         self.bbox = bb.bounds if bb is not None else _Bounds(self.gt_vertices).bounds
         self.ctx = _hip.Context(torch.cuda.current_device())
@@ -145,6 +145,9 @@ class PaMO(torch.nn.Module):
             from . import stage3
             self.config = stage3.Stage3Config()
             self.system = stage3.Stage3System(self.config)
+            # the input surface stays resident on the device for stage 3
+            self.gt_v_dev = torch.as_tensor(self.gt_vertices, dtype=torch.float32).cuda()
+            self.gt_f_dev = torch.as_tensor(self.gt_faces, dtype=torch.int32).cuda()
         self.target_faces = None
         self.timings = {}
 
@@ -160,13 +163,16 @@ class PaMO(torch.nn.Module):
         t1 = time.perf_counter()
         self.rounds = len(hist)
         self.face_hist = hist
-        verts_np = v2.cpu().numpy() + tris_mean
-        faces_np = f2.cpu().numpy()
         self.timings = {"stage2_s": t1 - t0}
         if self.use_stage3:
+            # pamo/__init__.py:191-203: the stage-2 result (+ centring offset,
+            # a float32 add) is projected back onto the input surface
             from . import stage3
-            t2 = time.perf_counter()
-            verts_np, faces_np = stage3.process(self.gt_vertices, self.gt_faces, verts_np, faces_np, 5,
-                                                system=self.system, config=self.config)
-            self.timings["stage3_s"] = time.perf_counter() - t2
+            v3 = v2 + torch.from_numpy(tris_mean).to(v2.device)
+            verts_np, faces_np = stage3.process(self.gt_v_dev, self.gt_f_dev, v3, f2, 5, system=self.system,
+                                                config=self.config)
+            self.timings["stage3_s"] = time.perf_counter() - t1
+            return verts_np, faces_np.astype(np.int64)
+        verts_np = v2.cpu().numpy() + tris_mean
+        faces_np = f2.cpu().numpy()
         return verts_np, faces_np
