@@ -4,11 +4,12 @@ Workload (synthetic): marching-tetrahedra contour of the seed-1 union of 16
 rotated ellipsoids and 4 boxes on [-1,1]^3 at grid 296 (2,006,528 faces,
 1,003,254 vertices, fp32 positions, int32 faces, watertight and manifold),
 simplified with ratio 0.01, tolerance 4, threshold 1e-3, iter 100000,
-min_verts 0, as PaMO(mesh, use_stage1=False, ...).run(...).
+min_verts 0, as PaMO(mesh, use_stage1=False, use_stage3=True).run(...).
 
-One step = one full run from device-resident input arrays to device output
-arrays (centring, every stage-2 round with its guard and undo loop, and
-stage 3 when enabled).  Prints one JSON line.
+One step = one full PaMO.run from device-resident input arrays to the numpy
+result the reference returns: centring, every stage-2 round with its guard
+and undo loop, and stage 3 (five steps of ten Newton iterations against the
+2M-face input surface).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
-    p.add_argument("--stage3", type=int, default=-1, help="1/0 force stage 3 on/off (default: on if available)")
+    p.add_argument("--stage3", type=int, default=1, help="1/0: stage 3 on/off")
     return p.parse_args()
 
 
@@ -42,7 +43,7 @@ def main():
     import numpy as np
     import torch
     from pamo_amd import synth, _hip
-    from pamo_amd.pamo import face_centroid_mean, simplify_device
+    from pamo_amd.pamo import PaMO, face_centroid_mean
 
     ratio, tol, thr, iters = 0.01, 4, 1e-3, 100000
     V, F = synth.headline_mesh(1)
@@ -50,13 +51,14 @@ def main():
     dev = torch.device("cuda", 0)
     Vd = torch.from_numpy(V).to(dev)
     Fd = torch.from_numpy(F).to(dev)
-    ctx = _hip.Context(0)
+    use_s3 = bool(args.stage3)
+    pm = PaMO((V, F), use_stage1=False, use_stage3=use_s3)
+    ctx = pm.ctx
     prof = _hip.Profiler(ctx)
 
     def one_run():
-        mean = face_centroid_mean(ctx, Vd, Fd)
-        verts = (Vd - torch.from_numpy(mean).to(dev)).contiguous()
-        return simplify_device(ctx, verts, Fd, ratio, tol, thr, iters, 0)
+        vo, fo = pm.run(Vd, Fd, ratio, tol, thr, iters, 0)
+        return vo, fo, pm.face_hist
 
     for _ in range(args.warmup):
         one_run()
@@ -74,8 +76,11 @@ def main():
     prof.enable(True, kernels=[dom_name])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    s2, s3 = 0.0, 0.0
     for _ in range(args.steps):
         vo, fo, hist = one_run()
+        s2 += pm.timings["stage2_s"]
+        s3 += pm.timings.get("stage3_s", 0.0)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof.enable(False)
@@ -120,7 +125,7 @@ def main():
                          f"same workload in {t_cpu:.2f}s, scaled by face-rounds"}
 
     out = {
-        "metric": "2M->20k faces wall-time (stage 2)",
+        "metric": "2M->20k faces wall-time (stages 2+3)" if use_s3 else "2M->20k faces wall-time (stage 2)",
         "value": round(per_step, 4),
         "unit": "s",
         "n_gpus": 1,
@@ -134,7 +139,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ellipsoid/box union, marching tetrahedra grid 296", "faces_in": nF0,
                    "verts_in": nV0, "faces_out": int(len(fo)), "ratio": ratio, "tolerance": tol,
-                   "threshold": thr, "rounds": int(len(hist)), "stage3": False},
+                   "threshold": thr, "rounds": int(len(hist)), "stage3": use_s3},
+        "stage2_s": round(s2 / args.steps, 4),
+        "stage3_s": round(s3 / args.steps, 4),
         "faces_per_s": round(nF0 / per_step, 1),
         "face_rounds": face_rounds,
         "roofline": roofline,

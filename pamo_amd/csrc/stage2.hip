@@ -337,6 +337,9 @@ static int run_guard(Ctx& c, int nT) {
                        (const int*)c.dcnt, (const float4*)c.bbox + 0, (const float4*)c.bpre, (const float4*)c.bsuf,
                        (const float4*)c.blk, (const float4*)c.blkpre, (const float4*)c.blksuf,
                        (const float4*)c.sblk, c.bbox);
+    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * nT, kBlock)), dim3(kBlock), 0, c.stream, (const int*)c.bleft,
+                       (const int*)c.bright, (const int*)c.bparent, (const int*)c.dcnt, (const uint32_t*)c.bval,
+                       c.bbox);
     prof_end(K_BVH_BOXES, c.stream);
     PAMO_HIP(hipGetLastError());
     // queries: changed live faces in Morton order
@@ -344,8 +347,7 @@ static int run_guard(Ctx& c, int nT) {
     PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
     LAUNCHP(K_GUARD_QUERY, 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT,
-            guard_query_k, nT, m, c.qlist, c.dcnt, c.changed, c.fbox, c.bbox, c.bleft, c.bright, c.bparent, c.bval,
-            1e-3f, c.flagged);
+            guard_query_k, nT, m, c.qlist, c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, 1e-3f, c.flagged);
     return 0;
 }
 

@@ -806,6 +806,33 @@ __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ ra
 }
 
 // ---------------------------------------------------------------- guard (A12, A13)
+// Escape ("rope") links: node records become two float4 (lo.xyz, left child
+// or, for a leaf, its item id in lo.w; hi.xyz, escape node in hi.w).  The
+// escape of x is the next node in depth-first order once x's subtree is
+// done: the right sibling of the first ancestor-or-self that is a left
+// child, -1 past the end.  Traversal then needs one 32-byte record per step
+// and no stack or parent walk.
+__global__ void __launch_bounds__(kBlock) rope_k(const int* __restrict__ left, const int* __restrict__ right,
+                                                 const int* __restrict__ parent, const int* __restrict__ cnt,
+                                                 const uint32_t* __restrict__ val, float4* __restrict__ node) {
+    const int n = cnt[C_NLIVE];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n <= 0 || x >= 2 * n - 1) return;
+    int esc = -1;
+    for (int y = x; y != 0;) {
+        const int p = parent[y];
+        if (left[p] == y) {
+            esc = right[p];
+            break;
+        }
+        y = p;
+    }
+    const int lw = x >= n - 1 ? (int)val[x - (n - 1)] : left[x];
+    float* r = reinterpret_cast<float*>(node + 2 * (long long)x);
+    r[3] = __int_as_float(lw);
+    r[7] = __int_as_float(esc);
+}
+
 __device__ __forceinline__ bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
     if (ahi.x < blo.x || bhi.x < alo.x) return false;
     if (ahi.y < blo.y || bhi.y < alo.y) return false;
@@ -874,14 +901,21 @@ constexpr int kQueue = 128;  // per-wave pair queue (< 64 pending + one append o
 // lanes that reach a candidate append it to a per-wave LDS queue, and the
 // whole wave evaluates the FP64 predicates 64 pairs at a time, so the long
 // predicate code runs with every lane busy instead of one lane at a time.
+// Candidate pairs: every queried face a against every live face b != a whose
+// box overlaps a's box (closed intervals).  A pair of queried faces is taken
+// from the larger index only, and pairs sharing two vertex indices are
+// dropped before any arithmetic (the reference rejects them after its
+// coplanarity test in every outcome).
+//
+// Each lane walks the rope-linked tree for its own query, one node per loop
+// iteration; candidates go to a per-wave LDS queue and the whole wave
+// evaluates the FP64 predicates 64 pairs at a time, so the long predicate
+// code runs with every lane busy and no lane waits for another's walk.
 __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
                                                         const int* __restrict__ cnt,
                                                         const uint8_t* __restrict__ changed,
                                                         const float4* __restrict__ fbox,
-                                                        const float4* __restrict__ box,
-                                                        const int* __restrict__ left, const int* __restrict__ right,
-                                                        const int* __restrict__ parent,
-                                                        const uint32_t* __restrict__ val, float epsilon,
+                                                        const float4* __restrict__ node, float epsilon,
                                                         uint8_t* __restrict__ flagged) {
     __shared__ int qa[kBlock / kWave][kQueue], qb[kBlock / kWave][kQueue];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
@@ -898,40 +932,30 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
         qlo = fbox[2 * (long long)a];
         qhi = fbox[2 * (long long)a + 1];
     }
-    int cur = 0, prev = -1;
-    int qn = 0;  // wave-uniform queue length
+    int x = 0;      // root
+    int qn = 0;     // wave-uniform queue length
     while (true) {
-        // advance to the next accepted candidate leaf (or finish)
         int found = -1;
-        while (active && found < 0) {
-            if (cur < 0) {
-                active = false;
-                break;
-            }
-            const int par = cur == 0 ? -1 : parent[cur];
-            int next;
-            if (prev == par) {
-                if (!overlap(qlo, qhi, box[2 * (long long)cur], box[2 * (long long)cur + 1])) {
-                    next = par;
-                } else if (cur >= n - 1) {
-                    const int b = (int)val[cur - (n - 1)];
-                    next = par;
+        if (active) {
+            const float4 lo = node[2 * (long long)x], hi = node[2 * (long long)x + 1];
+            const int lw = __float_as_int(lo.w), esc = __float_as_int(hi.w);
+            if (overlap(qlo, qhi, lo, hi)) {
+                if (x >= n - 1) {
+                    const int b = lw;
                     if (b != a && !(changed[b] && b < a)) {
                         const int b0 = m.i[b], b1 = m.j[b], b2 = m.k[b];
                         const int sh = (ia0 == b0) + (ia0 == b1) + (ia0 == b2) + (ia1 == b0) + (ia1 == b1) +
                                        (ia1 == b2) + (ia2 == b0) + (ia2 == b1) + (ia2 == b2);
                         if (sh != 2) found = b;
                     }
+                    x = esc;
                 } else {
-                    next = left[cur];
+                    x = lw;
                 }
-            } else if (prev == left[cur]) {
-                next = right[cur];
             } else {
-                next = par;
+                x = esc;
             }
-            prev = cur;
-            cur = next;
+            if (x < 0) active = false;
         }
         const uint64_t fm = __ballot(found >= 0);
         if (found >= 0) {
@@ -941,11 +965,13 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
         }
         qn += (int)__popcll(fm);
         const bool any_active = __ballot(active) != 0;
-        wave_sync();
-        while (qn >= kWave || (!any_active && qn > 0)) {
-            const int take = qn < kWave ? qn : kWave;
-            if (lane < take) eval_pair(m, qa[w][qn - take + lane], qb[w][qn - take + lane], epsilon, flagged);
-            qn -= take;
+        if (qn >= kWave || (!any_active && qn > 0)) {
+            wave_sync();
+            while (qn >= kWave || (!any_active && qn > 0)) {
+                const int take = qn < kWave ? qn : kWave;
+                if (lane < take) eval_pair(m, qa[w][qn - take + lane], qb[w][qn - take + lane], epsilon, flagged);
+                qn -= take;
+            }
             wave_sync();
         }
         if (!any_active) break;

@@ -132,6 +132,8 @@ static int tree_build(Tree& t, int n, int* ws, hipStream_t st) {
     hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
                        (const float4*)t.box, (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blk,
                        (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
+    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * n, kBlock)), dim3(kBlock), 0, st, (const int*)t.left,
+                       (const int*)t.right, (const int*)t.parent, (const int*)t.cnt, (const uint32_t*)t.val, t.box);
     PAMO_HIP(hipGetLastError());
     return 0;
 }
@@ -206,6 +208,20 @@ struct Ctx {
     float *area = nullptr, *rest_angle = nullptr, *rest_elen = nullptr;
     float4* inv_dm = nullptr;
     float* hblock = nullptr;  // 144 per hinge (4x4 blocks of 3x3, row-major)
+    float* hsym = nullptr;    // 45 per hinge: projected 9x9, packed upper triangle
+    // incidence lists (sorted by element) and per-slot HVP outputs
+    int *vf_off = nullptr, *vf_ent = nullptr, *vh_off = nullptr, *vh_ent = nullptr;
+    int *vc_off = nullptr, *vc_ent = nullptr;
+    float4 *fc = nullptr, *hc = nullptr;
+    float* dn = nullptr;
+    // active contacts (d < d_hat) of the current Newton iteration
+    int *act = nullptr, *act_flag = nullptr, *act_pos = nullptr, *n_act = nullptr;
+    float* sA = nullptr;
+    int nA = 0;
+    uint32_t *skey = nullptr, *sval = nullptr, *skey2 = nullptr, *sval2 = nullptr;
+    long long cap_sort = 0;
+    // CG partial sums: z.r double-buffered, v.Av
+    double *zr_part = nullptr, *vav_part = nullptr;
     // GT
     float4* gt_v = nullptr;
     int4* gt_f = nullptr;
@@ -240,7 +256,9 @@ static void ctx_free(Ctx& c) {
     void* ptrs[] = {c.q, c.qn, c.qd, c.qrest, c.p, c.target, c.grad, c.diag, c.r, c.v, c.Av, c.z, c.vor, c.tri,
                     c.edge, c.hinge, c.area, c.rest_angle, c.rest_elen, c.inv_dm, c.hblock, c.gt_v, c.gt_f,
                     c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
-                    c.dense_A, c.dense_b};
+                    c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
+                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.sA, c.skey, c.sval, c.skey2,
+                    c.sval2, c.zr_part, c.vav_part};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcount) (void)hipHostFree(c.hcount);
@@ -290,18 +308,6 @@ __global__ void __launch_bounds__(kB) finish_sum_k(const double* __restrict__ pa
     if (threadIdx.x == 0) out[slot] = s;
 }
 
-__global__ void dot_part_k(const float4* __restrict__ a, const float4* __restrict__ b, int n,
-                           double* __restrict__ part) {
-    __shared__ double red[kB];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double x = 0.0;
-    if (i < n) {
-        float4 u = a[i], w = b[i];
-        x = (double)u.x * (double)w.x + (double)u.y * (double)w.y + (double)u.z * (double)w.z;
-    }
-    double s = block_sum(x, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
 
 // ------------------------------------------------------------ preprocessing kernels
 // elastic_preprocess_kernel (elastic_energy.py:8-38)
@@ -424,55 +430,111 @@ __global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__
 }
 
 // ------------------------------------------------------------ contact detection (B5)
-__global__ void __launch_bounds__(kB) detect_pt_k(const float4* __restrict__ x, int nV, const int4* __restrict__ tri,
-                                                  TreeView T, float d_hat, float radius, int* counter, int cap,
-                                                  int4* __restrict__ cidx, int* __restrict__ ctype1) {
-    int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i0 >= nV) return;
-    const V3 x0 = v3(x[i0]);
-    const float thr = 2.0f * radius + d_hat;
-    const V3 lo = x0 - mk(thr, thr, thr), hi = x0 + mk(thr, thr, thr);
-    tree_query(T, lo, hi, [&](int f) {
-        int4 t = tri[f];
-        if (i0 == t.x || i0 == t.y || i0 == t.z) return;
-        V3 x1 = v3(x[t.x]), x2 = v3(x[t.y]), x3 = v3(x[t.z]);
-        int ty = pt_classify(x0, x1, x2, x3);
-        float d = pt_distance(x0, x1, x2, x3, ty);
-        if (d <= thr) {
-            int bid = atomicAdd(counter, 1);
-            if (bid < cap) {
-                cidx[bid] = make_int4(i0, t.x, t.y, t.z);
-                ctype1[bid] = BLOCK_PT;
-            }
-        }
-    });
-}
+// detect_pt_contact_bvh_kernel / detect_ee_contact_bvh_kernel
+// (contact_detection.py:119-216): PT = vertex i0 against every triangle not
+// holding it whose box overlaps x0 +- (2r + d_hat); EE = edge e0 against every
+// edge e1 > e0 sharing no vertex whose box overlaps e0's box grown by the same
+// amount; a pair is kept when its classified distance is <= 2r + d_hat.
+//
+// Each lane walks the rope-linked tree for its query one node per loop
+// iteration and queues box hits in LDS; the wave classifies 64 queued pairs
+// at a time and reserves output slots with one atomic per wave.
+constexpr int kDetQ = 128;
 
-__global__ void __launch_bounds__(kB) detect_ee_k(const float4* __restrict__ x, int nE, const int2* __restrict__ edge,
-                                                  TreeView T, float d_hat, float radius, float ee_thres,
-                                                  int* counter, int cap, int4* __restrict__ cidx,
-                                                  int* __restrict__ ctype1) {
-    int e0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e0 >= nE) return;
-    const int2 a = edge[e0];
-    const V3 x0 = v3(x[a.x]), x1 = v3(x[a.y]);
+template <bool kPT>
+__global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int nq, const int4* __restrict__ tri,
+                                               const int2* __restrict__ edge, const float4* __restrict__ node,
+                                               int n_leaves, float d_hat, float radius, float ee_thres,
+                                               int* counter, int cap, int4* __restrict__ cidx,
+                                               int* __restrict__ ctype1) {
+    __shared__ int qa[kB / kWave][kDetQ], qb[kB / kWave][kDetQ];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (__ballot(q < nq) == 0 || n_leaves <= 0) return;  // uniform per wave
+    bool active = q < nq;
     const float thr = 2.0f * radius + d_hat;
-    const V3 lo = vmin(x0, x1) - mk(thr, thr, thr), hi = vmax(x0, x1) + mk(thr, thr, thr);
-    tree_query(T, lo, hi, [&](int e1) {
-        if (!(e0 < e1)) return;
-        int2 b = edge[e1];
-        if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
-        V3 x2 = v3(x[b.x]), x3 = v3(x[b.y]);
-        int ty = ee_classify(x0, x1, x2, x3, ee_thres);
-        float d = ee_distance(x0, x1, x2, x3, ty);
-        if (d <= thr) {
-            int bid = atomicAdd(counter, 1);
-            if (bid < cap) {
-                cidx[bid] = make_int4(a.x, a.y, b.x, b.y);
-                ctype1[bid] = BLOCK_EE;
-            }
+    V3 lo = mk(0.f, 0.f, 0.f), hi = lo;
+    if (active) {
+        if (kPT) {
+            const V3 x0 = v3(x[q]);
+            lo = x0 - mk(thr, thr, thr);
+            hi = x0 + mk(thr, thr, thr);
+        } else {
+            const int2 e = edge[q];
+            const V3 a = v3(x[e.x]), b = v3(x[e.y]);
+            lo = vmin(a, b) - mk(thr, thr, thr);
+            hi = vmax(a, b) + mk(thr, thr, thr);
         }
-    });
+    }
+    int cur = 0, qn = 0;
+    while (true) {
+        int found = -1;
+        if (active) {
+            const float4 blo = node[2 * (long long)cur], bhi = node[2 * (long long)cur + 1];
+            const int lw = __float_as_int(blo.w), esc = __float_as_int(bhi.w);
+            if (box_hit(lo, hi, blo, bhi)) {
+                if (cur >= n_leaves - 1) {
+                    if (kPT || lw > q) found = lw;
+                    cur = esc;
+                } else {
+                    cur = lw;
+                }
+            } else {
+                cur = esc;
+            }
+            if (cur < 0) active = false;
+        }
+        const uint64_t fm = __ballot(found >= 0);
+        if (found >= 0) {
+            const int slot = qn + lanes_below(fm);
+            qa[w][slot] = q;
+            qb[w][slot] = found;
+        }
+        qn += (int)__popcll(fm);
+        const bool any_active = __ballot(active) != 0;
+        if (qn >= kWave || (!any_active && qn > 0)) {
+            wave_sync();
+            while (qn >= kWave || (!any_active && qn > 0)) {
+                const int take = qn < kWave ? qn : kWave;
+                bool keep = false;
+                int4 ids = make_int4(0, 0, 0, 0);
+                if (lane < take) {
+                    const int qa_ = qa[w][qn - take + lane], qb_ = qb[w][qn - take + lane];
+                    if (kPT) {
+                        const int4 t = tri[qb_];
+                        if (qa_ != t.x && qa_ != t.y && qa_ != t.z) {
+                            const V3 x0 = v3(x[qa_]), x1 = v3(x[t.x]), x2 = v3(x[t.y]), x3 = v3(x[t.z]);
+                            const int ty = pt_classify(x0, x1, x2, x3);
+                            keep = pt_distance(x0, x1, x2, x3, ty) <= thr;
+                            ids = make_int4(qa_, t.x, t.y, t.z);
+                        }
+                    } else {
+                        const int2 e0 = edge[qa_], e1 = edge[qb_];
+                        if (e0.x != e1.x && e0.x != e1.y && e0.y != e1.x && e0.y != e1.y) {
+                            const V3 x0 = v3(x[e0.x]), x1 = v3(x[e0.y]), x2 = v3(x[e1.x]), x3 = v3(x[e1.y]);
+                            const int ty = ee_classify(x0, x1, x2, x3, ee_thres);
+                            keep = ee_distance(x0, x1, x2, x3, ty) <= thr;
+                            ids = make_int4(e0.x, e0.y, e1.x, e1.y);
+                        }
+                    }
+                }
+                const uint64_t km = __ballot(keep);
+                int base = 0;
+                if (lane == 0 && km) base = atomicAdd(counter, (int)__popcll(km));
+                base = __shfl(base, 0);
+                if (keep) {
+                    const int slot = base + lanes_below(km);
+                    if (slot < cap) {
+                        cidx[slot] = ids;
+                        ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
+                    }
+                }
+                qn -= take;
+            }
+            wave_sync();
+        }
+        if (!any_active) break;
+    }
 }
 
 // ------------------------------------------------------------ energies (B7)
@@ -572,7 +634,10 @@ __device__ __forceinline__ void atomic_add3(float4* a, int i, V3 v) {
 // Symmetric eigen-clamp of the 9x9 leading block (cyclic Jacobi), then rows /
 // columns of vertex 3 rebuilt by translation invariance (sum over a block
 // row / column vanishes), as block_spd_project_kernel (utils_kernels.py:21-39).
-__device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row-major 3x3 */) {
+// Also writes the projected 9x9 as a packed upper triangle (45 floats,
+// row-major), which is all the Hessian-vector product needs.
+__device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row-major 3x3 */,
+                                  float* __restrict__ S45) {
     float A[9][9], Vm[9][9];
 #pragma unroll
     for (int r = 0; r < 9; ++r)
@@ -632,6 +697,7 @@ __device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row
 #pragma unroll
             for (int k = 0; k < 9; ++k) s += Vm[r][k] * lam[k] * Vm[c][k];
             B[9 * (4 * (r / 3) + (c / 3)) + 3 * (r % 3) + (c % 3)] = s;
+            if (c >= r) S45[9 * r - (r * (r - 1)) / 2 + (c - r)] = s;
         }
     // row 3: block[3][j] = -(block[0][j] + block[1][j] + block[2][j]) for j < 3
     for (int j = 0; j < 3; ++j)
@@ -666,7 +732,8 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
                                              int nC, float coeff, const float* __restrict__ cd,
                                              const int* __restrict__ ctype2, float4* __restrict__ grad,
                                              float4* __restrict__ diag, float* __restrict__ hblock,
-                                             float4* __restrict__ cdd) {
+                                             float* __restrict__ hsym, const int* __restrict__ act,
+                                             const int* __restrict__ n_act, float4* __restrict__ cdd) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nV) {
         float w = tp.vor[i] * P.k_m2gt;
@@ -755,10 +822,13 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
             const float* D = B + 9 * (4 * a + a);
             atomic_add3(diag, ids[a], mk(D[0], D[4], D[8]));
         }
-        spd_project_hinge(B);
-    } else if ((i -= nH) < nC) {
+        spd_project_hinge(B, hsym + 45 * (long long)i);
+    } else if ((i -= nH) < nC && i < n_act[0]) {
+        // active contacts only (d < d_hat); their distance gradients are kept
+        // in list order for the Hessian-vector products
+        const long long j = i;
+        i = act[j];
         float d = cd[i];
-        if (d >= P.d_hat) return;
         int4 k = tp.cidx[i];
         V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
         V3 g[4];
@@ -768,26 +838,67 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         float c2 = barrier_d2(d, P.d_hat) * P.kappa;
         const int ids[4] = {k.x, k.y, k.z, k.w};
         for (int a = 0; a < 4; ++a) {
-            cdd[4 * i + a] = f4(g[a]);
+            cdd[4 * j + a] = f4(g[a]);
             atomic_add3(grad, ids[a], g[a] * c1);
             atomic_add3(diag, ids[a], c2 * mk(g[a].x * g[a].x, g[a].y * g[a].y, g[a].z * g[a].z));
         }
     }
 }
 
-// Hessian-vector product H dx (Gauss-Newton for contacts).
-__global__ void __launch_bounds__(kB) hvp_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF, int nH,
-                                            int nC, const float* __restrict__ cd, const float* __restrict__ hblock,
-                                            const float4* __restrict__ cdd, const float4* __restrict__ dx,
-                                            float4* __restrict__ out) {
+// ------------------------------------------------------------ Hessian-vector products and PCG (B7, B8)
+// The product y = H dx is formed without atomics in two passes: per element
+// (membrane face, hinge, active contact, dense row) the contribution to each
+// of its vertices is written to its own slot; then per vertex the slots of
+// its incident elements (incidence lists sorted by element) are summed in a
+// fixed order, with the M2GT term w dx added.  The CG direction is formed on
+// the fly, dx = z + beta v, beta = zr_k / zr_{k-1} re-reduced inside every
+// block from the FP64 partial sums (fixed order), so one CG iteration is
+// three launches: elements, gather (+ v . Av partials), update (+ z . r
+// partials).
+
+// sum of part[0..n) in a fixed order; every thread of the block returns it
+__device__ __forceinline__ double reduce_parts(const double* __restrict__ part, int n, double* red) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kB) acc += part[i];
+    return block_sum(acc, red);
+}
+
+struct CgVec {
+    const float4* z;      // current z
+    const float4* v_old;  // previous direction (unused when first)
+    const double* zr_new; // partials of z_k . r_k
+    const double* zr_old; // partials of z_{k-1} . r_{k-1}
+    int nparts;
+    int first;            // dx = z (k = 0, or a plain product)
+};
+
+__device__ __forceinline__ float cg_beta(const CgVec& cv, double* red) {
+    if (cv.first) return 0.0f;
+    const double a = reduce_parts(cv.zr_new, cv.nparts, red);
+    const double b = reduce_parts(cv.zr_old, cv.nparts, red);
+    return (float)(a / b);
+}
+__device__ __forceinline__ V3 cg_dx(const CgVec& cv, float beta, int v) {
+    return cv.first ? v3(cv.z[v]) : v3(cv.z[v]) + beta * v3(cv.v_old[v]);
+}
+
+// items: [0, nF) faces -> fc[3f + k]; [nF, +nH) hinges -> hc[4h + a];
+// [.., +nA) active contacts -> sA[j] = c2 (g . dx); [.., +3 nV_dense) dense rows
+__global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, Topo tp, Params P, int nF, int nH,
+                                                 int nA_max, const int* __restrict__ n_act, int n_dense,
+                                                 CgVec cv, const float* __restrict__ cd, const int* __restrict__ act,
+                                                 const float* __restrict__ hsym, const float4* __restrict__ cdd,
+                                                 const double* __restrict__ dA, float4* __restrict__ fc,
+                                                 float4* __restrict__ hc, float* __restrict__ sA,
+                                                 float* __restrict__ dn) {
+    __shared__ double red[kB];
+    const float beta = cg_beta(cv, red);
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nV) {
-        float w = tp.vor[i] * P.k_m2gt;
-        atomic_add3(out, (int)i, v3(dx[i]) * w);
-    } else if ((i -= nV) < nF) {
+    if (i < nF) {
+        // elastic_hess_dx_kernel (elastic_energy.py:149-220), local basis fixed
         int4 f = tp.tri[i];
         V3 x0 = v3(x[f.x]), x1 = v3(x[f.y]), x2 = v3(x[f.z]);
-        V3 d0 = v3(dx[f.x]), d1 = v3(dx[f.y]), d2 = v3(dx[f.z]);
+        V3 d0 = cg_dx(cv, beta, f.x), d1 = cg_dx(cv, beta, f.y), d2 = cg_dx(cv, beta, f.z);
         float4 idm = tp.inv_dm[i];
         Membrane m = membrane(x0, x1, x2, idm);
         V3 dx01 = d1 - d0, dx02 = d2 - d0;
@@ -795,7 +906,6 @@ __global__ void __launch_bounds__(kB) hvp_k(const float4* __restrict__ x, Topo t
         float Dm[4] = {idm.x, idm.y, idm.z, idm.w};
         float dF[4];
         mat22_mul(dDs, Dm, dF);
-        // dE = 0.5 (dF^T F + F^T dF)
         float a[4] = {dF[0] * m.F[0] + dF[2] * m.F[2], dF[0] * m.F[1] + dF[2] * m.F[3],
                       dF[1] * m.F[0] + dF[3] * m.F[2], dF[1] * m.F[1] + dF[3] * m.F[3]};
         float b[4] = {m.F[0] * dF[0] + m.F[2] * dF[2], m.F[0] * dF[1] + m.F[2] * dF[3],
@@ -818,48 +928,99 @@ __global__ void __launch_bounds__(kB) hvp_k(const float4* __restrict__ x, Topo t
         for (int k = 0; k < 4; ++k) dH[k] *= W;
         V3 df1 = dH[0] * m.b1 + dH[2] * m.b2;
         V3 df2 = dH[1] * m.b1 + dH[3] * m.b2;
-        V3 df0 = -df1 - df2;
-        atomic_add3(out, f.x, df0);
-        atomic_add3(out, f.y, df1);
-        atomic_add3(out, f.z, df2);
+        fc[3 * i] = f4(-df1 - df2);
+        fc[3 * i + 1] = f4(df1);
+        fc[3 * i + 2] = f4(df2);
     } else if ((i -= nF) < nH) {
+        // projected hinge Hessian: rows / columns of vertex 3 follow from the
+        // 9x9 block by translation invariance, so y = A (d_a - d_3) and the
+        // fourth vertex receives -(y_0 + y_1 + y_2)
         int4 k = tp.hinge[i];
-        const int ids[4] = {k.x, k.y, k.z, k.w};
-        V3 xs[4];
-        for (int a = 0; a < 4; ++a) xs[a] = v3(dx[ids[a]]);
-        const float* B = hblock + 144 * (long long)i;
-        for (int a = 0; a < 4; ++a) {
-            V3 acc = mk(0.0f, 0.0f, 0.0f);
-            for (int bb = 0; bb < 4; ++bb) {
-                const float* M = B + 9 * (4 * a + bb);
-                V3 w = xs[bb];
-                acc = acc + mk(M[0] * w.x + M[1] * w.y + M[2] * w.z, M[3] * w.x + M[4] * w.y + M[5] * w.z,
-                               M[6] * w.x + M[7] * w.y + M[8] * w.z);
+        V3 d3 = cg_dx(cv, beta, k.w);
+        V3 e[3] = {cg_dx(cv, beta, k.x) - d3, cg_dx(cv, beta, k.y) - d3, cg_dx(cv, beta, k.z) - d3};
+        float u[9] = {e[0].x, e[0].y, e[0].z, e[1].x, e[1].y, e[1].z, e[2].x, e[2].y, e[2].z};
+        const float* S = hsym + 45 * i;
+        float y[9];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) y[r] = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int c = r; c < 9; ++c) {
+                const float a = S[9 * r - (r * (r - 1)) / 2 + (c - r)];
+                y[r] += a * u[c];
+                if (c != r) y[c] += a * u[r];
             }
-            atomic_add3(out, ids[a], acc);
+        V3 y0 = mk(y[0], y[1], y[2]), y1 = mk(y[3], y[4], y[5]), y2 = mk(y[6], y[7], y[8]);
+        hc[4 * i] = f4(y0);
+        hc[4 * i + 1] = f4(y1);
+        hc[4 * i + 2] = f4(y2);
+        hc[4 * i + 3] = f4(-(y0 + y1 + y2));
+    } else if ((i -= nH) < nA_max) {
+        if (i >= n_act[0]) return;
+        const int c = act[i];
+        const int4 k = tp.cidx[c];
+        const float c2 = barrier_d2(cd[c], P.d_hat) * P.kappa;
+        float sdot = dot(v3(cdd[4 * i]), cg_dx(cv, beta, k.x)) + dot(v3(cdd[4 * i + 1]), cg_dx(cv, beta, k.y)) +
+                     dot(v3(cdd[4 * i + 2]), cg_dx(cv, beta, k.z)) + dot(v3(cdd[4 * i + 3]), cg_dx(cv, beta, k.w));
+        sA[i] = sdot * c2;
+    } else if ((i -= nA_max) < n_dense) {
+        const int n3 = (int)n_dense;
+        double acc = 0.0;
+        for (int kk = 0; kk < n3; ++kk) {
+            const V3 d = cg_dx(cv, beta, kk / 3);
+            const float comp = kk % 3 == 0 ? d.x : (kk % 3 == 1 ? d.y : d.z);
+            acc += dA[i * n3 + kk] * (double)comp;
         }
-    } else if ((i -= nH) < nC) {
-        float d = cd[i];
-        if (d >= P.d_hat) return;
-        int4 k = tp.cidx[i];
-        const int ids[4] = {k.x, k.y, k.z, k.w};
-        float c2 = barrier_d2(d, P.d_hat) * P.kappa;
-        V3 g[4];
-        float s = 0.0f;
-        for (int a = 0; a < 4; ++a) {
-            g[a] = v3(cdd[4 * i + a]);
-            s += dot(g[a], v3(dx[ids[a]]));
-        }
-        for (int a = 0; a < 4; ++a) atomic_add3(out, ids[a], g[a] * (s * c2));
+        dn[i] = (float)acc;
     }
 }
 
-// ------------------------------------------------------------ PCG (B8)
-// z0 = r / (diag + 1e-6); v = z; zr partials (cg_solver.py:120-142)
+struct Incid {
+    const int *vf_off, *vf_ent;  // vertex -> face corner slots (3 f + k)
+    const int *vh_off, *vh_ent;  // vertex -> hinge slots (4 h + a)
+    const int *vc_off, *vc_ent;  // vertex -> active contact slots (4 j + a)
+};
+
+// Av = H dx per vertex; the direction dx is stored to v_out; block partials
+// of dx . Av to vav.
+__global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2gt, int use_elem, int use_hinge,
+                                                   int use_contact, int use_dense, const float* __restrict__ vor,
+                                                   CgVec cv, Incid ic, const float4* __restrict__ fc,
+                                                   const float4* __restrict__ hc, const float* __restrict__ sA,
+                                                   const float4* __restrict__ cdd, const float* __restrict__ dn,
+                                                   float4* __restrict__ v_out, float4* __restrict__ Av,
+                                                   double* __restrict__ vav) {
+    __shared__ double red[kB];
+    const float beta = cg_beta(cv, red);
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double dotp = 0.0;
+    if (v < nV) {
+        const V3 d = cg_dx(cv, beta, v);
+        V3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (use_m2gt) acc = acc + d * (vor[v] * P.k_m2gt);
+        if (use_elem)
+            for (int e = ic.vf_off[v]; e < ic.vf_off[v + 1]; ++e) acc = acc + v3(fc[ic.vf_ent[e]]);
+        if (use_hinge)
+            for (int e = ic.vh_off[v]; e < ic.vh_off[v + 1]; ++e) acc = acc + v3(hc[ic.vh_ent[e]]);
+        if (use_contact)
+            for (int e = ic.vc_off[v]; e < ic.vc_off[v + 1]; ++e) {
+                const int s = ic.vc_ent[e];
+                acc = acc + v3(cdd[s]) * sA[s >> 2];
+            }
+        if (use_dense) acc = acc + mk(dn[3 * v], dn[3 * v + 1], dn[3 * v + 2]);
+        Av[v] = f4(acc);
+        v_out[v] = f4(d);
+        dotp = (double)d.x * acc.x + (double)d.y * acc.y + (double)d.z * acc.z;
+    }
+    const double sum = block_sum(dotp, red);
+    if (threadIdx.x == 0) vav[blockIdx.x] = sum;
+}
+
+// r = grad; z = r / (diag + 1e-6); p = 0; partials of z . r (cg_solver.py:120-142)
 __global__ void __launch_bounds__(kB) cg_init_k(const float4* __restrict__ grad, const float4* __restrict__ diag,
                                                 int n, float4* __restrict__ r, float4* __restrict__ z,
-                                                float4* __restrict__ v, float4* __restrict__ p,
-                                                double* __restrict__ part) {
+                                                float4* __restrict__ p, double* __restrict__ part) {
     __shared__ double red[kB];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double x = 0.0;
@@ -868,7 +1029,6 @@ __global__ void __launch_bounds__(kB) cg_init_k(const float4* __restrict__ grad,
         V3 zz = mk(rr.x / (dg.x + 1e-6f), rr.y / (dg.y + 1e-6f), rr.z / (dg.z + 1e-6f));
         r[i] = f4(rr);
         z[i] = f4(zz);
-        v[i] = f4(zz);
         p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         x = (double)zz.x * rr.x + (double)zz.y * rr.y + (double)zz.z * rr.z;
     }
@@ -876,20 +1036,22 @@ __global__ void __launch_bounds__(kB) cg_init_k(const float4* __restrict__ grad,
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha = zr_last / vAv (if vAv > 1e-16); p += alpha v; r -= alpha Av;
-// z = r / diag; zr partials (solver_kernels.py:4-23)
+// alpha = zr_k / vAv (if vAv > 1e-16); p += alpha v; r -= alpha Av; z = r / diag;
+// partials of z . r (update_p_r_z_compute_zr_kernel, solver_kernels.py:4-23)
 __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, const float4* __restrict__ Av,
-                                                  const float4* __restrict__ diag, const double* __restrict__ sc,
-                                                  int n, float4* __restrict__ p, float4* __restrict__ r,
-                                                  float4* __restrict__ z, double* __restrict__ part) {
+                                                  const float4* __restrict__ diag, const double* __restrict__ vav,
+                                                  const double* __restrict__ zr_cur, int nparts, int n,
+                                                  float4* __restrict__ p, float4* __restrict__ r,
+                                                  float4* __restrict__ z, double* __restrict__ zr_next) {
     __shared__ double red[kB];
+    const double vAv = reduce_parts(vav, nparts, red);
+    const double zr = reduce_parts(zr_cur, nparts, red);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double x = 0.0;
     if (i < n) {
-        const double vAv = sc[SC_VAV];
         V3 rr = v3(r[i]);
         if (vAv > 1e-16) {
-            float alpha = (float)(sc[SC_ZRLAST] / vAv);
+            float alpha = (float)(zr / vAv);
             p[i] = f4(v3(p[i]) + alpha * v3(v[i]));
             rr = rr - alpha * v3(Av[i]);
             r[i] = f4(rr);
@@ -900,17 +1062,49 @@ __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, 
         x = (double)zz.x * rr.x + (double)zz.y * rr.y + (double)zz.z * rr.z;
     }
     double s = block_sum(x, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) zr_next[blockIdx.x] = s;
 }
 
-// v = z + (zr / zr_last) v; also clears Av for the next product.
-__global__ void cg_dir_k(const float4* __restrict__ z, const double* __restrict__ sc, int n, float4* __restrict__ v,
-                         float4* __restrict__ Av) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = (float)(sc[SC_ZR] / sc[SC_ZRLAST]);
-    v[i] = f4(v3(z[i]) + s * v3(v[i]));
-    Av[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// ---- incidence lists
+// keys[s] = vertex of slot s of element s / width, vals[s] = s
+__global__ void slot_keys_k(const int* __restrict__ elem, int width, int stride, int n_slots,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const int e = s / width, k = s % width;
+    keys[s] = (uint32_t)elem[(long long)e * stride + k];
+    vals[s] = (uint32_t)s;
+}
+// active contacts: slot s = 4 j + a of active contact act[j]
+__global__ void contact_keys_k(const int4* __restrict__ cidx, const int* __restrict__ act, const int* __restrict__ n_act,
+                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= 4 * n_act[0]) return;
+    const int4 k = cidx[act[s >> 2]];
+    const int a = s & 3;
+    keys[s] = (uint32_t)(a == 0 ? k.x : (a == 1 ? k.y : (a == 2 ? k.z : k.w)));
+    vals[s] = (uint32_t)s;
+}
+// off[v] = first position whose sorted key is >= v, for v in [0, nV]
+__global__ void slot_offsets_k(const uint32_t* __restrict__ keys, int n, int nV, int* __restrict__ off) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int lo = i == 0 ? 0 : (int)keys[i - 1] + 1;
+    const int hi = i == n ? nV : (int)keys[i];
+    for (int v = lo; v <= hi; ++v) off[v] = i;
+}
+// active contacts: cd < d_hat
+__global__ void act_flag_k(const float* __restrict__ cd, int nC, float d_hat, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nC) return;
+    flag[i] = (i < nC && cd[i] < d_hat) ? 1 : 0;
+}
+__global__ void act_list_k(const int* __restrict__ flag, const int* __restrict__ pos, int nC, int* __restrict__ act,
+                           int* __restrict__ n_act) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nC) return;
+    if (i == nC) n_act[0] = pos[nC];
+    else if (flag[i]) act[pos[i]] = i;
 }
 
 __global__ void copy_slot_k(double* sc, int from, int to) { sc[to] = sc[from]; }
@@ -1013,15 +1207,6 @@ __global__ void __launch_bounds__(kB) dense_diff_k(const float4* __restrict__ x,
     atomicAdd(&((float*)&diag[r / 3])[r % 3], (float)A[(long long)r * n3 + r]);
 }
 
-__global__ void __launch_bounds__(kB) dense_hvp_k(int nV, const double* __restrict__ A, const float4* __restrict__ dx,
-                                                  float4* __restrict__ out) {
-    const int n3 = 3 * nV;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n3) return;
-    double ax = 0.0;
-    for (int k = 0; k < n3; ++k) ax += A[(long long)r * n3 + k] * (double)((const float*)&dx[k / 3])[k % 3];
-    atomicAdd(&((float*)&out[r / 3])[r % 3], (float)ax);
-}
 
 // ------------------------------------------------------------ host orchestration
 #define L3(kern, n, ...)                                                                                  \
@@ -1061,35 +1246,93 @@ static int compute_energy(Ctx& c, int nC, int slot, int mask) {
 
 // grad (coefficient -1) and Jacobi diagonal; hinge blocks and contact
 // distance gradients are stored for the Hessian-vector products.
+static int ensure_sort(Ctx& c, long long n) {
+    if (n > c.cap_sort) {
+        c.cap_sort = n + n / 4 + 1024;
+        PAMO_TRY(dalloc(&c.skey, (size_t)c.cap_sort));
+        PAMO_TRY(dalloc(&c.sval, (size_t)c.cap_sort));
+        PAMO_TRY(dalloc(&c.skey2, (size_t)c.cap_sort));
+        PAMO_TRY(dalloc(&c.sval2, (size_t)c.cap_sort));
+    }
+    return ensure_ws(c, n + 64);
+}
+
+static int key_bits(int nV) {
+    int b = 8;
+    while (b < 32 && (nV - 1) >> b) b += 8;
+    return b;
+}
+
+// off (nV + 1) / ent: the slots of each vertex in ascending slot order, from
+// keys[s] = vertex of slot s (already in skey / sval).
+static int finish_incidence(Ctx& c, long long n, int* off, int* ent) {
+    PAMO_TRY(radix_sort_pairs(c.skey, c.sval, c.skey2, c.sval2, (int)n, key_bits(c.nV), c.ws, c.stream));
+    L3(slot_offsets_k, n + 1, (const uint32_t*)c.skey, (int)n, c.nV, off);
+    PAMO_HIP(hipMemcpyAsync(ent, c.sval, sizeof(int) * n, hipMemcpyDeviceToDevice, c.stream));
+    return 0;
+}
+
+static int build_incidence(Ctx& c, const int* elem, int n_elem, int width, int stride, int* off, int* ent) {
+    const long long n = (long long)n_elem * width;
+    if (n == 0) return fill(off, 0, c.nV + 1, c.stream);
+    PAMO_TRY(ensure_sort(c, n));
+    L3(slot_keys_k, n, elem, width, stride, (int)n, c.skey, c.sval);
+    return finish_incidence(c, n, off, ent);
+}
+
+// _compute_diff (system.py:334-339): grad (coefficient -1) and Jacobi
+// diagonal; projected hinge Hessians; the active contacts (d < d_hat) are
+// listed and their distance gradients kept, with a vertex -> contact slot
+// incidence list for the Hessian-vector products.
 static int compute_diff(Ctx& c, int nC, int mask) {
-    const Counts k = counts(c, nC, mask);
     PAMO_TRY(fill(c.grad, make_float4(0, 0, 0, 0), c.nV, c.stream));
     PAMO_TRY(fill(c.diag, make_float4(0, 0, 0, 0), c.nV, c.stream));
+    c.nA = 0;
+    if ((mask & PAMO_S3_E_COLLISION) && nC > 0) {
+        PAMO_TRY(ensure_ws(c, (long long)nC + 64));
+        L3(act_flag_k, nC + 1, (const float*)c.cd, nC, c.P.d_hat, c.act_flag);
+        PAMO_TRY(scan_exclusive(c.act_flag, c.act_pos, nC + 1, c.ws, c.stream));
+        L3(act_list_k, nC + 1, (const int*)c.act_flag, (const int*)c.act_pos, nC, c.act, c.n_act);
+        PAMO_HIP(hipMemcpyAsync(c.hcount, c.n_act, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        PAMO_HIP(hipStreamSynchronize(c.stream));
+        c.nA = c.hcount[0];
+    } else {
+        PAMO_TRY(fill(c.n_act, 0, 1, c.stream));
+    }
+    Counts k = counts(c, c.nA, mask);
     if (k.total() > 0)
         L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
-           (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.cdd);
+           (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.hsym, (const int*)c.act, (const int*)c.n_act, c.cdd);
     if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
         L3(dense_diff_k, 3 * c.nV, (const float4*)c.q, c.nV, (const double*)c.dense_A, (const double*)c.dense_b, -1.0f,
            c.grad, c.diag);
+    if (c.nA > 0) {
+        const long long n = 4LL * c.nA;
+        PAMO_TRY(ensure_sort(c, n));
+        L3(contact_keys_k, n, (const int4*)c.cidx, (const int*)c.act, (const int*)c.n_act, c.skey, c.sval);
+        PAMO_TRY(finish_incidence(c, n, c.vc_off, c.vc_ent));
+    } else {
+        PAMO_TRY(fill(c.vc_off, 0, c.nV + 1, c.stream));
+    }
     return 0;
 }
 
-// out += H dx (out must be cleared by the caller)
-static int compute_hvp(Ctx& c, int nC, int mask, const float4* dx, float4* out) {
-    const Counts k = counts(c, nC, mask);
-    if (k.total() > 0)
-        L3(hvp_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, (const float*)c.cd,
-           (const float*)c.hblock, (const float4*)c.cdd, dx, out);
-    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
-        L3(dense_hvp_k, 3 * c.nV, c.nV, (const double*)c.dense_A, dx, out);
-    return 0;
-}
-
-static int dot_into(Ctx& c, const float4* a, const float4* b, int slot) {
-    const unsigned nb = grid_for(c.nV, kB);
-    hipLaunchKernelGGL(dot_part_k, dim3(nb), dim3(kB), 0, c.stream, a, b, c.nV, c.part);
-    hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, slot);
-    PAMO_HIP(hipGetLastError());
+// Av = H dx with dx from cv (see hvp_elem_k); dx is also stored to v_out
+// and block partials of dx . Av to vav.
+static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4* Av, double* vav) {
+    const int nF = (mask & PAMO_S3_E_ELASTIC) ? c.nF : 0;
+    const int nH = (mask & PAMO_S3_E_HINGE) ? c.nH : 0;
+    const int nA = (mask & PAMO_S3_E_COLLISION) ? c.nA : 0;
+    const int nd = ((mask & PAMO_S3_E_DENSE) && c.dense_A) ? 3 * c.nV : 0;
+    const long long total = (long long)nF + nH + nA + nd;
+    if (total > 0)
+        L3(hvp_elem_k, total, (const float4*)c.q, topo(c), c.P, nF, nH, nA, (const int*)c.n_act, nd, cv,
+           (const float*)c.cd, (const int*)c.act, (const float*)c.hsym, (const float4*)c.cdd,
+           (const double*)c.dense_A, c.fc, c.hc, c.sA, c.dn);
+    Incid ic{c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent};
+    L3(hvp_gather_k, c.nV, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
+       (int)(nA > 0), (int)(nd > 0), (const float*)c.vor, cv, ic, (const float4*)c.fc, (const float4*)c.hc,
+       (const float*)c.sA, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
     return 0;
 }
 
@@ -1102,6 +1345,11 @@ static int ensure_contacts(Ctx& c, int n) {
     PAMO_TRY(dalloc(&c.ctype2, a));
     PAMO_TRY(dalloc(&c.cd, a));
     PAMO_TRY(dalloc(&c.cdd, 4 * (size_t)a));
+    PAMO_TRY(dalloc(&c.act, a));
+    PAMO_TRY(dalloc(&c.act_flag, (size_t)a + 1));
+    PAMO_TRY(dalloc(&c.act_pos, (size_t)a + 1));
+    PAMO_TRY(dalloc(&c.sA, a));
+    PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a));
     return 0;
 }
 
@@ -1120,10 +1368,12 @@ static int detect_contacts(Ctx& c, int& nC) {
         while (true) {
             PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
             const int cap = std::min(c.capC, c.P.max_blocks);
-            L3(detect_pt_k, c.nV, (const float4*)c.q, c.nV, (const int4*)c.tri, view(c.tri_tree), c.P.d_hat,
-               c.radius, c.ccount, cap, c.cidx, c.ctype1);
-            L3(detect_ee_k, c.nE, (const float4*)c.q, c.nE, (const int2*)c.edge, view(c.edge_tree), c.P.d_hat,
-               c.radius, c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1);
+            L3(detect_k<true>, c.nV, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
+               (const float4*)c.tri_tree.box, c.nF, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
+               c.ctype1);
+            L3(detect_k<false>, c.nE, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
+               (const float4*)c.edge_tree.box, c.nE, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
+               c.ctype1);
             PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, sizeof(int), hipMemcpyDeviceToHost, c.stream));
             PAMO_HIP(hipStreamSynchronize(c.stream));
             const int n = c.hcount[0];
@@ -1153,27 +1403,20 @@ static int update_target(Ctx& c) {
 }
 
 // CGSolver.solve (cg_solver.py:96-158): Jacobi PCG with n_cg fixed
-// iterations, p = H^-1 grad.
-static int cg_solve(Ctx& c, int nC) {
+// iterations, p = H^-1 grad; three launches per iteration.
+static int cg_solve(Ctx& c) {
     const int nV = c.nV, mask = c.P.mask;
-    const unsigned nb = grid_for(nV, kB);
-    hipLaunchKernelGGL(cg_init_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.grad, (const float4*)c.diag, nV,
-                       c.r, c.z, c.v, c.p, c.part);
-    hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, (int)SC_ZR);
-    PAMO_HIP(hipGetLastError());
-    PAMO_TRY(fill(c.Av, make_float4(0, 0, 0, 0), nV, c.stream));
+    const int np = (int)grid_for(nV, kB);
+    L3(cg_init_k, nV, (const float4*)c.grad, (const float4*)c.diag, nV, c.r, c.z, c.p, c.zr_part);
     for (int k = 0; k < c.P.n_cg; ++k) {
         ++c.stat_cg;
-        PAMO_TRY(compute_hvp(c, nC, mask, c.v, c.Av));
-        PAMO_TRY(dot_into(c, c.v, c.Av, SC_VAV));
-        hipLaunchKernelGGL(copy_slot_k, dim3(1), dim3(1), 0, c.stream, c.sc, (int)SC_ZR, (int)SC_ZRLAST);
-        hipLaunchKernelGGL(cg_update_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.v, (const float4*)c.Av,
-                           (const float4*)c.diag, (const double*)c.sc, nV, c.p, c.r, c.z, c.part);
-        hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc,
-                           (int)SC_ZR);
-        hipLaunchKernelGGL(cg_dir_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.z, (const double*)c.sc, nV, c.v,
-                           c.Av);
-        PAMO_HIP(hipGetLastError());
+        const int cur = k & 1;
+        double* zr_cur = c.zr_part + (size_t)cur * np;
+        double* zr_oth = c.zr_part + (size_t)(cur ^ 1) * np;
+        CgVec cv{c.z, c.v, zr_cur, zr_oth, np, k == 0 ? 1 : 0};
+        PAMO_TRY(compute_hvp(c, mask, cv, c.v, c.Av, c.vav_part));
+        L3(cg_update_k, nV, (const float4*)c.v, (const float4*)c.Av, (const float4*)c.diag, (const double*)c.vav_part,
+           (const double*)zr_cur, np, nV, c.p, c.r, c.z, zr_oth);
     }
     return 0;
 }
@@ -1208,7 +1451,7 @@ static int step_once(Ctx& c) {
         PAMO_TRY(detect_contacts(c, nC));
         PAMO_TRY(compute_energy(c, nC, SC_E, mask));
         PAMO_TRY(compute_diff(c, nC, mask));
-        PAMO_TRY(cg_solve(c, nC));
+        PAMO_TRY(cg_solve(c));
         PAMO_TRY(clamp_and_ccd(c, nC));
         // _line_search (system.py:393-409)
         hipLaunchKernelGGL(copy_slot_k, dim3(1), dim3(1), 0, c.stream, c.sc, (int)SC_E, (int)SC_EPREV);
@@ -1260,6 +1503,8 @@ int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->ccount, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->n_act, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->n_act, 0, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void**)&c->sc, sizeof(double) * pamo::s3::SC_N);
     if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(double) * pamo::s3::SC_N);
     if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, sizeof(float));
@@ -1331,12 +1576,21 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         float4** vv[] = {&c.q, &c.qn, &c.qd, &c.qrest, &c.p, &c.target, &c.grad, &c.diag, &c.r, &c.v, &c.Av, &c.z};
         for (float4** q : vv) PAMO_TRY(dalloc(q, c.capV));
         PAMO_TRY(dalloc(&c.vor, c.capV));
+        PAMO_TRY(dalloc(&c.vf_off, (size_t)c.capV + 1));
+        PAMO_TRY(dalloc(&c.vh_off, (size_t)c.capV + 1));
+        PAMO_TRY(dalloc(&c.vc_off, (size_t)c.capV + 1));
+        PAMO_TRY(dalloc(&c.dn, 3 * (size_t)c.capV));
+        const size_t np = pamo::grid_for(c.capV, pamo::s3::kB) + 2;
+        PAMO_TRY(dalloc(&c.zr_part, 2 * np));
+        PAMO_TRY(dalloc(&c.vav_part, np));
     }
     if (n_f > c.capF) {
         c.capF = n_f + n_f / 4 + 64;
         PAMO_TRY(dalloc(&c.tri, c.capF));
         PAMO_TRY(dalloc(&c.area, c.capF));
         PAMO_TRY(dalloc(&c.inv_dm, c.capF));
+        PAMO_TRY(dalloc(&c.vf_ent, 3 * (size_t)c.capF));
+        PAMO_TRY(dalloc(&c.fc, 3 * (size_t)c.capF));
     }
     if (n_e > c.capE) {
         c.capE = n_e + n_e / 4 + 64;
@@ -1348,6 +1602,9 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         PAMO_TRY(dalloc(&c.rest_angle, c.capH));
         PAMO_TRY(dalloc(&c.rest_elen, c.capH));
         PAMO_TRY(dalloc(&c.hblock, 144 * (size_t)c.capH));
+        PAMO_TRY(dalloc(&c.hsym, 45 * (size_t)c.capH));
+        PAMO_TRY(dalloc(&c.vh_ent, 4 * (size_t)c.capH));
+        PAMO_TRY(dalloc(&c.hc, 4 * (size_t)c.capH));
     }
     c.nV = n_v;
     c.nF = n_f;
@@ -1375,6 +1632,9 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         hipLaunchKernelGGL(pamo::s3::hinge_rest_k, dim3(pamo::grid_for(n_h, 256)), dim3(256), 0, c.stream,
                            (const float4*)c.qrest, (const int4*)c.hinge, n_h, c.rest_angle, c.rest_elen);
     PAMO_HIP(hipGetLastError());
+    // vertex -> face corner / hinge slot incidence, fixed for this mesh
+    PAMO_TRY(pamo::s3::build_incidence(c, (const int*)c.tri, n_f, 3, 4, c.vf_off, c.vf_ent));
+    PAMO_TRY(pamo::s3::build_incidence(c, (const int*)c.hinge, n_h, 4, 4, c.vh_off, c.vh_ent));
     PAMO_HIP(hipStreamSynchronize(c.stream));
     return 0;
 }
@@ -1544,11 +1804,11 @@ int pamo_s3_debug(pamo_s3* s, int what, const double* h_in, long long n_in, doub
             PAMO_TRY(read_vec(c, c.diag, h_out + n3));
         } else if (what == PAMO_S3_DBG_HVP) {
             PAMO_TRY(write_vec(c, h_in, c.v));
-            PAMO_TRY(pamo::fill(c.Av, make_float4(0, 0, 0, 0), c.nV, c.stream));
-            PAMO_TRY(compute_hvp(c, nC, c.P.mask, c.v, c.Av));
+            CgVec cv{c.v, c.v, c.zr_part, c.zr_part, 1, 1};  // dx = v as given
+            PAMO_TRY(compute_hvp(c, c.P.mask, cv, c.v, c.Av, c.vav_part));
             PAMO_TRY(read_vec(c, c.Av, h_out));
         } else if (what == PAMO_S3_DBG_SOLVE) {
-            PAMO_TRY(cg_solve(c, nC));
+            PAMO_TRY(cg_solve(c));
             PAMO_TRY(read_vec(c, c.p, h_out));
         } else {
             std::vector<float> hb(144 * (size_t)c.nH);

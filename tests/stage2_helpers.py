@@ -53,3 +53,27 @@ def run_rounds(forward, V, F, ratio, tolerance=4, threshold=1e-3, max_rounds=100
         if count == tolerance:
             break
     return V, F, np.array(hist, np.int32)
+
+
+def escape_links(left, right, n):
+    """Depth-first escape link of every node of a binary tree with internal
+    nodes 0..n-2 (children in left/right) and leaves n-1..2n-2: the right
+    sibling of the first ancestor-or-self that is a left child, else -1."""
+    import numpy as np
+    N = 2 * n - 1
+    parent = np.full(N, -1, dtype=np.int64)
+    is_left = np.zeros(N, dtype=bool)
+    for i in range(n - 1):
+        parent[left[i]] = i
+        parent[right[i]] = i
+        is_left[left[i]] = True
+    esc = np.full(N, -1, dtype=np.int64)
+    # top-down: esc(left child) = right sibling; esc(right child) = esc(parent)
+    order = [0]
+    while order:
+        x = order.pop()
+        if x < n - 1:
+            esc[left[x]] = right[x]
+            esc[right[x]] = esc[x]
+            order += [left[x], right[x]]
+    return esc

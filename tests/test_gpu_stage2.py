@@ -7,7 +7,7 @@ compared bit for bit as well (same FP32 operations, no contraction).
 import numpy as np
 import pytest
 
-from stage2_helpers import centred, run_rounds
+from stage2_helpers import escape_links, centred, run_rounds
 
 pytestmark = pytest.mark.gpu
 
@@ -163,3 +163,7 @@ def test_lbvh_bit_exact(torch_cuda, oracle_lib):
     # internal: exact union of children
     assert np.array_equal(lo[: n - 1], np.minimum(lo[left_o], lo[right_o]))
     assert np.array_equal(hi[: n - 1], np.maximum(hi[left_o], hi[right_o]))
+    # rope links: left child (internal) / face (leaf) in lo.w, escape in hi.w
+    w = boxes.view(np.int32)
+    assert np.array_equal(w[: n - 1, 3], left_o) and np.array_equal(w[n - 1:, 3], faces_o)
+    assert np.array_equal(w[:, 7], escape_links(left_o, right_o, n))

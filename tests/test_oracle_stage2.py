@@ -397,3 +397,31 @@ def test_default_min_verts_stops_after_one_round(oracle_lib):
     V, F = synth.icosphere_sh(3, 0)
     _, _, hist = oracle_lib.run(centred(V, F), F, 0.05)
     assert len(hist) == 1
+
+
+def test_rope_walk_visits_every_node_in_depth_first_order(oracle_lib):
+    """Escape links (as the device writes them into the node records) make a
+    stackless walk that, for a query overlapping everything, visits every
+    node exactly once in depth-first order and ends; a query overlapping
+    nothing ends after the root."""
+    from stage2_helpers import escape_links
+    for level in (2, 3):
+        from pamo_amd import synth
+        V, F = synth.icosphere_sh(level, 1)
+        codes, faces, left, right = oracle_lib.lbvh(V.astype(np.float32), F)
+        n = len(codes)
+        esc = escape_links(left, right, n)
+        seen, x = [], 0
+        while x >= 0 and len(seen) <= 2 * n:
+            seen.append(x)
+            x = left[x] if x < n - 1 else esc[x]
+        dfs, st = [], [0]
+        while st:
+            y = st.pop()
+            dfs.append(y)
+            if y < n - 1:
+                st += [right[y], left[y]]
+        assert seen == dfs and len(seen) == 2 * n - 1
+        assert esc[0] == -1
+        # leaves in left-to-right order
+        assert [y - (n - 1) for y in seen if y >= n - 1] == list(range(n))
