@@ -347,7 +347,8 @@ static int run_guard(Ctx& c, int nT) {
     PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
     LAUNCHP(K_GUARD_QUERY, 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT,
-            guard_query_k, nT, m, c.qlist, c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, 1e-3f, c.flagged);
+            guard_query_k, (long long)nT << kGuardLog, m, c.qlist, c.dcnt, c.changed, c.fbox,
+            (const float4*)c.bbox, 1e-3f, c.flagged);
     return 0;
 }
 

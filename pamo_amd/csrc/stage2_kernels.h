@@ -833,6 +833,25 @@ __global__ void __launch_bounds__(kBlock) rope_k(const int* __restrict__ left, c
     r[7] = __int_as_float(esc);
 }
 
+// Splits a rope-linked tree into the 2^kLog subtrees under the root (fewer
+// where a leaf is reached early): lane k of a query walks subtree k, from the
+// returned node until *stop (that node's escape).  The right child of a node
+// is its left child's escape.  Returns -1 for lanes with no subtree.
+template <int kLog>
+__device__ __forceinline__ int subtree_start(const float4* __restrict__ node, int n_leaves, int k, int* stop) {
+    int cur = 0;
+    for (int d = kLog - 1; d >= 0; --d) {
+        if (cur >= n_leaves - 1) {
+            if ((k & ((2 << d) - 1)) != 0) return -1;
+            break;
+        }
+        const int left = __float_as_int(node[2 * (long long)cur].w);
+        cur = ((k >> d) & 1) ? __float_as_int(node[2 * (long long)left + 1].w) : left;
+    }
+    *stop = __float_as_int(node[2 * (long long)cur + 1].w);
+    return cur;
+}
+
 __device__ __forceinline__ bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
     if (ahi.x < blo.x || bhi.x < alo.x) return false;
     if (ahi.y < blo.y || bhi.y < alo.y) return false;
@@ -890,6 +909,7 @@ __device__ __forceinline__ void eval_pair(Mesh m, int a, int b, float epsilon, u
 }
 
 constexpr int kQueue = 128;  // per-wave pair queue (< 64 pending + one append of <= 64)
+constexpr int kGuardLog = 3;  // 8 lanes per guard query
 
 // Candidate pairs: every queried face a against every live face b != a whose
 // box overlaps a's box (closed intervals).  A pair of queried faces is taken
@@ -907,10 +927,11 @@ constexpr int kQueue = 128;  // per-wave pair queue (< 64 pending + one append o
 // dropped before any arithmetic (the reference rejects them after its
 // coplanarity test in every outcome).
 //
-// Each lane walks the rope-linked tree for its own query, one node per loop
-// iteration; candidates go to a per-wave LDS queue and the whole wave
-// evaluates the FP64 predicates 64 pairs at a time, so the long predicate
-// code runs with every lane busy and no lane waits for another's walk.
+// kGuardLanes lanes serve one query, each walking one subtree of the
+// rope-linked tree, one node per loop iteration; candidates go to a
+// per-wave LDS queue and the whole wave evaluates the FP64 predicates 64
+// pairs at a time, so the long predicate code runs with every lane busy and
+// no lane waits for another's walk.
 __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
                                                         const int* __restrict__ cnt,
                                                         const uint8_t* __restrict__ changed,
@@ -919,11 +940,14 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
                                                         uint8_t* __restrict__ flagged) {
     __shared__ int qa[kBlock / kWave][kQueue], qb[kBlock / kWave][kQueue];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = (int)(t >> kGuardLog), sub = (int)(t & ((1 << kGuardLog) - 1));
     const int nq = cnt[C_NQUERY];
     const int n = cnt[C_NLIVE];
     if (__ballot(q < nq) == 0 || n <= 1) return;  // uniform per wave
-    bool active = q < nq;
+    int x = -1, stop = -1;
+    if (q < nq) x = subtree_start<kGuardLog>(node, n, sub, &stop);
+    bool active = x >= 0;
     int a = 0, ia0 = 0, ia1 = 0, ia2 = 0;
     float4 qlo = make_float4(0.f, 0.f, 0.f, 0.f), qhi = qlo;
     if (active) {
@@ -932,8 +956,7 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
         qlo = fbox[2 * (long long)a];
         qhi = fbox[2 * (long long)a + 1];
     }
-    int x = 0;      // root
-    int qn = 0;     // wave-uniform queue length
+    int qn = 0;  // wave-uniform queue length
     while (true) {
         int found = -1;
         if (active) {
@@ -955,7 +978,7 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
             } else {
                 x = esc;
             }
-            if (x < 0) active = false;
+            if (x == stop) active = false;
         }
         const uint64_t fm = __ballot(found >= 0);
         if (found >= 0) {

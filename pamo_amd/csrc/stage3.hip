@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__
 // iteration and queues box hits in LDS; the wave classifies 64 queued pairs
 // at a time and reserves output slots with one atomic per wave.
 constexpr int kDetQ = 128;
+constexpr int kDetLog = 4;  // 16 lanes per query, one subtree each
 
 template <bool kPT>
 __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int nq, const int4* __restrict__ tri,
@@ -449,9 +450,12 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
                                                int* __restrict__ ctype1) {
     __shared__ int qa[kB / kWave][kDetQ], qb[kB / kWave][kDetQ];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = (int)(tid >> kDetLog), sub = (int)(tid & ((1 << kDetLog) - 1));
     if (__ballot(q < nq) == 0 || n_leaves <= 0) return;  // uniform per wave
-    bool active = q < nq;
+    int cur = -1, stop = -1;
+    if (q < nq) cur = subtree_start<kDetLog>(node, n_leaves, sub, &stop);
+    bool active = cur >= 0;
     const float thr = 2.0f * radius + d_hat;
     V3 lo = mk(0.f, 0.f, 0.f), hi = lo;
     if (active) {
@@ -466,7 +470,7 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
             hi = vmax(a, b) + mk(thr, thr, thr);
         }
     }
-    int cur = 0, qn = 0;
+    int qn = 0;
     while (true) {
         int found = -1;
         if (active) {
@@ -482,7 +486,7 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
             } else {
                 cur = esc;
             }
-            if (cur < 0) active = false;
+            if (cur == stop) active = false;
         }
         const uint64_t fm = __ballot(found >= 0);
         if (found >= 0) {
@@ -983,7 +987,12 @@ struct Incid {
 };
 
 // Av = H dx per vertex; the direction dx is stored to v_out; block partials
-// of dx . Av to vav.
+// of dx . Av to vav.  A group of kGather lanes serves one vertex (its
+// incidence lists are strided over the group and reduced with a fixed
+// butterfly), so a vertex with many incident contacts does not hold up a
+// whole wave.
+constexpr int kGather = 16;
+
 __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2gt, int use_elem, int use_hinge,
                                                    int use_contact, int use_dense, const float* __restrict__ vor,
                                                    CgVec cv, Incid ic, const float4* __restrict__ fc,
@@ -993,21 +1002,30 @@ __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2g
                                                    double* __restrict__ vav) {
     __shared__ double red[kB];
     const float beta = cg_beta(cv, red);
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    double dotp = 0.0;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = (int)(t / kGather), g = (int)(t % kGather);
+    V3 acc = mk(0.0f, 0.0f, 0.0f);
     if (v < nV) {
-        const V3 d = cg_dx(cv, beta, v);
-        V3 acc = mk(0.0f, 0.0f, 0.0f);
-        if (use_m2gt) acc = acc + d * (vor[v] * P.k_m2gt);
         if (use_elem)
-            for (int e = ic.vf_off[v]; e < ic.vf_off[v + 1]; ++e) acc = acc + v3(fc[ic.vf_ent[e]]);
+            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) acc = acc + v3(fc[ic.vf_ent[e]]);
         if (use_hinge)
-            for (int e = ic.vh_off[v]; e < ic.vh_off[v + 1]; ++e) acc = acc + v3(hc[ic.vh_ent[e]]);
+            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) acc = acc + v3(hc[ic.vh_ent[e]]);
         if (use_contact)
-            for (int e = ic.vc_off[v]; e < ic.vc_off[v + 1]; ++e) {
+            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
                 const int s = ic.vc_ent[e];
                 acc = acc + v3(cdd[s]) * sA[s >> 2];
             }
+    }
+#pragma unroll
+    for (int m = kGather / 2; m > 0; m >>= 1) {
+        acc.x += __shfl_xor(acc.x, m, kGather);
+        acc.y += __shfl_xor(acc.y, m, kGather);
+        acc.z += __shfl_xor(acc.z, m, kGather);
+    }
+    double dotp = 0.0;
+    if (v < nV && g == 0) {
+        const V3 d = cg_dx(cv, beta, v);
+        if (use_m2gt) acc = acc + d * (vor[v] * P.k_m2gt);
         if (use_dense) acc = acc + mk(dn[3 * v], dn[3 * v + 1], dn[3 * v + 2]);
         Av[v] = f4(acc);
         v_out[v] = f4(d);
@@ -1040,11 +1058,11 @@ __global__ void __launch_bounds__(kB) cg_init_k(const float4* __restrict__ grad,
 // partials of z . r (update_p_r_z_compute_zr_kernel, solver_kernels.py:4-23)
 __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, const float4* __restrict__ Av,
                                                   const float4* __restrict__ diag, const double* __restrict__ vav,
-                                                  const double* __restrict__ zr_cur, int nparts, int n,
+                                                  int n_vav, const double* __restrict__ zr_cur, int nparts, int n,
                                                   float4* __restrict__ p, float4* __restrict__ r,
                                                   float4* __restrict__ z, double* __restrict__ zr_next) {
     __shared__ double red[kB];
-    const double vAv = reduce_parts(vav, nparts, red);
+    const double vAv = reduce_parts(vav, n_vav, red);
     const double zr = reduce_parts(zr_cur, nparts, red);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double x = 0.0;
@@ -1125,6 +1143,10 @@ __global__ void __launch_bounds__(kB) accd_k(const float4* __restrict__ x, const
                                              Params P, int nC, float* __restrict__ ccd_step) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nC) return;
+    // the result is the minimum over pairs and a pair's t only grows, so a
+    // pair stops as soon as t reaches the minimum found so far
+    const volatile float* vstep = ccd_step;
+    if (*vstep <= 0.0f) return;
     int4 k = tp.cidx[b];
     V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
     V3 v0 = v3(vel[k.x]), v1 = v3(vel[k.y]), v2 = v3(vel[k.z]), v3_ = v3(vel[k.w]);
@@ -1160,6 +1182,7 @@ __global__ void __launch_bounds__(kB) accd_k(const float4* __restrict__ x, const
         }
         t = t + tl;
         if (t > 1.0f) return;
+        if ((it & 7) == 7 && t >= *vstep) return;
         tl = 0.8f * dmt / lv;
         ++it;
     }
@@ -1330,7 +1353,7 @@ static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4*
            (const float*)c.cd, (const int*)c.act, (const float*)c.hsym, (const float4*)c.cdd,
            (const double*)c.dense_A, c.fc, c.hc, c.sA, c.dn);
     Incid ic{c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent};
-    L3(hvp_gather_k, c.nV, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
+    L3(hvp_gather_k, (long long)c.nV * kGather, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
        (int)(nA > 0), (int)(nd > 0), (const float*)c.vor, cv, ic, (const float4*)c.fc, (const float4*)c.hc,
        (const float*)c.sA, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
     return 0;
@@ -1368,10 +1391,10 @@ static int detect_contacts(Ctx& c, int& nC) {
         while (true) {
             PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
             const int cap = std::min(c.capC, c.P.max_blocks);
-            L3(detect_k<true>, c.nV, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
+            L3(detect_k<true>, (long long)c.nV << kDetLog, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.tri_tree.box, c.nF, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
                c.ctype1);
-            L3(detect_k<false>, c.nE, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
+            L3(detect_k<false>, (long long)c.nE << kDetLog, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.edge_tree.box, c.nE, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
                c.ctype1);
             PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -1416,7 +1439,7 @@ static int cg_solve(Ctx& c) {
         CgVec cv{c.z, c.v, zr_cur, zr_oth, np, k == 0 ? 1 : 0};
         PAMO_TRY(compute_hvp(c, mask, cv, c.v, c.Av, c.vav_part));
         L3(cg_update_k, nV, (const float4*)c.v, (const float4*)c.Av, (const float4*)c.diag, (const double*)c.vav_part,
-           (const double*)zr_cur, np, nV, c.p, c.r, c.z, zr_oth);
+           (int)grid_for((long long)nV * kGather, kB), (const double*)zr_cur, np, nV, c.p, c.r, c.z, zr_oth);
     }
     return 0;
 }
@@ -1582,7 +1605,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         PAMO_TRY(dalloc(&c.dn, 3 * (size_t)c.capV));
         const size_t np = pamo::grid_for(c.capV, pamo::s3::kB) + 2;
         PAMO_TRY(dalloc(&c.zr_part, 2 * np));
-        PAMO_TRY(dalloc(&c.vav_part, np));
+        PAMO_TRY(dalloc(&c.vav_part, pamo::grid_for((long long)c.capV * pamo::s3::kGather, pamo::s3::kB) + 2));
     }
     if (n_f > c.capF) {
         c.capF = n_f + n_f / 4 + 64;

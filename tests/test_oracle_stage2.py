@@ -425,3 +425,46 @@ def test_rope_walk_visits_every_node_in_depth_first_order(oracle_lib):
         assert esc[0] == -1
         # leaves in left-to-right order
         assert [y - (n - 1) for y in seen if y >= n - 1] == list(range(n))
+
+
+def _subtree_start(left, esc, n, k, log):
+    """The device's split of a rope-linked tree into 2^log subtrees."""
+    cur = 0
+    for d in range(log - 1, -1, -1):
+        if cur >= n - 1:
+            if k & ((2 << d) - 1):
+                return -1, -1
+            break
+        lc = left[cur]
+        cur = esc[lc] if (k >> d) & 1 else lc
+    return cur, esc[cur]
+
+
+@pytest.mark.parametrize("log", [3, 4])
+def test_subtree_walks_partition_the_leaves(oracle_lib, log):
+    """Lane k of a query walks from its subtree root to that root's escape;
+    over all lanes every leaf is visited exactly once (trees of 1, 2, 3 and
+    many leaves)."""
+    from stage2_helpers import escape_links
+    from pamo_amd import synth
+    V, F = synth.icosphere_sh(3, 2)
+    V = V.astype(np.float32)
+    for nf in (1, 2, 3, 5, 17, len(F)):
+        Fs = F.copy()
+        Fs[nf:] = -1
+        codes, faces, left, right = oracle_lib.lbvh(V, Fs)
+        n = len(codes)
+        esc = escape_links(left, right, n)
+        seen = []
+        for k in range(1 << log):
+            x, stop = _subtree_start(left, esc, n, k, log)
+            steps = 0
+            while x >= 0 and x != stop:
+                steps += 1
+                assert steps <= 2 * n
+                if x >= n - 1:
+                    seen.append(x - (n - 1))
+                    x = esc[x]
+                else:
+                    x = left[x]
+        assert sorted(seen) == list(range(n)), nf
