@@ -31,6 +31,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
     p.add_argument("--stage3", type=int, default=1, help="1/0: stage 3 on/off")
+    p.add_argument("--iters", type=int, default=100000,
+                   help="cap on stage-2 rounds (the reference's iter; the workload uses 100000)")
+    p.add_argument("--calib-copy-mb", type=int, default=0,
+                   help="after the run, one device-to-device copy of this many MiB (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -45,8 +49,9 @@ def main():
     from pamo_amd import synth, _hip
     from pamo_amd.pamo import PaMO, face_centroid_mean
 
-    ratio, tol, thr, iters = 0.01, 4, 1e-3, 100000
+    ratio, tol, thr, iters = 0.01, 4, 1e-3, args.iters
     V, F = synth.headline_mesh(1)
+    print(f"[bench] mesh ready: {len(F)} faces", file=sys.stderr, flush=True)
     nF0, nV0 = len(F), len(V)
     dev = torch.device("cuda", 0)
     Vd = torch.from_numpy(V).to(dev)
@@ -63,6 +68,7 @@ def main():
     for _ in range(args.warmup):
         one_run()
     torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
     # untimed pass with every kernel bracketed by HIP events: per-kernel table
     prof.reset()
     prof.enable(True)
@@ -70,6 +76,7 @@ def main():
     torch.cuda.synchronize()
     prof.enable(False)
     stats = prof.stats()
+    print("[bench] per-kernel pass done", file=sys.stderr, flush=True)
     dom_name = max(stats, key=lambda s: s["total_ms"])["name"]
     # timed region: only the dominant kernel is bracketed
     prof.reset()
@@ -148,6 +155,15 @@ def main():
         "cpu_baseline": cpu,
         "kernels": stats,
     }
+    if args.calib_copy_mb > 0:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # convert the counters of the kernels above into bytes
+        src = torch.empty(args.calib_copy_mb << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 2 * src.numel() * 4
     print(json.dumps(out))
 
 

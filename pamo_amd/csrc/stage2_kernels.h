@@ -258,6 +258,8 @@ struct CostParams {
 };
 
 // compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge.
+constexpr int kLinkRing = 24;
+
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
                                                       const int4* __restrict__ rv, const int* __restrict__ eu,
                                                       const int* __restrict__ ev, int nE,
@@ -273,23 +275,45 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     }
     if (cp.is_stuck) return;
     const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
-    // link condition: successors of u and v in their rings (:405-444)
+    // link condition: successors of u and v in their rings (:405-444); the
+    // successors of v's ring sit in registers (static indices) when it has
+    // at most kLinkRing faces
     int dup = 0;
-    for (int a = ub; a < ue; ++a) {
-        const int su = succ_in(rv[a], u);
-        for (int b = vb; b < ve; ++b) {
-            dup += (su == succ_in(rv[b], v)) ? 1 : 0;
+    if (ve - vb <= kLinkRing) {
+        int sv[kLinkRing];
+#pragma unroll
+        for (int b = 0; b < kLinkRing; ++b) sv[b] = vb + b < ve ? succ_in(rv[vb + b], v) : -2;
+        for (int a = ub; a < ue; ++a) {
+            const int su = succ_in(rv[a], u);
+#pragma unroll
+            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
             if (dup > 2) {
                 cost[e] = 0xFFFFFFFFu;
                 return;
             }
         }
+    } else {
+        for (int a = ub; a < ue; ++a) {
+            const int su = succ_in(rv[a], u);
+            for (int b = vb; b < ve; ++b) {
+                dup += (su == succ_in(rv[b], v)) ? 1 : 0;
+                if (dup > 2) {
+                    cost[e] = 0xFFFFFFFFu;
+                    return;
+                }
+            }
+        }
     }
     if (dup != 2) return;
     const F3 p0 = ld(m, u), p1 = ld(m, v);
-    float elen = 0.0f;
-    int nedge = 0;
+    const F3 vm = f3divs(f3add(p0, p1), 2.0f);
+    const float k4s3 = 4.0f * sqrtf(3.0f);
+    // one pass over both rings: the edge-length sum and the skinny / flip
+    // terms read the same three vertices; each sum keeps its own order
+    float elen = 0.0f, qa = 0.0f;
+    int nedge = 0, ntri = 0;
     for (int side = 0; side < 2; ++side) {
+        const int w = side ? v : u, other = side ? u : v;
         const int b0 = side ? vb : ub, b1 = side ? ve : ue;
         for (int a = b0; a < b1; ++a) {
             const int4 t = rv[a];
@@ -297,23 +321,9 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
             if (t.x > t.y) { elen += f3norm(f3sub(pi, pj)); ++nedge; }
             if (t.y > t.z) { elen += f3norm(f3sub(pj, pk)); ++nedge; }
             if (t.z > t.x) { elen += f3norm(f3sub(pk, pi)); ++nedge; }
-        }
-    }
-    elen = elen / (float)nedge / cp.edge_s * cp.tres;
-    elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
-    const F3 vm = f3divs(f3add(p0, p1), 2.0f);
-    const float k4s3 = 4.0f * sqrtf(3.0f);
-    float qa = 0.0f;
-    int ntri = 0;
-    for (int side = 0; side < 2; ++side) {
-        const int w = side ? v : u, other = side ? u : v;
-        const int b0 = side ? vb : ub, b1 = side ? ve : ue;
-        for (int a = b0; a < b1; ++a) {
-            const int4 t = rv[a];
             if (t.x == other || t.y == other || t.z == other) continue;
-            const F3 o0 = ld(m, t.x), o1 = ld(m, t.y), o2 = ld(m, t.z);
-            const F3 n0 = t.x == w ? vm : o0, n1 = t.y == w ? vm : o1, n2 = t.z == w ? vm : o2;
-            F3 on = f3cross(f3sub(o1, o0), f3sub(o2, o0));
+            const F3 n0 = t.x == w ? vm : pi, n1 = t.y == w ? vm : pj, n2 = t.z == w ? vm : pk;
+            F3 on = f3cross(f3sub(pj, pi), f3sub(pk, pi));
             on = f3divs(on, f3norm(on));
             F3 nn = f3cross(f3sub(n1, n0), f3sub(n2, n0));
             nn = f3divs(nn, f3norm(nn));
@@ -327,6 +337,8 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
             ++ntri;
         }
     }
+    elen = elen / (float)nedge / cp.edge_s * cp.tres;
+    elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
     qa = 5.0f * qa;
     // M = Q_u + Q_v; storage 0:00 1:01 2:02 3:03 4:11 5:12 6:13 7:22 8:23 9:33
     const float4 ua = Q[3 * (long long)u], ub4 = Q[3 * (long long)u + 1], uc = Q[3 * (long long)u + 2];
