@@ -137,6 +137,25 @@ int pamo_s3_set_mesh(pamo_s3* s3, const float* d_v, int n_v, const int* d_f, int
                      const int* h_edges, int n_e, const int* h_hinges, int n_h, const float* h_voronoi,
                      int do_nothing);
 
+/* The connectivity part of Stage3System.register_mesh done on the host
+ * (system.py:175-228: trimesh edges_unique, the hinge search of
+ * hinge_energy.py:21-57, igl Voronoi mass matrix, igl internal angles):
+ * host float32 (n_v,3) / int32 (n_f,3) in, the normalisation (scale, t3)
+ * applied in FP64.  Outputs sized 2*(3*n_f) edges ints, 4*(3*n_f) hinge ints
+ * and n_v floats; n_e / n_h receive the counts. */
+int pamo_s3_prepare_host(const float* h_v, int n_v, const int* h_f, int n_f, double scale, const double* t3,
+                         int* h_edges, int* n_e, int* h_hinges, int* n_h, float* h_voronoi, double* min_angle_deg);
+
+/* processing.py:15-20 get_normalization_transform for device float32
+ * (n_v,3) vertices, reduced in FP64: scale = 1 / max extent, t = -mean * scale. */
+int pamo_s3_normalization(pamo_s3* s3, const float* d_v, int n_v, double* scale, double* t3);
+
+/* pamo_s3_set_mesh with the connectivity prepared by pamo_s3_prepare_host
+ * from the device mesh (the whole of register_mesh for the mesh to project,
+ * including the min-angle do_nothing rule, system.py:220-224). */
+int pamo_s3_set_mesh_device(pamo_s3* s3, const float* d_v, int n_v, const int* d_f, int n_f, double scale,
+                            const double* t3);
+
 /* n_steps calls of Stage3System.step (system.py:411-470): closest-point
  * targets, then n_newton_iters of {contacts, gradient, Jacobi PCG, clamp,
  * ACCD, halving line search}. */
@@ -175,7 +194,7 @@ int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, dou
 /* Statistics (no counterpart in the reference): [0] last total energy,
  * [1] contact pairs summed over Newton iterations, [2] Newton iterations,
  * [3] CG iterations, [4] line-search evaluations, [5] current detection
- * radius, [6] do_nothing flag, [7] reserved. */
+ * radius, [6] do_nothing flag, [7] number of hinges. */
 int pamo_s3_stats(pamo_s3* s3, double* out8);
 
 #ifdef __cplusplus

@@ -84,6 +84,10 @@ SIGNATURES = [
     ("pamo_s3_set_dense", c_int, [c_vp, c_vp, c_vp, c_double, c_int]),
     ("pamo_s3_debug", c_int, [c_vp, c_int, c_vp, c_ll, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_s3_stats", c_int, [c_vp, c_vp]),
+    ("pamo_s3_prepare_host", c_int, [c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double), c_vp,
+                                     ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_double)]),
+    ("pamo_s3_normalization", c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    ("pamo_s3_set_mesh_device", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double)]),
 ]
 
 _lib = None

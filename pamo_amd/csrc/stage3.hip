@@ -1491,6 +1491,139 @@ static int step_once(Ctx& c) {
     return 0;
 }
 
+// ------------------------------------------------------------ host-side mesh preparation
+// The connectivity work of register_mesh (system.py:175-228) that the
+// reference does in trimesh / igl: unique edges in trimesh's order (by
+// (max, min)), hinges in (face, corner) order, Voronoi areas (Meyer et al.
+// 2003 mixed areas, igl MASSMATRIX_TYPE_VORONOI) and the smallest interior
+// angle, all in FP64 on the normalised positions.
+struct HostPrep {
+    std::vector<int> edges, hinges;
+    std::vector<float> vor;
+    double min_angle_deg = 180.0;
+};
+
+static void host_prepare(const float* V, int nV, const int* F, int nF, double scale, const double* t, HostPrep& hp) {
+    std::vector<double> X(3 * (size_t)nV);
+    for (size_t i = 0; i < 3 * (size_t)nV; ++i) X[i] = (double)V[i] * scale + t[i % 3];
+    // unique edges
+    std::vector<uint64_t> keys(3 * (size_t)nF);
+    for (int f = 0; f < nF; ++f)
+        for (int k = 0; k < 3; ++k) {
+            uint32_t a = (uint32_t)F[3 * f + k], b = (uint32_t)F[3 * f + (k + 1) % 3];
+            uint32_t lo = std::min(a, b), hi = std::max(a, b);
+            keys[3 * (size_t)f + k] = ((uint64_t)hi << 32) | lo;
+        }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    hp.edges.resize(2 * keys.size());
+    for (size_t e = 0; e < keys.size(); ++e) {
+        hp.edges[2 * e] = (int)(keys[e] & 0xFFFFFFFFu);
+        hp.edges[2 * e + 1] = (int)(keys[e] >> 32);
+    }
+    // hinges: directed edge (a, b) of slot s = corner * nF + face; the partner
+    // is the smallest slot holding (b, a)
+    const size_t nS = 3 * (size_t)nF;
+    std::vector<std::pair<uint64_t, uint32_t>> dir(nS);
+    for (int k = 0; k < 3; ++k)
+        for (int f = 0; f < nF; ++f) {
+            const uint32_t a = (uint32_t)F[3 * f + k], b = (uint32_t)F[3 * f + (k + 1) % 3];
+            const size_t sl = (size_t)k * nF + f;
+            dir[sl] = {((uint64_t)a << 32) | b, (uint32_t)sl};
+        }
+    std::vector<std::pair<uint64_t, uint32_t>> sorted = dir;
+    std::sort(sorted.begin(), sorted.end());
+    hp.hinges.clear();
+    for (int f = 0; f < nF; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const int a = F[3 * f + k], b = F[3 * f + (k + 1) % 3], o = F[3 * f + (k + 2) % 3];
+            if (!(a < b)) continue;
+            const uint64_t rk = ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+            auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(rk, (uint32_t)0));
+            if (it == sorted.end() || it->first != rk) continue;
+            const uint32_t sl = it->second;
+            const int f2 = (int)(sl % (uint32_t)nF), k2 = (int)(sl / (uint32_t)nF);
+            hp.hinges.push_back(o);
+            hp.hinges.push_back(a);
+            hp.hinges.push_back(b);
+            hp.hinges.push_back(F[3 * f2 + (k2 + 2) % 3]);
+        }
+    // Voronoi areas and the smallest interior angle
+    std::vector<double> vor(nV, 0.0);
+    double min_ang = 1e300;
+    bool any_nan = false;  // numpy's min propagates NaN (a zero-length edge), and NaN <= thres is false
+    for (int f = 0; f < nF; ++f) {
+        const double* P[3] = {&X[3 * (size_t)F[3 * f]], &X[3 * (size_t)F[3 * f + 1]], &X[3 * (size_t)F[3 * f + 2]]};
+        auto d3 = [](const double* a, const double* b, double* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; };
+        auto nrm = [](const double* a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
+        double e12[3], e20[3], e01[3];
+        d3(P[1], P[2], e12);
+        d3(P[2], P[0], e20);
+        d3(P[0], P[1], e01);
+        const double l[3] = {nrm(e12), nrm(e20), nrm(e01)};  // opposite corner k
+        const double l2[3] = {l[0] * l[0], l[1] * l[1], l[2] * l[2]};
+        const double cs[3] = {(l2[1] + l2[2] - l2[0]) / (2 * l[1] * l[2]), (l2[2] + l2[0] - l2[1]) / (2 * l[2] * l[0]),
+                              (l2[0] + l2[1] - l2[2]) / (2 * l[0] * l[1])};
+        double u[3], w[3], c[3];
+        d3(P[1], P[0], u);
+        d3(P[2], P[0], w);
+        c[0] = u[1] * w[2] - u[2] * w[1];
+        c[1] = u[2] * w[0] - u[0] * w[2];
+        c[2] = u[0] * w[1] - u[1] * w[0];
+        const double area = 0.5 * nrm(c);
+        double bary[3] = {cs[0] * l[0], cs[1] * l[1], cs[2] * l[2]};
+        const double bs = bary[0] + bary[1] + bary[2];
+        for (int k = 0; k < 3; ++k) bary[k] /= bs;
+        const double part[3] = {bary[0] * area, bary[1] * area, bary[2] * area};
+        double quad[3] = {(part[1] + part[2]) * 0.5, (part[2] + part[0]) * 0.5, (part[0] + part[1]) * 0.5};
+        for (int k = 0; k < 3; ++k)
+            if (cs[k] < 0) {
+                quad[k] = 0.5 * area;
+                quad[(k + 1) % 3] = 0.25 * area;
+                quad[(k + 2) % 3] = 0.25 * area;
+            }
+        for (int k = 0; k < 3; ++k) vor[F[3 * f + k]] += quad[k];
+        // interior angles (igl.internal_angles)
+        for (int k = 0; k < 3; ++k) {
+            double a[3], b[3];
+            d3(P[(k + 1) % 3], P[k], a);
+            d3(P[(k + 2) % 3], P[k], b);
+            const double cc = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / (nrm(a) * nrm(b));
+            if (!(cc == cc)) {
+                any_nan = true;
+                continue;
+            }
+            const double ang = std::acos(std::min(1.0, std::max(-1.0, cc)));
+            min_ang = std::min(min_ang, ang);
+        }
+    }
+    hp.vor.resize(nV);
+    for (int i = 0; i < nV; ++i) hp.vor[i] = (float)vor[i];
+    hp.min_angle_deg = nF == 0 ? 180.0 : (any_nan ? NAN : min_ang * 180.0 / M_PI);
+}
+
+// normalization transform (processing.py:15-20) reduced on the device in FP64
+__global__ void __launch_bounds__(kB) gt_stats_k(const float* __restrict__ v, int n, double* __restrict__ part) {
+    __shared__ double red[kB];
+    __shared__ float rlo[6][kBlock];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sx = 0, sy = 0, sz = 0;
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    if (i < n) {
+        const float x = v[3 * (long long)i], y = v[3 * (long long)i + 1], z = v[3 * (long long)i + 2];
+        sx = x; sy = y; sz = z;
+        lo[0] = hi[0] = x; lo[1] = hi[1] = y; lo[2] = hi[2] = z;
+    }
+    const double a = block_sum(sx, red), b = block_sum(sy, red), c = block_sum(sz, red);
+    block_minmax6(lo, hi, rlo);
+    if (threadIdx.x == 0) {
+        double* o = part + 9 * (long long)blockIdx.x;
+        o[0] = a; o[1] = b; o[2] = c;
+        for (int k = 0; k < 6; ++k) o[3 + k] = rlo[k][0];
+    }
+}
+
 }  // namespace s3
 }  // namespace pamo
 
@@ -1866,6 +1999,68 @@ int pamo_s3_debug(pamo_s3* s, int what, const double* h_in, long long n_in, doub
     }
 }
 
+int pamo_s3_prepare_host(const float* h_v, int n_v, const int* h_f, int n_f, double scale, const double* t3,
+                         int* h_edges, int* n_e, int* h_hinges, int* n_h, float* h_voronoi, double* min_angle_deg) {
+    pamo::clear_error();
+    PAMO_CHECK(h_v && h_f && t3 && h_edges && n_e && h_hinges && n_h && h_voronoi && min_angle_deg,
+               "pamo_s3_prepare_host: null argument");
+    pamo::s3::HostPrep hp;
+    pamo::s3::host_prepare(h_v, n_v, h_f, n_f, scale, t3, hp);
+    std::copy(hp.edges.begin(), hp.edges.end(), h_edges);
+    std::copy(hp.hinges.begin(), hp.hinges.end(), h_hinges);
+    std::copy(hp.vor.begin(), hp.vor.end(), h_voronoi);
+    *n_e = (int)(hp.edges.size() / 2);
+    *n_h = (int)(hp.hinges.size() / 4);
+    *min_angle_deg = hp.min_angle_deg;
+    return 0;
+}
+
+int pamo_s3_normalization(pamo_s3* s, const float* d_v, int n_v, double* scale, double* t3) {
+    pamo::clear_error();
+    PAMO_CHECK(s && d_v && scale && t3 && n_v > 0, "pamo_s3_normalization: bad argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    const int nb = (int)pamo::grid_for(n_v, pamo::s3::kB);
+    PAMO_TRY(pamo::s3::ensure_part(c, 9LL * nb * pamo::s3::kB));
+    hipLaunchKernelGGL(pamo::s3::gt_stats_k, dim3(nb), dim3(pamo::s3::kB), 0, c.stream, d_v, n_v, c.part);
+    PAMO_HIP(hipGetLastError());
+    std::vector<double> h(9 * (size_t)nb);
+    PAMO_HIP(hipMemcpyAsync(h.data(), c.part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    double sum[3] = {0, 0, 0}, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 3; ++k) {
+            sum[k] += h[9 * (size_t)b + k];
+            lo[k] = std::min(lo[k], h[9 * (size_t)b + 3 + k]);
+            hi[k] = std::max(hi[k], h[9 * (size_t)b + 6 + k]);
+        }
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    *scale = 1.0 / ext;
+    for (int k = 0; k < 3; ++k) t3[k] = -(sum[k] / n_v) * *scale;
+    return 0;
+}
+
+int pamo_s3_set_mesh_device(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int n_f, double scale,
+                            const double* t3) {
+    pamo::clear_error();
+    PAMO_CHECK(s && d_v && d_f && t3 && n_v > 0 && n_f > 0, "pamo_s3_set_mesh_device: bad argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    PAMO_HIP(hipSetDevice(c.device));
+    std::vector<float> hv(3 * (size_t)n_v);
+    std::vector<int> hf(3 * (size_t)n_f);
+    PAMO_HIP(hipMemcpyAsync(hv.data(), d_v, sizeof(float) * hv.size(), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipMemcpyAsync(hf.data(), d_f, sizeof(int) * hf.size(), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    pamo::s3::HostPrep hp;
+    pamo::s3::host_prepare(hv.data(), n_v, hf.data(), n_f, scale, t3, hp);
+    const int use_hinge = (c.P.mask & PAMO_S3_E_HINGE) != 0;
+    const int n_e = (int)(hp.edges.size() / 2), n_h = use_hinge ? (int)(hp.hinges.size() / 4) : 0;
+    PAMO_CHECK(!use_hinge || n_h == n_e, "Number of hinges != number of edges (the mesh is not closed and manifold)");
+    const int do_nothing = hp.min_angle_deg <= 0.0 ? 1 : 0;  // min_angle_thres 0 (config.py:80)
+    return pamo_s3_set_mesh(s, d_v, n_v, d_f, n_f, scale, t3, hp.edges.data(), n_e,
+                            n_h ? hp.hinges.data() : nullptr, n_h, hp.vor.data(), do_nothing);
+}
+
 int pamo_s3_stats(pamo_s3* s, double* out8) {
     pamo::clear_error();
     PAMO_CHECK(s && out8, "pamo_s3_stats: null argument");
@@ -1877,7 +2072,7 @@ int pamo_s3_stats(pamo_s3* s, double* out8) {
     out8[4] = (double)c.stat_ls;
     out8[5] = (double)c.radius;
     out8[6] = c.do_nothing ? 1.0 : 0.0;
-    out8[7] = 0.0;
+    out8[7] = (double)c.nH;
     return 0;
 }
 

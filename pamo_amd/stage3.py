@@ -228,34 +228,37 @@ class Stage3System:
 
     def set_mesh(self, V, F, scale: float, t):
         """Stage3System.register_mesh (system.py:167-272) for the mesh to
-        project; V is in the input frame, normalised with (scale, t)."""
-        c = self.config
-        F_np = np.ascontiguousarray(np.asarray(F.cpu().numpy() if isinstance(F, torch.Tensor) else F,
-                                               dtype=np.int64))
-        V_np = np.asarray(V.cpu().numpy() if isinstance(V, torch.Tensor) else V, dtype=np.float64)
-        # the rest-shape host terms use the FP64 normalised positions, as the
-        # reference's igl calls do (system.py:209-228)
-        Vr = V_np.astype(np.float32).astype(np.float64) * scale + np.asarray(t, dtype=np.float64)
-        mask = c.energy_mask()
-        E = edges_unique(F_np) if len(F_np) else np.zeros((0, 2), np.int32)
-        H = hinges(F_np) if (mask & _hip.PAMO_S3_E_HINGE) and len(F_np) else np.zeros((0, 4), np.int32)
-        if (mask & _hip.PAMO_S3_E_HINGE) and len(H) != len(E):
-            raise AssertionError(f"Number of hinges {len(H)} != number of edges {len(E)}")
-        vor = (voronoi_areas(Vr, F_np) if len(F_np) else np.zeros(len(Vr))).astype(np.float32)
-        self.do_nothing = bool(len(F_np)) and min_internal_angle_deg(Vr, F_np) <= c.min_angle_thres
-        v = _dev_tensor(np.asarray(V_np, dtype=np.float32), torch.float32)
-        f = _dev_tensor(F_np.astype(np.int32), torch.int32) if len(F_np) else None
+        project; V is in the input frame and is normalised with (scale, t).
+        The connectivity preparation (edges, hinges, Voronoi areas, minimum
+        angle) runs in the library (pamo_s3_prepare_host)."""
+        v = _dev_tensor(V, torch.float32)
+        f = _dev_tensor(F, torch.int32)
+        tt = (ctypes.c_double * 3)(*[float(x) for x in t])
+        torch.cuda.current_stream().synchronize()
+        _hip.check(_hip.lib().pamo_s3_set_mesh_device(self.ptr, v.data_ptr(), int(v.shape[0]), f.data_ptr(),
+                                                      int(f.shape[0]), float(scale), tt), "pamo_s3_set_mesh_device")
+        st = self.stats()
+        self.n_particles = int(v.shape[0])
+        self.n_hinges = st["n_hinges"]
+        self.do_nothing = st["do_nothing"]
+        self.scale, self.t = float(scale), np.asarray(t, dtype=np.float64)
+
+    def set_mesh_prepared(self, V, F, E, H, vor, do_nothing, scale: float, t):
+        """set_mesh with the connectivity given (tests)."""
+        v = _dev_tensor(V, torch.float32)
+        f = _dev_tensor(F, torch.int32) if len(F) else None
         E = np.ascontiguousarray(E, dtype=np.int32)
         H = np.ascontiguousarray(H, dtype=np.int32)
-        vor = np.ascontiguousarray(vor)
+        vor = np.ascontiguousarray(vor, dtype=np.float32)
         tt = (ctypes.c_double * 3)(*[float(x) for x in t])
         torch.cuda.current_stream().synchronize()
         _hip.check(_hip.lib().pamo_s3_set_mesh(
-            self.ptr, v.data_ptr(), int(v.shape[0]), f.data_ptr() if f is not None else None, len(F_np),
-            float(scale), tt, E.ctypes.data if len(E) else None, len(E), H.ctypes.data if len(H) else None, len(H),
-            vor.ctypes.data, int(self.do_nothing)), "pamo_s3_set_mesh")
+            self.ptr, v.data_ptr(), int(v.shape[0]), f.data_ptr() if f is not None else None, len(F), float(scale),
+            tt, E.ctypes.data if len(E) else None, len(E), H.ctypes.data if len(H) else None, len(H),
+            vor.ctypes.data, int(bool(do_nothing))), "pamo_s3_set_mesh")
         self.n_particles = int(v.shape[0])
-        self.n_edges, self.n_hinges = len(E), len(H)
+        self.n_hinges = len(H)
+        self.do_nothing = bool(do_nothing)
         self.scale, self.t = float(scale), np.asarray(t, dtype=np.float64)
 
     def register_mesh(self, V_gt, F_gt, V, F):
@@ -277,7 +280,7 @@ class Stage3System:
         o = np.zeros(8)
         _hip.check(_hip.lib().pamo_s3_stats(self.ptr, o.ctypes.data), "pamo_s3_stats")
         return {"energy": o[0], "contacts": int(o[1]), "newton": int(o[2]), "cg": int(o[3]), "line_search": int(o[4]),
-                "radius": o[5], "do_nothing": bool(o[6])}
+                "radius": o[5], "do_nothing": bool(o[6]), "n_hinges": int(o[7])}
 
     # -- test access (no counterpart in the reference's API)
     def set_dense(self, A, b, c0):
@@ -304,14 +307,16 @@ class Stage3System:
         return out[: n.value]
 
 
-def normalization_transform_device(V: torch.Tensor):
-    """normalization_transform on a device tensor, reduced in FP64."""
-    Vd = V.double()
-    center = Vd.mean(dim=0)
-    extents = Vd.amax(dim=0) - Vd.amin(dim=0)
-    scale = 1.0 / extents.max()
-    t = -center * scale
-    return float(scale.item()), t.cpu().numpy()
+def normalization_transform_device(system: "Stage3System", V: torch.Tensor):
+    """normalization_transform of a device float32 (n, 3) tensor, reduced in
+    FP64 by the library."""
+    v = V.to(torch.float32).contiguous()
+    sc = ctypes.c_double()
+    t = (ctypes.c_double * 3)()
+    torch.cuda.current_stream().synchronize()
+    _hip.check(_hip.lib().pamo_s3_normalization(system.ptr, v.data_ptr(), int(v.shape[0]), ctypes.byref(sc), t),
+               "pamo_s3_normalization")
+    return sc.value, np.array([t[0], t[1], t[2]])
 
 
 def process(gt_V, gt_F, stage2_V, stage2_F, n_iters, system: Stage3System | None = None,
@@ -324,7 +329,7 @@ def process(gt_V, gt_F, stage2_V, stage2_F, n_iters, system: Stage3System | None
     else:
         system.clear()
     if isinstance(gt_V, torch.Tensor) and gt_V.is_cuda:
-        scale, t = normalization_transform_device(gt_V)
+        scale, t = normalization_transform_device(system, gt_V)
     else:
         scale, t = normalization_transform(np.asarray(gt_V))
     system.set_target(gt_V, gt_F, scale, t)

@@ -171,7 +171,8 @@ def test_dense_quadratic_kat_on_device():
     c.energy_calcs = ["DummyEnergyCalculator"]
     c.contact_detection_radius = 1e9
     g = stage3.Stage3System(c)
-    g.set_mesh(torch.from_numpy(d["x0"]).cuda(), np.zeros((0, 3), np.int32), 1.0, np.zeros(3))
+    g.set_mesh_prepared(torch.from_numpy(d["x0"]).cuda(), np.zeros((0, 3), np.int32), np.zeros((0, 2), np.int32),
+                        np.zeros((0, 4), np.int32), np.ones(2, np.float32), False, 1.0, np.zeros(3))
     g.set_dense(d["A"], d["b"], d["c"])
     g.step(1)
     x = g.get_vertices()

@@ -301,3 +301,28 @@ def test_step_decreases_energy_and_keeps_sheets_apart():
     assert np.all(r_in < r_out + 1e-9)
     e1 = o.energy()[5]
     assert e1 < e0
+
+
+def test_host_preparation_matches_numpy_restatement():
+    """pamo_s3_prepare_host (C++, used by the device path) against the numpy
+    functions of pamo_amd.stage3 on a closed mesh and on the two-sheet mesh."""
+    import ctypes
+    from pamo_amd import _hip
+    for kind in ("plain", "sheets"):
+        s = scene(kind)
+        V, F = s["V"], s["F"]
+        nF = len(F)
+        E = np.zeros(6 * nF, np.int32)
+        H = np.zeros(12 * nF, np.int32)
+        vor = np.zeros(len(V), np.float32)
+        ne, nh, ma = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        t = (ctypes.c_double * 3)(*s["t"])
+        _hip.check(_hip.lib().pamo_s3_prepare_host(
+            np.ascontiguousarray(V).ctypes.data, len(V), np.ascontiguousarray(F).ctypes.data, nF, s["scale"], t,
+            E.ctypes.data, ctypes.byref(ne), H.ctypes.data, ctypes.byref(nh), vor.ctypes.data, ctypes.byref(ma)),
+            "prepare")
+        Vr = V.astype(np.float64) * s["scale"] + s["t"]
+        assert np.array_equal(E[: 2 * ne.value].reshape(-1, 2), stage3.edges_unique(F))
+        assert np.array_equal(H[: 4 * nh.value].reshape(-1, 4), stage3.hinges(F))
+        assert np.allclose(vor, stage3.voronoi_areas(Vr, F).astype(np.float32), rtol=1e-6, atol=0)
+        assert ma.value == pytest.approx(stage3.min_internal_angle_deg(Vr, F), abs=1e-9)
