@@ -33,9 +33,109 @@ def parse():
     p.add_argument("--stage3", type=int, default=1, help="1/0: stage 3 on/off")
     p.add_argument("--iters", type=int, default=100000,
                    help="cap on stage-2 rounds (the reference's iter; the workload uses 100000)")
+    p.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
+                   help="headline: one 2M-face mesh; batch: many meshes, batched vs plain; serve: a request stream")
+    p.add_argument("--batch-n", type=int, default=96, help="batch: number of meshes")
+    p.add_argument("--slots", type=int, default=4, help="batch / serve: concurrent slots")
+    p.add_argument("--serve-seconds", type=float, default=5.0, help="serve: length of the arrival stream")
+    p.add_argument("--serve-rate", type=float, default=40.0, help="serve: Poisson arrival rate (jobs/s)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="after the run, one device-to-device copy of this many MiB (HBM counter calibration)")
     return p.parse_args()
+
+
+def bench_batch(args):
+    """Batched stages 2+3 over many meshes (pamo_batch_run) against the plain
+    form, one PaMO.run per mesh in sequence, on the same device-resident
+    meshes."""
+    import numpy as np
+    import torch
+    from pamo_amd import synth
+    from pamo_amd.batch import pack_csr, simplify_batch
+    from pamo_amd.pamo import PaMO
+
+    meshes, ratios = synth.batch_workload(args.batch_n)
+    nF = int(sum(len(F) for _, F in meshes))
+    print(f"[bench] batch: {len(meshes)} meshes, {nF} faces", file=sys.stderr, flush=True)
+    packed = pack_csr(meshes)
+    dev = [(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()) for V, F in meshes]
+    # warm-up: a small batch and one plain run
+    simplify_batch(meshes[:4], ratios[:4], True, n_slots=args.slots)
+    pms = [PaMO((V, F), use_stage1=False, use_stage3=True) for V, F in meshes]
+    pms[0].run(dev[0][0], dev[0][1], float(ratios[0]), 4, 1e-3, 100000, 0)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plain = [pm.run(v, f, float(r), 4, 1e-3, 100000, 0) for pm, (v, f), r in zip(pms, dev, ratios)]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    print(f"[bench] plain {t_plain:.2f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    out, secs = simplify_batch(None, ratios, True, n_slots=args.slots, packed=packed)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    same_faces = all(np.array_equal(a[1].astype(np.int64), b[1].astype(np.int64)) for a, b in zip(out, plain))
+    n = len(meshes)
+    res = {"metric": "batched meshes/s (stages 2+3)", "value": round(n / t_batch, 3), "unit": "meshes/s",
+           "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{n} ellipsoid/box unions, faces log-uniform in [20k, 800k], ratio in "
+                                  "{0.005, 0.01, 0.02, 0.05}", "faces_in": nF, "slots": args.slots},
+           "plain_meshes_per_s": round(n / t_plain, 3), "speedup_vs_plain": round(t_plain / t_batch, 3),
+           "batch_s": round(t_batch, 3), "plain_s": round(t_plain, 3), "faces_identical_to_plain": same_faces}
+    print(json.dumps(res))
+
+
+def bench_serve(args):
+    """A Poisson request stream served by pamo_serve_* with --slots slots,
+    against FIFO one-at-a-time service (one slot) of the same stream."""
+    import numpy as np
+    import torch
+    from pamo_amd import synth
+    from pamo_amd.batch import Server
+
+    pool, jobs = synth.serve_stream(args.serve_seconds, args.serve_rate)
+    dev = [(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()) for V, F in pool]
+    print(f"[bench] serve: {len(jobs)} jobs over {args.serve_seconds}s, pool of {len(pool)} meshes",
+          file=sys.stderr, flush=True)
+
+    def run_stream(n_slots):
+        srv = Server(n_slots=n_slots, stage3=True)
+        srv.result(srv.submit(*dev[0], 0.05, True))  # warm-up
+        ids = []
+        t0 = time.perf_counter()
+        for (ta, k, r, s3) in jobs:
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(dev[k][0], dev[k][1], r, s3))
+        lat, qd = [], []
+        for i in ids:
+            info = srv.poll(i, wait=True)
+            lat.append(info["latency_s"])
+            qd.append(info["queue_s"])
+            srv.result(i)
+        t_end = time.perf_counter() - t0
+        srv.close()
+        return np.array(lat), np.array(qd), t_end
+
+    lat, qd, t_all = run_stream(args.slots)
+    print(f"[bench] served in {t_all:.1f}s", file=sys.stderr, flush=True)
+    lat1, qd1, t1 = run_stream(1)
+    print(f"[bench] fifo in {t1:.1f}s", file=sys.stderr, flush=True)
+    n = len(jobs)
+    res = {"metric": "served meshes/s and p50/p99 latency (stages 2+3 on half the jobs)",
+           "value": round(n / t_all, 3), "unit": "meshes/s", "n_gpus": 1, "higher_is_better": True,
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {n} jobs, faces "
+                                  f"log-uniform in [10k, 2M] over a pool of {len(pool)} meshes, ratio U[0.002, 0.05]",
+                      "slots": args.slots},
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+           "fifo_meshes_per_s": round(n / t1, 3),
+           "fifo_p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
+           "fifo_p99_latency_s": round(float(np.percentile(lat1, 99)), 3),
+           "speedup_vs_fifo": round(t1 / t_all, 3)}
+    print(json.dumps(res))
 
 
 def main():
@@ -44,6 +144,10 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               "build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     import numpy as np
     import torch
     from pamo_amd import synth, _hip

@@ -21,6 +21,11 @@ typedef struct pamo_ctx pamo_ctx;
 int pamo_ctx_create(pamo_ctx** out, int device);
 int pamo_ctx_destroy(pamo_ctx* ctx);
 
+/* Returns a context to the state of a freshly created one, keeping its
+ * device allocations (no counterpart in the reference, which constructs a
+ * new CUDSP_Free per PaMO object, simp_cuda/pamo/__init__.py:21). */
+int pamo_ctx_reset(pamo_ctx* ctx);
+
 /* Last error message of the calling thread ("" if none).
  * Replaces the reference's printed CHECK_CUDA messages (cusimp_free.cu:17-27). */
 const char* pamo_last_error(void);
@@ -196,6 +201,45 @@ int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, dou
  * [3] CG iterations, [4] line-search evaluations, [5] current detection
  * radius, [6] do_nothing flag, [7] number of hinges. */
 int pamo_s3_stats(pamo_s3* s3, double* out8);
+
+/* ---------------------------------------------------------------- batched and served forms
+ * No counterpart in the reference, which runs one mesh per PaMO.run call
+ * (simp_cuda/pamo/__init__.py:122-205).  Each job is that whole call --
+ * centring, the stage-2 round loop, and stage 3 against the job's own mesh
+ * when requested -- and gives the same result as running it alone.  Jobs run
+ * concurrently on a pool of n_slots slots (each: stage-2 and stage-3
+ * contexts with their own HIP streams, and a worker thread). */
+
+/* A batch in CSR mesh-of-meshes layout: device d_verts (sum V,3) f32 and
+ * d_faces (sum F,3) i32 with face indices local to each mesh; host
+ * vert_off / face_off (n_meshes+1) int64 offsets (in vertices / faces);
+ * host per-mesh ratio and use_stage3.  Outputs go to d_out_verts /
+ * d_out_faces at the input offsets (capacity = input sizes), counts to host
+ * out_nv / out_nf; *seconds receives the wall time.  s3 may be NULL when no
+ * mesh uses stage 3.  Meshes are started longest first. */
+int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_meshes, const float* d_verts,
+                   const int* d_faces, const long long* vert_off, const long long* face_off, const double* ratio,
+                   const int* use_stage3, int tolerance, float threshold, long long iters, long long min_verts,
+                   float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf, double* seconds);
+
+/* Request loop: jobs are admitted in arrival order when a slot is free and
+ * the estimated HBM footprint of the running jobs plus the new one fits
+ * hbm_budget_bytes (<= 0: 200 GiB); a job larger than the budget runs alone.
+ * The caller keeps a job's inputs alive until it is done. */
+typedef struct pamo_server pamo_server;
+int pamo_serve_create(pamo_server** out, int device, int n_slots, long long hbm_budget_bytes,
+                      const pamo_s3_params* s3);
+int pamo_serve_destroy(pamo_server* srv);
+int pamo_serve_submit(pamo_server* srv, const float* d_verts, int n_verts, const int* d_faces, int n_faces,
+                      double ratio, int use_stage3, int tolerance, float threshold, long long iters,
+                      long long min_verts, long long* job_id);
+/* state: 0 queued, 1 running, 2 done, 3 failed; wait != 0 blocks until done.
+ * latency_s = completion - submission; queue_s = start - submission. */
+int pamo_serve_poll(pamo_server* srv, long long job_id, int wait, int* state, int* n_verts_out, int* n_faces_out,
+                    double* latency_s, double* queue_s);
+/* Copies a finished job's output (n_verts_out x 3 f32, n_faces_out x 3 i32)
+ * to device buffers and releases the job. */
+int pamo_serve_fetch(pamo_server* srv, long long job_id, float* d_out_verts, int* d_out_faces);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,7 @@ class S3Params(ctypes.Structure):
 SIGNATURES = [
     ("pamo_ctx_create", c_int, [ctypes.POINTER(c_vp), c_int]),
     ("pamo_ctx_destroy", c_int, [c_vp]),
+    ("pamo_ctx_reset", c_int, [c_vp]),
     ("pamo_last_error", ctypes.c_char_p, []),
     ("pamo_round", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_int,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.POINTER(c_int)]),
@@ -88,6 +89,15 @@ SIGNATURES = [
                                      ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_double)]),
     ("pamo_s3_normalization", c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     ("pamo_s3_set_mesh_device", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double)]),
+    ("pamo_batch_run", c_int, [c_int, c_int, ctypes.POINTER(S3Params), c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                               c_int, c_float, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_double)]),
+    ("pamo_serve_create", c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_ll, ctypes.POINTER(S3Params)]),
+    ("pamo_serve_destroy", c_int, [c_vp]),
+    ("pamo_serve_submit", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_int, c_float, c_ll, c_ll,
+                                  ctypes.POINTER(c_ll)]),
+    ("pamo_serve_poll", c_int, [c_vp, c_ll, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    ("pamo_serve_fetch", c_int, [c_vp, c_ll, c_vp, c_vp]),
 ]
 
 _lib = None

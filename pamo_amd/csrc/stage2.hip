@@ -171,10 +171,33 @@ static int ensure_ws(Ctx& c, long long n_items) {
 // Point-sized buffers.  The logical capacity follows the reference's growth
 // rule; persistent buffers (pts_map, undo lists, invalid table) restart at
 // zero whenever it grows, as a fresh allocation would.
+// Clears n elements of a buffer on the context's stream (a fresh allocation
+// holds zeros).
+template <typename T>
+static int zero_async(Ctx& c, T* q, size_t n) {
+    PAMO_HIP(hipMemsetAsync(q, 0, sizeof(T) * std::max<size_t>(n, 1), c.stream));
+    return 0;
+}
+
+// Point-sized buffers.  The logical capacity grows like the reference's
+// (n + n/5) and the buffers restart at zero when it does; an allocation that
+// is already large enough (a context reset for a new mesh) is cleared in
+// place instead of being freed, which would synchronise the whole device.
 static int grow_pts(Ctx& c, size_t n) {
     if (n <= c.cap_pts) return 0;
     c.cap_pts = n + n / 5;
     size_t a = c.cap_pts + 1;
+    if (a <= c.alloc_pts) {
+        a = c.alloc_pts;
+        PAMO_TRY(zero_async(c, c.p, a)); PAMO_TRY(zero_async(c, c.op, a)); PAMO_TRY(zero_async(c, c.np, a));
+        PAMO_TRY(zero_async(c, c.Q, 3 * a));
+        PAMO_TRY(zero_async(c, c.occ, a + 1)); PAMO_TRY(zero_async(c, c.pts_map, a + 1));
+        PAMO_TRY(zero_async(c, c.ring_off, a + 2)); PAMO_TRY(zero_async(c, c.ring_fill, a + 1));
+        PAMO_TRY(zero_async(c, c.undo_list, 4 * a)); PAMO_TRY(zero_async(c, c.tmp_undo, 4 * a));
+        PAMO_TRY(zero_async(c, c.undo_prev, 4 * a));
+        PAMO_TRY(zero_async(c, c.invalid, a)); PAMO_TRY(zero_async(c, c.vmin, a));
+        return 0;
+    }
     c.alloc_pts = a;
     PAMO_TRY(dalloc(&c.p, a));
     PAMO_TRY(dalloc(&c.op, a));
@@ -209,6 +232,21 @@ static int grow_tris(Ctx& c, size_t n) {
     if (n <= c.cap_tris) return 0;
     c.cap_tris = n + n / 5;
     size_t a = c.cap_tris + 1;
+    if (a <= c.alloc_tris) {  // reset context: clear in place (see grow_pts)
+        a = c.alloc_tris;
+        int* ints[] = {c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk};
+        for (int* q : ints) PAMO_TRY(zero_async(c, q, a));
+        PAMO_TRY(zero_async(c, c.ring, 3 * a)); PAMO_TRY(zero_async(c, c.rv, 3 * a));
+        PAMO_TRY(zero_async(c, c.edge_off, a + 1));
+        PAMO_TRY(zero_async(c, c.changed, a)); PAMO_TRY(zero_async(c, c.flagged, a));
+        PAMO_TRY(zero_async(c, c.face_scan, a + 1)); PAMO_TRY(zero_async(c, c.qlist, a + 1));
+        PAMO_TRY(zero_async(c, c.shuf, 4 * a + 64));
+        PAMO_TRY(zero_async(c, c.eu, c.alloc_edges)); PAMO_TRY(zero_async(c, c.ev, c.alloc_edges));
+        PAMO_TRY(zero_async(c, c.esel, c.alloc_edges + 1)); PAMO_TRY(zero_async(c, c.recs, c.alloc_edges + 1));
+        PAMO_TRY(zero_async(c, c.pflag, c.alloc_edges + 1)); PAMO_TRY(zero_async(c, c.pushed, c.alloc_edges + 1));
+        PAMO_TRY(zero_async(c, c.edge_cost, c.alloc_edges + 1));
+        return 0;
+    }
     c.alloc_tris = a;
     PAMO_TRY(dalloc(&c.ti, a)); PAMO_TRY(dalloc(&c.tj, a)); PAMO_TRY(dalloc(&c.tk, a));
     PAMO_TRY(dalloc(&c.oi, a)); PAMO_TRY(dalloc(&c.oj, a)); PAMO_TRY(dalloc(&c.ok, a));
@@ -554,6 +592,20 @@ int pamo_ctx_destroy(pamo_ctx* ctx) {
     pamo::free_all(*c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return 0;
+}
+
+// Returns a context to the state of a freshly created one (logical
+// capacities and undo bookkeeping), keeping its device allocations: the next
+// call behaves exactly as on a new context.  No counterpart in the
+// reference, where a fresh CUDSP_Free is constructed per PaMO object.
+int pamo_ctx_reset(pamo_ctx* ctx) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_ctx_reset: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    c.cap_pts = c.cap_tris = c.cap_edges = 0;
+    c.n_pts = c.n_tris = c.n_edges = c.n_rec = c.n_regular = c.n_vertices_undo = 0;
+    c.undo_iters = 0;
     return 0;
 }
 

@@ -251,3 +251,45 @@ def euler_characteristic(F: np.ndarray, n_vertices: int):
     ne = edge_use_counts(F)[0]
     used = len(np.unique(np.asarray(F).reshape(-1)))
     return used - ne + len(F)
+
+
+# faces per res^2 of the ellipsoid/box union under marching tetrahedra
+# (2,006,528 faces at res 296 for seed 1)
+_FACES_PER_RES2 = 2006528 / 296 ** 2
+
+
+def sized_union(seed: int, target_faces: float):
+    """The headline generator at the grid resolution whose face count is
+    closest to target_faces (for this seed's shapes)."""
+    res = max(8, int(round((target_faces / _FACES_PER_RES2) ** 0.5)))
+    return ellipsoid_box_union(seed, res)
+
+
+def batch_workload(n: int = 96, seed0: int = 100, f_lo: float = 20e3, f_hi: float = 800e3, rng_seed: int = 3):
+    """Batched workload: n meshes of the headline generator, seeds seed0..,
+    face counts log-uniform in [f_lo, f_hi], ratio per mesh from
+    {0.005, 0.01, 0.02, 0.05}.  Returns ([(V, F)], ratios)."""
+    rng = np.random.default_rng(rng_seed)
+    targets = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), n))
+    ratios = rng.choice([0.005, 0.01, 0.02, 0.05], n)
+    meshes = [sized_union(seed0 + i, targets[i]) for i in range(n)]
+    return meshes, ratios.astype(np.float64)
+
+
+def serve_stream(duration_s: float = 20.0, rate: float = 40.0, pool: int = 24, f_lo: float = 10e3,
+                 f_hi: float = 2e6, seed: int = 7):
+    """Served workload: Poisson arrivals at `rate` per second for duration_s;
+    each job draws a mesh from a pool of `pool` distinct meshes of the
+    headline generator with face counts log-uniform in [f_lo, f_hi], a ratio
+    uniform in [0.002, 0.05] and stage 3 with probability 0.5.  Returns
+    (pool meshes, [(arrival_s, pool index, ratio, stage3)])."""
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), pool))
+    meshes = [sized_union(700 + i, targets[i]) for i in range(pool)]
+    t, jobs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration_s:
+            break
+        jobs.append((t, int(rng.integers(pool)), float(rng.uniform(0.002, 0.05)), bool(rng.random() < 0.5)))
+    return meshes, jobs

@@ -1,0 +1,93 @@
+"""Batched and served forms (pamo_batch_run, pamo_serve_*) on the GPU.
+
+Every job is one whole PaMO.run and must give the result of that call made
+alone on a fresh object: stage-2 outputs bit for bit, stage-3 vertices
+within 2e-5 of the mesh extent (the stage-3 gradient is accumulated with
+float atomics, whose order varies between runs).
+"""
+import numpy as np
+import pytest
+import torch
+
+from pamo_amd import _hip, synth
+from pamo_amd.batch import Server, simplify_batch
+from pamo_amd.pamo import PaMO, simplify_device
+from stage2_helpers import centred
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes():
+    out = []
+    for level, seed in [(4, 0), (3, 1), (4, 2), (3, 3)]:
+        V, F = synth.icosphere_sh(level, seed)
+        out.append((V.astype(np.float32), F.astype(np.int32)))
+    V, F = synth.icosphere_sh(3, 5)
+    V2, F2 = synth.offset_sheets(V, F, rel_offset=5e-3)
+    out.append((V2.astype(np.float32), F2.astype(np.int32)))
+    return out
+
+
+def _alone(V, F, ratio, stage3):
+    pm = PaMO((V, F), use_stage1=False, use_stage3=stage3)
+    v, f = pm.run(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda(), ratio, 4, 1e-3, 100000, 0)
+    return np.asarray(v), np.asarray(f)
+
+
+def _same(a, b, V, stage3):
+    (va, fa), (vb, fb) = a, b
+    assert np.array_equal(fa.astype(np.int64), fb.astype(np.int64))
+    if stage3:
+        ext = float((V.max(0) - V.min(0)).max())
+        assert np.abs(va.astype(np.float64) - vb.astype(np.float64)).max() <= 2e-5 * ext
+    else:
+        assert np.array_equal(va.astype(np.float32).view(np.uint32), vb.astype(np.float32).view(np.uint32))
+
+
+def test_context_reset_equals_fresh_context():
+    """A context that ran another mesh, then pamo_ctx_reset, gives the fresh
+    context's result bit for bit."""
+    (Va, Fa), (Vb, Fb) = _meshes()[0], _meshes()[1]
+    print("case: reset vs fresh, icosphere L4 seed 0 after L3 seed 1", flush=True)
+    Vca, Vcb = centred(Va, Fa), centred(Vb, Fb)
+    fresh = _hip.Context(0)
+    v1, f1, h1 = simplify_device(fresh, torch.from_numpy(Vca).cuda(), torch.from_numpy(Fa).cuda(), 0.05, min_verts=0)
+    used = _hip.Context(0)
+    simplify_device(used, torch.from_numpy(Vcb).cuda(), torch.from_numpy(Fb).cuda(), 0.05, min_verts=0)
+    _hip.check(_hip.lib().pamo_ctx_reset(used.ptr), "pamo_ctx_reset")
+    v2, f2, h2 = simplify_device(used, torch.from_numpy(Vca).cuda(), torch.from_numpy(Fa).cuda(), 0.05, min_verts=0)
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(f1.cpu().numpy(), f2.cpu().numpy())
+    assert np.array_equal(v1.cpu().numpy().view(np.uint32), v2.cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("n_slots", [1, 3])
+def test_batch_matches_each_mesh_alone(n_slots):
+    meshes = _meshes()
+    ratios = [0.05, 0.1, 0.02, 0.05, 0.05]
+    stage3 = [False, True, False, True, False]
+    print(f"case: batch of {len(meshes)} meshes, {n_slots} slots", flush=True)
+    out, secs = simplify_batch(meshes, ratios, stage3, n_slots=n_slots)
+    for (V, F), r, s3, got in zip(meshes, ratios, stage3, out):
+        _same(got, _alone(V, F, r, s3), V, s3)
+
+
+def test_served_results_same_alone_and_in_the_mix():
+    """Jobs submitted together to a 3-slot server (overlapping on the
+    device) give the results of the same jobs run one at a time."""
+    meshes = _meshes()
+    ratios = [0.05, 0.1, 0.02, 0.05, 0.05]
+    stage3 = [True, False, True, False, True]
+    print("case: server, 5 jobs in a 3-slot mix vs one at a time", flush=True)
+    srv = Server(n_slots=3, stage3=True)
+    ids = [srv.submit(V, F, r, s3) for (V, F), r, s3 in zip(meshes, ratios, stage3)]
+    mixed = [srv.result(i) for i in ids]
+    srv.close()
+    solo = Server(n_slots=1, stage3=True)
+    alone = []
+    for (V, F), r, s3 in zip(meshes, ratios, stage3):
+        alone.append(solo.result(solo.submit(V, F, r, s3)))
+    solo.close()
+    for (V, F), s3, m, a in zip(meshes, stage3, mixed, alone):
+        assert m[2]["state"] == "done" and m[2]["latency_s"] > 0
+        _same(m[:2], a[:2], V, s3)
