@@ -26,6 +26,13 @@ int pamo_ctx_destroy(pamo_ctx* ctx);
  * new CUDSP_Free per PaMO object, simp_cuda/pamo/__init__.py:21). */
 int pamo_ctx_reset(pamo_ctx* ctx);
 
+/* Test option (no counterpart in the reference): with value 1 the guard
+ * query runs with a 32-entry frontier per wave, so the exact overflow walk
+ * of the frontier traversal is exercised; 0 restores the default.  The
+ * result is identical either way. */
+#define PAMO_OPT_GUARD_SMALL_STACK 1
+int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
+
 /* Last error message of the calling thread ("" if none).
  * Replaces the reference's printed CHECK_CUDA messages (cusimp_free.cu:17-27). */
 const char* pamo_last_error(void);

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpamo_hip.so")
 
 c_int, c_float, c_double, c_ll, c_vp = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
 
+PAMO_OPT_GUARD_SMALL_STACK = 1
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3
@@ -62,6 +63,7 @@ SIGNATURES = [
     ("pamo_ctx_create", c_int, [ctypes.POINTER(c_vp), c_int]),
     ("pamo_ctx_destroy", c_int, [c_vp]),
     ("pamo_ctx_reset", c_int, [c_vp]),
+    ("pamo_ctx_set_option", c_int, [c_vp, c_int, c_ll]),
     ("pamo_last_error", ctypes.c_char_p, []),
     ("pamo_round", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_int,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.POINTER(c_int)]),

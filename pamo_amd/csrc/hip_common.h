@@ -44,7 +44,7 @@ constexpr int kWave = 64;
 enum KernelId {
     K_RING_COUNT, K_RING_FILL, K_RING_SORT, K_EDGE_COUNT, K_EDGE_FILL, K_QUADRIC, K_EDGE_COST, K_PROPAGATE,
     K_SELECT_COLLAPSE, K_LINE_FLAG, K_MARK_CHANGED, K_FACE_BOX, K_MORTON, K_RADIX_HIST, K_RADIX_SCATTER,
-    K_KARRAS, K_BVH_BOXES, K_GUARD_QUERY, K_UNDO_CAND, K_COMPACT_MESH, K_SCAN_TILES, K_SCAN_ADD, K_OTHER,
+    K_KARRAS, K_BVH_BOXES, K_GUARD_QUERY, K_UNDO_CAND, K_COMPACT_MESH, K_SCAN_TILES, K_SCAN_SUMS, K_OTHER,
     K_COUNT
 };
 

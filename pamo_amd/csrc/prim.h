@@ -1,5 +1,6 @@
 // Device-wide primitives written for wave64 / LDS on gfx950:
-//   * exclusive prefix sum of int32 (reduce-then-scan, 2048 items per block);
+//   * exclusive prefix sum of int32 (tile sums, then each 2048-item tile
+//     scanned from the sum of the tiles before it);
 //   * stable LSD radix sort of (uint32 key, uint32 value) pairs, 8 bits per
 //     pass; within a tile, ranks of equal digits come from wave64 ballots
 //     (match of the 8 digit bits) and an LDS prefix across the 4 waves, so
@@ -63,30 +64,59 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles_k(const int* __restrict
     if (threadIdx.x == 0 && tile_sums) tile_sums[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(kScanBlock) scan_add_k(int* __restrict__ out, const int* __restrict__ tile_off,
+// Tiles of kScanTile items: pass 1 writes each tile's sum; pass 2 gives
+// every tile the sum of the tile sums before it (read by the whole block)
+// and scans the tile from there.  Two launches for any n, no waiting between
+// blocks; a single tile is scanned in one launch.
+__global__ void __launch_bounds__(kScanBlock) tile_sum_k(const int* __restrict__ in, int* __restrict__ sums,
                                                          long long n) {
-    const long long base = (long long)blockIdx.x * kScanTile;
-    const int add = tile_off[blockIdx.x];
-    for (int i = threadIdx.x; i < kScanTile; i += kScanBlock) {
-        long long idx = base + i;
-        if (idx < n) out[idx] += add;
+    __shared__ int lds[4];
+    const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanIpt; ++i) {
+        const long long idx = base + (long long)i * kScanBlock;
+        s += idx < n ? in[idx] : 0;
+    }
+    int total;
+    (void)block_exclusive_scan(s, lds, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_tiles_prefix_k(const int* in, int* out,
+                                                                  const int* __restrict__ sums, long long n) {
+    __shared__ int lds[4];
+    int p = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kScanBlock) p += sums[t];
+    int before;
+    (void)block_exclusive_scan(p, lds, before);
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanIpt;
+    int v[kScanIpt];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanIpt; ++i) {
+        const long long idx = base + i;
+        v[i] = idx < n ? in[idx] : 0;
+        s += v[i];
+    }
+    int total;
+    int ex = before + block_exclusive_scan(s, lds, total);
+#pragma unroll
+    for (int i = 0; i < kScanIpt; ++i) {
+        const long long idx = base + i;
+        if (idx < n) out[idx] = ex;
+        ex += v[i];
     }
 }
 
 // Workspace (ints) needed by scan_exclusive for n elements.
-inline long long scan_workspace(long long n) {
-    long long w = 0;
-    while (n > kScanTile) {
-        n = (n + kScanTile - 1) / kScanTile;
-        w += n + 16;
-    }
-    return w + 16;
-}
+inline long long scan_workspace(long long n) { return (n + kScanTile - 1) / kScanTile + 16; }
 
-// out[i] = sum_{j<i} in[j]; in may alias out.
+// out[i] = sum_{j<i} in[j]; in may alias out (each tile is read and written
+// by one block only).
 inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStream_t st) {
     if (n <= 0) return 0;
-    long long nb = (n + kScanTile - 1) / kScanTile;
+    const long long nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 1) {
         prof_begin(K_SCAN_TILES, 8.0 * n, st);
         hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(kScanBlock), 0, st, in, out, (int*)nullptr, n);
@@ -95,14 +125,14 @@ inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStre
         return 0;
     }
     int* sums = ws;
-    prof_begin(K_SCAN_TILES, 8.0 * n, st);
-    hipLaunchKernelGGL(scan_tiles_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, sums, n);
-    prof_end(K_SCAN_TILES, st);
+    prof_begin(K_SCAN_SUMS, 4.0 * n, st);
+    hipLaunchKernelGGL(tile_sum_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, sums, n);
+    prof_end(K_SCAN_SUMS, st);
     PAMO_HIP(hipGetLastError());
-    PAMO_TRY(scan_exclusive(sums, sums, nb, ws + nb + 16, st));
-    prof_begin(K_SCAN_ADD, 8.0 * n, st);
-    hipLaunchKernelGGL(scan_add_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, out, (const int*)sums, n);
-    prof_end(K_SCAN_ADD, st);
+    prof_begin(K_SCAN_TILES, 8.0 * n, st);
+    hipLaunchKernelGGL(scan_tiles_prefix_k, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out,
+                       (const int*)sums, n);
+    prof_end(K_SCAN_TILES, st);
     PAMO_HIP(hipGetLastError());
     return 0;
 }

@@ -33,7 +33,7 @@ void clear_error() { g_err.clear(); }
 static const char* kKernelNames[K_COUNT] = {
     "ring_count", "ring_fill", "ring_sort", "edge_count", "edge_fill", "quadric", "edge_cost", "propagate",
     "select_collapse", "line_flag", "mark_changed", "face_box", "morton", "radix_hist", "radix_scatter",
-    "karras", "bvh_boxes", "guard_query", "undo_cand", "compact_mesh", "scan_tiles", "scan_add", "other"};
+    "karras", "bvh_boxes", "guard_query", "undo_cand", "compact_mesh", "scan_tiles", "scan_sums", "other"};
 
 struct Profiler {
     uint64_t mask = 0;
@@ -109,6 +109,7 @@ struct Ctx {
     int *ti = nullptr, *tj = nullptr, *tk = nullptr, *oi = nullptr, *oj = nullptr, *ok = nullptr;
     int *ni = nullptr, *nj = nullptr, *nk = nullptr;
     int* ring = nullptr;  // 3 * alloc_tris
+    int *ring_raw = nullptr, *ring_own = nullptr;  // unsorted ring slots and their vertex
     int4* rv = nullptr;   // 3 * alloc_tris
     int* edge_off = nullptr;
     uint8_t *changed = nullptr, *flagged = nullptr;
@@ -124,6 +125,7 @@ struct Ctx {
     int *eu = nullptr, *ev = nullptr;
     uint32_t* edge_cost = nullptr;
     int *esel = nullptr, *recs = nullptr, *pflag = nullptr, *pushed = nullptr;
+    int* eord = nullptr;  // edge ids grouped by smaller endpoint (edge-cost launch order)
     int* shuf = nullptr;
     int* ws = nullptr;
     int* dcnt = nullptr;
@@ -131,6 +133,7 @@ struct Ctx {
     // last round, host view
     int n_pts = 0, n_tris = 0, n_edges = 0, n_rec = 0, n_regular = 0, n_vertices_undo = 0;
     int undo_iters = 0;
+    int guard_small_stack = 0;  // test option: 32-entry guard frontier
     Profiler prof;
 };
 
@@ -225,6 +228,7 @@ static int alloc_edges(Ctx& c, size_t ae) {
     PAMO_TRY(dalloc(&c.pflag, ae + 1));
     PAMO_TRY(dalloc(&c.pushed, ae + 1));
     PAMO_TRY(dalloc(&c.edge_cost, ae + 1));
+    PAMO_TRY(dalloc(&c.eord, ae + 1));
     return 0;
 }
 
@@ -237,6 +241,7 @@ static int grow_tris(Ctx& c, size_t n) {
         int* ints[] = {c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk};
         for (int* q : ints) PAMO_TRY(zero_async(c, q, a));
         PAMO_TRY(zero_async(c, c.ring, 3 * a)); PAMO_TRY(zero_async(c, c.rv, 3 * a));
+        PAMO_TRY(zero_async(c, c.ring_raw, 3 * a)); PAMO_TRY(zero_async(c, c.ring_own, 3 * a));
         PAMO_TRY(zero_async(c, c.edge_off, a + 1));
         PAMO_TRY(zero_async(c, c.changed, a)); PAMO_TRY(zero_async(c, c.flagged, a));
         PAMO_TRY(zero_async(c, c.face_scan, a + 1)); PAMO_TRY(zero_async(c, c.qlist, a + 1));
@@ -252,6 +257,8 @@ static int grow_tris(Ctx& c, size_t n) {
     PAMO_TRY(dalloc(&c.oi, a)); PAMO_TRY(dalloc(&c.oj, a)); PAMO_TRY(dalloc(&c.ok, a));
     PAMO_TRY(dalloc(&c.ni, a)); PAMO_TRY(dalloc(&c.nj, a)); PAMO_TRY(dalloc(&c.nk, a));
     PAMO_TRY(dalloc(&c.ring, 3 * a));
+    PAMO_TRY(dalloc(&c.ring_raw, 3 * a));
+    PAMO_TRY(dalloc(&c.ring_own, 3 * a));
     PAMO_TRY(dalloc(&c.rv, 3 * a));
     PAMO_TRY(dalloc(&c.edge_off, a + 1));
     PAMO_TRY(dalloc(&c.changed, a));
@@ -293,10 +300,10 @@ static int grow_edges(Ctx& c, size_t n) {
 
 static void free_all(Ctx& c) {
     void* ptrs[] = {c.p, c.op, c.np, c.Q, c.occ, c.pts_map, c.ring_off, c.ring_fill, c.undo_list, c.tmp_undo,
-                    c.undo_prev, c.invalid, c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk, c.ring, c.rv,
+                    c.undo_prev, c.invalid, c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk, c.ring, c.ring_raw, c.ring_own, c.rv,
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
-                    c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed,
+                    c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
                     c.shuf, c.ws, c.dcnt};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -384,9 +391,14 @@ static int run_guard(Ctx& c, int nT) {
     LAUNCH(query_flag_k, nT + 1, c.bval, c.dcnt, c.changed, nT, c.face_scan);
     PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
-    LAUNCHP(K_GUARD_QUERY, 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT,
-            guard_query_k, (long long)nT << kGuardLog, m, c.qlist, c.dcnt, c.changed, c.fbox,
-            (const float4*)c.bbox, 1e-3f, c.flagged);
+    const long long guard_threads = (((long long)nT + kGuardQ - 1) / kGuardQ) * kWave;
+    const double guard_bytes = 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT;
+    if (c.guard_small_stack)
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, guard_query_k<kGuardStackSmall>, guard_threads, m, c.qlist, c.dcnt,
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged);
+    else
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, guard_query_k<kGuardStack>, guard_threads, m, c.qlist, c.dcnt,
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged);
     return 0;
 }
 
@@ -426,9 +438,10 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
     LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP, ring_count_k, nT, m, nT, c.ring_off);
     PAMO_TRY(scan_exclusive(c.ring_off, c.ring_off, nP + 1, c.ws, c.stream));
-    LAUNCHP(K_RING_FILL, 12.0 * nT + 12.0 * nT + 8.0 * nP, ring_fill_k, nT, m, nT, c.ring_off, c.ring_fill, c.ring);
-    LAUNCHP(K_RING_SORT, 4.0 * nP + 24.0 * nT + 12.0 * nT + 48.0 * nT, ring_sort_k, nP, m, c.ring_off, c.ring, c.rv,
-            nP);
+    LAUNCHP(K_RING_FILL, 12.0 * nT + 12.0 * nT + 32.0 * nT, ring_fill_k, nT, m, nT, c.ring_off, c.ring_fill,
+            c.ring_raw, c.ring_own);
+    LAUNCHP(K_RING_SORT, 4.0 * nP + 24.0 * nT + 12.0 * nT + 60.0 * nT, ring_rank_k, 3LL * nT, m, c.ring_off,
+            c.ring_raw, c.ring_own, 3 * nT, c.ring_off + nP, c.ring, c.rv);
     // edges
     LAUNCHP(K_EDGE_COUNT, 16.0 * nT, edge_count_k, nT + 1, m, nT, c.edge_off);
     PAMO_TRY(scan_exclusive(c.edge_off, c.edge_off, nT + 1, c.ws, c.stream));
@@ -440,9 +453,14 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev);
     // quadrics + cost
     LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
+    // edge-cost launch order; ring_fill is free once the rings are sorted
+    PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
+    LAUNCH(edge_order_count_k, nE, c.eu, nE, c.ring_fill);
+    PAMO_TRY(scan_exclusive(c.ring_fill, c.ring_fill, nP + 1, c.ws, c.stream));
+    LAUNCH(edge_order_scatter_k, nE, c.eu, nE, c.ring_fill, c.eord);
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
-    LAUNCHP(K_EDGE_COST, 8.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE, edge_cost_k,
-            nE, m, c.ring_off, c.rv, c.eu, c.ev, nE, c.invalid, c.Q, cp, c.edge_cost);
+    LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE,
+            edge_cost_k, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid, c.Q, cp, c.edge_cost);
     // independent set + collapse
     PAMO_TRY(fill<unsigned long long>((unsigned long long*)c.vmin, ~0ull, nP, c.stream));
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
@@ -599,6 +617,16 @@ int pamo_ctx_destroy(pamo_ctx* ctx) {
 // capacities and undo bookkeeping), keeping its device allocations: the next
 // call behaves exactly as on a new context.  No counterpart in the
 // reference, where a fresh CUDSP_Free is constructed per PaMO object.
+int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_ctx_set_option: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    PAMO_CHECK(option == PAMO_OPT_GUARD_SMALL_STACK, "pamo_ctx_set_option: unknown option");
+    PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: value must be 0 or 1");
+    c.guard_small_stack = (int)value;
+    return 0;
+}
+
 int pamo_ctx_reset(pamo_ctx* ctx) {
     pamo::clear_error();
     PAMO_CHECK(ctx != nullptr, "pamo_ctx_reset: null context");

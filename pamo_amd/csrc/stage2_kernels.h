@@ -159,35 +159,41 @@ __global__ void ring_count_k(Mesh m, int nT, int* __restrict__ cnt) {
 }
 
 __global__ void ring_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ fill,
-                            int* __restrict__ ring) {
+                            int* __restrict__ raw, int* __restrict__ owner) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nT) return;
     int a = m.i[f], b = m.j[f], c = m.k[f];
     if (face_dead(a, b, c)) return;
-    ring[off[a] + atomicAdd(&fill[a], 1)] = f;
-    ring[off[b] + atomicAdd(&fill[b], 1)] = f;
-    ring[off[c] + atomicAdd(&fill[c], 1)] = f;
+    int q = off[a] + atomicAdd(&fill[a], 1);
+    raw[q] = f;
+    owner[q] = a;
+    q = off[b] + atomicAdd(&fill[b], 1);
+    raw[q] = f;
+    owner[q] = b;
+    q = off[c] + atomicAdd(&fill[c], 1);
+    raw[q] = f;
+    owner[q] = c;
 }
 
-// Canonical ring order (ascending face index), then the packed int4 entries.
-__global__ void ring_sort_k(Mesh m, const int* __restrict__ off, int* __restrict__ ring, int4* __restrict__ rv,
-                            int nP) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nP) return;
-    int b = off[v], e = off[v + 1];
-    for (int q = b + 1; q < e; ++q) {
-        int x = ring[q];
-        int r = q - 1;
-        while (r >= b && ring[r] > x) {
-            ring[r + 1] = ring[r];
-            --r;
-        }
-        ring[r + 1] = x;
-    }
+// Canonical ring order (ascending face index) with the packed int4 entries,
+// one lane per ring slot: the slot's place is its rank among the slots of its
+// vertex, ties (a face holding the vertex twice) broken by slot.  Independent
+// loads only, so a high-valence ring costs no serial chain.
+__global__ void __launch_bounds__(kBlock) ring_rank_k(Mesh m, const int* __restrict__ off,
+                                                      const int* __restrict__ raw, const int* __restrict__ owner,
+                                                      int n_slots_cap, const int* __restrict__ n_slots,
+                                                      int* __restrict__ ring, int4* __restrict__ rv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots_cap || s >= *n_slots) return;
+    const int v = owner[s], b = off[v], e = off[v + 1];
+    const int x = raw[s];
+    int r = 0;
     for (int q = b; q < e; ++q) {
-        int f = ring[q];
-        rv[q] = make_int4(m.i[f], m.j[f], m.k[f], f);
+        const int y = raw[q];
+        r += (y < x || (y == x && q < s)) ? 1 : 0;
     }
+    ring[b + r] = x;
+    rv[b + r] = make_int4(m.i[x], m.j[x], m.k[x], x);
 }
 
 __global__ void edge_count_k(Mesh m, int nT, int* __restrict__ cnt) {
@@ -257,17 +263,36 @@ struct CostParams {
     int is_stuck;
 };
 
-// compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge.
+// Launch order of the edge-cost lanes: edge ids grouped by their smaller
+// endpoint (counting sort: histogram, exclusive scan, scatter).  Vertex ids of
+// a mesh are spatially coherent while edge ids follow the shuffled face
+// order, so in this order the lanes of a wave read neighbouring rings,
+// positions and quadrics.  The order inside one group depends on atomic
+// timing; it changes only which lane computes an edge, never a result.
+__global__ void edge_order_count_k(const int* __restrict__ eu, int nE, int* __restrict__ cnt) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nE) atomicAdd(&cnt[eu[e]], 1);
+}
+
+__global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __restrict__ cursor,
+                                     int* __restrict__ order) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nE) order[atomicAdd(&cursor[eu[e]], 1)] = e;
+}
+
+// compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge; lane
+// t takes edge order[t] and writes cost[order[t]].
 constexpr int kLinkRing = 24;
 
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
                                                       const int4* __restrict__ rv, const int* __restrict__ eu,
-                                                      const int* __restrict__ ev, int nE,
-                                                      const uint8_t* __restrict__ invalid,
+                                                      const int* __restrict__ ev, const int* __restrict__ order,
+                                                      int nE, const uint8_t* __restrict__ invalid,
                                                       const float4* __restrict__ Q, CostParams cp,
                                                       uint32_t* __restrict__ cost) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nE) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nE) return;
+    const int e = order[t];
     const int u = eu[e], v = ev[e];
     if (invalid[u] && invalid[v]) {
         cost[e] = 0xFFFFFFFFu;
@@ -278,30 +303,38 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     // link condition: successors of u and v in their rings (:405-444); the
     // successors of v's ring sit in registers (static indices) when it has
     // at most kLinkRing faces
+    // dup = #{(a, b) : succ_u(a) == succ_v(b)} is symmetric in u and v and
+    // only its total matters (> 2 rejects, != 2 keeps the stale cost), so the
+    // smaller ring sits in registers, kLinkRing successors per pass, and the
+    // larger one streams past it four faces per load batch
+    const bool swap = (ve - vb) > (ue - ub);
+    const int ib = swap ? ub : vb, ie = swap ? ue : ve, iw = swap ? u : v;
+    const int ob = swap ? vb : ub, oe = swap ? ve : ue, ow = swap ? v : u;
     int dup = 0;
-    if (ve - vb <= kLinkRing) {
+    for (int c0 = ib; c0 < ie; c0 += kLinkRing) {
         int sv[kLinkRing];
 #pragma unroll
-        for (int b = 0; b < kLinkRing; ++b) sv[b] = vb + b < ve ? succ_in(rv[vb + b], v) : -2;
-        for (int a = ub; a < ue; ++a) {
-            const int su = succ_in(rv[a], u);
+        for (int b = 0; b < kLinkRing; ++b) sv[b] = c0 + b < ie ? succ_in(rv[c0 + b], iw) : -2;
+        int a = ob;
+        for (; a + 4 <= oe; a += 4) {
+            const int4 t0 = rv[a], t1 = rv[a + 1], t2 = rv[a + 2], t3 = rv[a + 3];
+            const int s0 = succ_in(t0, ow), s1 = succ_in(t1, ow), s2 = succ_in(t2, ow), s3 = succ_in(t3, ow);
 #pragma unroll
-            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
+            for (int b = 0; b < kLinkRing; ++b)
+                dup += (s0 == sv[b] ? 1 : 0) + (s1 == sv[b] ? 1 : 0) + (s2 == sv[b] ? 1 : 0) + (s3 == sv[b] ? 1 : 0);
             if (dup > 2) {
                 cost[e] = 0xFFFFFFFFu;
                 return;
             }
         }
-    } else {
-        for (int a = ub; a < ue; ++a) {
-            const int su = succ_in(rv[a], u);
-            for (int b = vb; b < ve; ++b) {
-                dup += (su == succ_in(rv[b], v)) ? 1 : 0;
-                if (dup > 2) {
-                    cost[e] = 0xFFFFFFFFu;
-                    return;
-                }
-            }
+        for (; a < oe; ++a) {
+            const int su = succ_in(rv[a], ow);
+#pragma unroll
+            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
+        }
+        if (dup > 2) {
+            cost[e] = 0xFFFFFFFFu;
+            return;
         }
     }
     if (dup != 2) return;
@@ -920,96 +953,143 @@ __device__ __forceinline__ void eval_pair(Mesh m, int a, int b, float epsilon, u
     }
 }
 
-constexpr int kQueue = 128;  // per-wave pair queue (< 64 pending + one append of <= 64)
-constexpr int kGuardLog = 3;  // 8 lanes per guard query
+constexpr int kQueue = 128;   // per-wave pair queue (< 64 pending + one append of <= 64)
+constexpr int kGuardQ = 16;   // guard queries per wave
+constexpr int kGuardStack = 512;       // per-wave frontier entries
+constexpr int kGuardStackSmall = 32;   // test variant: forces the overflow walk
 
-// Candidate pairs: every queried face a against every live face b != a whose
-// box overlaps a's box (closed intervals).  A pair of queried faces is taken
-// from the larger index only, and pairs sharing two vertex indices are
+// Candidate filter for queried face a (vertex ids ia0..ia2) and a live face b
+// whose box overlaps a's (closed intervals): b != a; a pair of queried faces
+// is taken from the larger index only; pairs sharing two vertex indices are
 // dropped before any arithmetic (the reference rejects them after its
 // coplanarity test in every outcome).
+__device__ __forceinline__ bool guard_candidate(Mesh m, const uint8_t* __restrict__ changed, int a, int ia0,
+                                                int ia1, int ia2, int b) {
+    if (b == a || (changed[b] && b < a)) return false;
+    const int b0 = m.i[b], b1 = m.j[b], b2 = m.k[b];
+    const int sh = (ia0 == b0) + (ia0 == b1) + (ia0 == b2) + (ia1 == b0) + (ia1 == b1) + (ia1 == b2) +
+                   (ia2 == b0) + (ia2 == b1) + (ia2 == b2);
+    return sh != 2;
+}
+
+// Stackless walk of the subtree below internal node x: from its left child
+// until x's escape, evaluating candidates in this lane.
+__device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restrict__ changed,
+                                                const float4* __restrict__ node, int n, int x, int stop,
+                                                float4 qlo, float4 qhi, int a, int ia0, int ia1, int ia2,
+                                                float epsilon, uint8_t* __restrict__ flagged) {
+    while (x != stop) {
+        const float4 lo = node[2 * (long long)x], hi = node[2 * (long long)x + 1];
+        const int lw = __float_as_int(lo.w), esc = __float_as_int(hi.w);
+        if (overlap(qlo, qhi, lo, hi)) {
+            if (x >= n - 1) {
+                if (guard_candidate(m, changed, a, ia0, ia1, ia2, lw)) eval_pair(m, a, lw, epsilon, flagged);
+                x = esc;
+            } else {
+                x = lw;
+            }
+        } else {
+            x = esc;
+        }
+    }
+}
+
+// Guard queries (self_intersect.cuh:151-392 restricted to changed faces):
+// every queried face a against every live face b whose box overlaps a's.
 //
-// Each lane walks the tree for its own query without a stack (parent links);
-// lanes that reach a candidate append it to a per-wave LDS queue, and the
-// whole wave evaluates the FP64 predicates 64 pairs at a time, so the long
-// predicate code runs with every lane busy instead of one lane at a time.
-// Candidate pairs: every queried face a against every live face b != a whose
-// box overlaps a's box (closed intervals).  A pair of queried faces is taken
-// from the larger index only, and pairs sharing two vertex indices are
-// dropped before any arithmetic (the reference rejects them after its
-// coplanarity test in every outcome).
-//
-// kGuardLanes lanes serve one query, each walking one subtree of the
-// rope-linked tree, one node per loop iteration; candidates go to a
-// per-wave LDS queue and the whole wave evaluates the FP64 predicates 64
-// pairs at a time, so the long predicate code runs with every lane busy and
-// no lane waits for another's walk.
+// A wave serves kGuardQ queries with one shared LIFO frontier of
+// (query, node) entries in LDS: each step pops up to 64 entries (one per
+// lane), tests their boxes, pushes both children of overlapping internal
+// nodes and appends overlapping leaves to a pair queue, which the wave
+// evaluates 64 pairs at a time.  The critical path is then the wave's total
+// node count / 64 plus the tree depth, instead of the longest single walk.
+// When the frontier is full, a lane walks the rest of its subtree by escape
+// links (exact, only slower).  The flags are idempotent, so the visiting
+// order does not change the result.
+template <int kStackCap>
 __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
                                                         const int* __restrict__ cnt,
                                                         const uint8_t* __restrict__ changed,
                                                         const float4* __restrict__ fbox,
-                                                        const float4* __restrict__ node, float epsilon,
+                                                        const float4* __restrict__ node,
+                                                        const int* __restrict__ right, float epsilon,
                                                         uint8_t* __restrict__ flagged) {
-    __shared__ int qa[kBlock / kWave][kQueue], qb[kBlock / kWave][kQueue];
+    constexpr int kW = kBlock / kWave;
+    __shared__ int2 stk[kW][kStackCap];
+    __shared__ int2 pq[kW][kQueue];
+    __shared__ float4 qbox[kW][kGuardQ][2];
+    __shared__ int4 qid[kW][kGuardQ];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = (int)(t >> kGuardLog), sub = (int)(t & ((1 << kGuardLog) - 1));
+    const long long q0 = ((long long)blockIdx.x * kW + w) * kGuardQ;
     const int nq = cnt[C_NQUERY];
     const int n = cnt[C_NLIVE];
-    if (__ballot(q < nq) == 0 || n <= 1) return;  // uniform per wave
-    int x = -1, stop = -1;
-    if (q < nq) x = subtree_start<kGuardLog>(node, n, sub, &stop);
-    bool active = x >= 0;
-    int a = 0, ia0 = 0, ia1 = 0, ia2 = 0;
-    float4 qlo = make_float4(0.f, 0.f, 0.f, 0.f), qhi = qlo;
-    if (active) {
-        a = list[q];
-        ia0 = m.i[a]; ia1 = m.j[a]; ia2 = m.k[a];
-        qlo = fbox[2 * (long long)a];
-        qhi = fbox[2 * (long long)a + 1];
+    if (q0 >= nq || n <= 1) return;  // uniform per wave
+    const int nl = nq - q0 < kGuardQ ? (int)(nq - q0) : kGuardQ;
+    if (lane < nl) {
+        const int a = list[q0 + lane];
+        qid[w][lane] = make_int4(a, m.i[a], m.j[a], m.k[a]);
+        qbox[w][lane][0] = fbox[2 * (long long)a];
+        qbox[w][lane][1] = fbox[2 * (long long)a + 1];
+        stk[w][lane] = make_int2(lane, 0);  // root
     }
-    int qn = 0;  // wave-uniform queue length
-    while (true) {
-        int found = -1;
-        if (active) {
+    wave_sync();
+    int size = nl, qn = 0;  // wave-uniform
+    while (size > 0) {
+        const int take = size < kWave ? size : kWave;
+        int l = 0, x = -1;
+        if (lane < take) {
+            const int2 e = stk[w][size - take + lane];
+            l = e.x;
+            x = e.y;
+        }
+        size -= take;
+        int c0 = -1, c1 = -1, esc = -1, leaf = -1;
+        float4 qlo = make_float4(0.f, 0.f, 0.f, 0.f), qhi = qlo;
+        if (x >= 0) {
             const float4 lo = node[2 * (long long)x], hi = node[2 * (long long)x + 1];
-            const int lw = __float_as_int(lo.w), esc = __float_as_int(hi.w);
+            const int r = x < n - 1 ? right[x] : -1;
+            qlo = qbox[w][l][0];
+            qhi = qbox[w][l][1];
             if (overlap(qlo, qhi, lo, hi)) {
                 if (x >= n - 1) {
-                    const int b = lw;
-                    if (b != a && !(changed[b] && b < a)) {
-                        const int b0 = m.i[b], b1 = m.j[b], b2 = m.k[b];
-                        const int sh = (ia0 == b0) + (ia0 == b1) + (ia0 == b2) + (ia1 == b0) + (ia1 == b1) +
-                                       (ia1 == b2) + (ia2 == b0) + (ia2 == b1) + (ia2 == b2);
-                        if (sh != 2) found = b;
-                    }
-                    x = esc;
+                    leaf = __float_as_int(lo.w);
                 } else {
-                    x = lw;
+                    c0 = __float_as_int(lo.w);
+                    c1 = r;
+                    esc = __float_as_int(hi.w);
                 }
+            }
+        }
+        wave_sync();  // every pop has read its entry before pushes reuse the slots
+        const uint64_t lm = __ballot(leaf >= 0);
+        if (leaf >= 0) pq[w][qn + lanes_below(lm)] = make_int2(l, leaf);
+        qn += (int)__popcll(lm);
+        const uint64_t pm = __ballot(c0 >= 0);
+        const int below = lanes_below(pm);
+        const bool fits = size + 2 * (below + 1) <= kStackCap;  // pushes that fit form a prefix
+        if (c0 >= 0) {
+            if (fits) {
+                stk[w][size + 2 * below] = make_int2(l, c0);
+                stk[w][size + 2 * below + 1] = make_int2(l, c1);
             } else {
-                x = esc;
+                const int4 id = qid[w][l];
+                guard_walk_subtree(m, changed, node, n, c0, esc, qlo, qhi, id.x, id.y, id.z, id.w, epsilon, flagged);
             }
-            if (x == stop) active = false;
         }
-        const uint64_t fm = __ballot(found >= 0);
-        if (found >= 0) {
-            const int slot = qn + lanes_below(fm);
-            qa[w][slot] = a;
-            qb[w][slot] = found;
-        }
-        qn += (int)__popcll(fm);
-        const bool any_active = __ballot(active) != 0;
-        if (qn >= kWave || (!any_active && qn > 0)) {
-            wave_sync();
-            while (qn >= kWave || (!any_active && qn > 0)) {
-                const int take = qn < kWave ? qn : kWave;
-                if (lane < take) eval_pair(m, qa[w][qn - take + lane], qb[w][qn - take + lane], epsilon, flagged);
-                qn -= take;
-            }
+        size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));
+        wave_sync();
+        if (qn >= kWave) {
+            const int2 e = pq[w][qn - kWave + lane];
+            const int4 id = qid[w][e.x];
+            if (guard_candidate(m, changed, id.x, id.y, id.z, id.w, e.y)) eval_pair(m, id.x, e.y, epsilon, flagged);
+            qn -= kWave;
             wave_sync();
         }
-        if (!any_active) break;
+    }
+    if (lane < qn) {
+        const int2 e = pq[w][lane];
+        const int4 id = qid[w][e.x];
+        if (guard_candidate(m, changed, id.x, id.y, id.z, id.w, e.y)) eval_pair(m, id.x, e.y, epsilon, flagged);
     }
 }
 

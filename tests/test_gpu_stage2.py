@@ -98,16 +98,42 @@ def test_round_parity_undo_path(torch_cuda, oracle_lib):
     assert stats["undo_rounds"] > 0, "case did not exercise the undo loop"
 
 
-def test_fused_loop_matches_oracle(torch_cuda, oracle_lib):
+@pytest.mark.parametrize("level", [5, 6])
+def test_fused_loop_matches_oracle(torch_cuda, oracle_lib, level):
+    """Level 6 (81920 faces) runs its first rounds through the tiled prefix
+    scan (arrays over 65536 items), level 5 through the one-workgroup scan."""
     from pamo_amd import synth, _hip
     from pamo_amd.pamo import simplify_device
     torch = torch_cuda
-    V, F = synth.icosphere_sh(5, 0)
+    V, F = synth.icosphere_sh(level, 0)
     Vc = centred(V, F)
-    print("case: fused loop icosphere L5", flush=True)
+    print(f"case: fused loop icosphere L{level}, {len(F)} faces", flush=True)
     ctx = _hip.Context(0)
     vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.05, min_verts=0)
     vo, fo, ho = oracle_lib.run(Vc, F, 0.05, min_verts=0)
+    assert np.array_equal(hg, ho)
+    assert np.array_equal(fg.cpu().numpy(), fo)
+    assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32))
+
+
+@pytest.mark.parametrize("small_stack", [0, 1])
+def test_guard_frontier_matches_oracle(torch_cuda, oracle_lib, small_stack):
+    """Undo-heavy input (offset sheets) through the fused loop.  With the
+    32-entry guard frontier almost every wave overflows and walks subtrees
+    by escape links; both frontier sizes give the oracle's result."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(4, 3)
+    V2, F2 = synth.offset_sheets(V, F, rel_offset=5e-3)
+    Vc = centred(V2, F2)
+    print(f"case: guard frontier small_stack={small_stack}, offset sheets L4 seed 3", flush=True)
+    ctx = _hip.Context(0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_GUARD_SMALL_STACK, small_stack), "set_option")
+    vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F2).cuda(), 0.05, min_verts=0)
+    octx = oracle_lib.Round()
+    vo, fo, ho = oracle_lib.run(Vc, F2, 0.05, min_verts=0, ctx=octx)
+    assert oracle_lib.lib().orc_guard_pairs(octx.ctx) > 0, "case evaluated no guard pairs"
     assert np.array_equal(hg, ho)
     assert np.array_equal(fg.cpu().numpy(), fo)
     assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32))
