@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
                    help="headline: one 2M-face mesh; batch: many meshes, batched vs plain; serve: a request stream")
     p.add_argument("--batch-n", type=int, default=96, help="batch: number of meshes")
-    p.add_argument("--slots", type=int, default=4, help="batch / serve: concurrent slots")
+    p.add_argument("--slots", type=int, default=8, help="batch / serve: concurrent slots")
     p.add_argument("--serve-seconds", type=float, default=5.0, help="serve: length of the arrival stream")
     p.add_argument("--serve-rate", type=float, default=40.0, help="serve: Poisson arrival rate (jobs/s)")
     p.add_argument("--calib-copy-mb", type=int, default=0,
@@ -61,17 +61,32 @@ def bench_batch(args):
     dev = [(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()) for V, F in meshes]
     # warm-up: a small batch and one plain run
     simplify_batch(meshes[:4], ratios[:4], True, n_slots=args.slots)
-    pms = [PaMO((V, F), use_stage1=False, use_stage3=True) for V, F in meshes]
-    pms[0].run(dev[0][0], dev[0][1], float(ratios[0]), 4, 1e-3, 100000, 0)
+    PaMO(meshes[0], use_stage1=False, use_stage3=True).run(dev[0][0], dev[0][1], float(ratios[0]), 4, 1e-3,
+                                                           100000, 0)
     torch.cuda.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
+    # plain form: one PaMO object and one run per mesh, as a caller of the
+    # reference does
     t0 = time.perf_counter()
-    plain = [pm.run(v, f, float(r), 4, 1e-3, 100000, 0) for pm, (v, f), r in zip(pms, dev, ratios)]
+    plain = []
+    for k, ((V, F), (v, f), r) in enumerate(zip(meshes, dev, ratios)):
+        plain.append(PaMO((V, F), use_stage1=False, use_stage3=True).run(v, f, float(r), 4, 1e-3, 100000, 0))
+        if (k + 1) % 16 == 0:
+            print(f"[bench] plain {k + 1}/{len(meshes)} meshes, {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+                  flush=True)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     print(f"[bench] plain {t_plain:.2f}s", file=sys.stderr, flush=True)
+    done = []
+
+    def on_done(mesh, status):
+        done.append(mesh)
+        if len(done) % 8 == 0 or status != 0:
+            print(f"[bench] batch {len(done)}/{len(meshes)} meshes (mesh {mesh} status {status})", file=sys.stderr,
+                  flush=True)
+
     t0 = time.perf_counter()
-    out, secs = simplify_batch(None, ratios, True, n_slots=args.slots, packed=packed)
+    out, secs = simplify_batch(None, ratios, True, n_slots=args.slots, packed=packed, on_done=on_done)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     same_faces = all(np.array_equal(a[1].astype(np.int64), b[1].astype(np.int64)) for a, b in zip(out, plain))
@@ -90,7 +105,7 @@ def bench_serve(args):
     against FIFO one-at-a-time service (one slot) of the same stream."""
     import numpy as np
     import torch
-    from pamo_amd import synth
+    from pamo_amd import synth, _hip
     from pamo_amd.batch import Server
 
     pool, jobs = synth.serve_stream(args.serve_seconds, args.serve_rate)
@@ -108,19 +123,30 @@ def bench_serve(args):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(dev[k][0], dev[k][1], r, s3))
-        lat, qd = [], []
-        for i in ids:
+        lat, qd, failed = [], [], []
+        for k, i in enumerate(ids):
             info = srv.poll(i, wait=True)
             lat.append(info["latency_s"])
             qd.append(info["queue_s"])
-            srv.result(i)
+            if info["state"] == "failed":
+                # the reference raises on the same input (stage 3's closed-manifold assertion)
+                ta, m, r, s3 = jobs[k]
+                failed.append({"pool_mesh": m, "faces": int(len(pool[m][1])), "ratio": round(r, 5), "stage3": s3})
+                try:
+                    srv.result(i)
+                except _hip.PamoError as err:
+                    failed[-1]["error"] = str(err).split(": ")[-1]
+            else:
+                srv.result(i)
+            if (k + 1) % 25 == 0:
+                print(f"[bench] {n_slots}-slot stream: {k + 1}/{len(ids)} jobs done", file=sys.stderr, flush=True)
         t_end = time.perf_counter() - t0
         srv.close()
-        return np.array(lat), np.array(qd), t_end
+        return np.array(lat), np.array(qd), t_end, failed
 
-    lat, qd, t_all = run_stream(args.slots)
+    lat, qd, t_all, failed = run_stream(args.slots)
     print(f"[bench] served in {t_all:.1f}s", file=sys.stderr, flush=True)
-    lat1, qd1, t1 = run_stream(1)
+    lat1, qd1, t1, failed1 = run_stream(1)
     print(f"[bench] fifo in {t1:.1f}s", file=sys.stderr, flush=True)
     n = len(jobs)
     res = {"metric": "served meshes/s and p50/p99 latency (stages 2+3 on half the jobs)",
@@ -134,7 +160,8 @@ def bench_serve(args):
            "fifo_meshes_per_s": round(n / t1, 3),
            "fifo_p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
            "fifo_p99_latency_s": round(float(np.percentile(lat1, 99)), 3),
-           "speedup_vs_fifo": round(t1 / t_all, 3)}
+           "speedup_vs_fifo": round(t1 / t_all, 3),
+           "failed_jobs": failed, "fifo_failed_jobs": len(failed1)}
     print(json.dumps(res))
 
 

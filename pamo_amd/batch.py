@@ -38,12 +38,15 @@ def pack_csr(meshes):
 
 
 def simplify_batch(meshes, ratio, use_stage3=False, n_slots: int = 4, tolerance: int = 4, threshold: float = 1e-3,
-                   iters: int = 100000, min_verts: int = 0, config: Stage3Config | None = None, packed=None):
+                   iters: int = 100000, min_verts: int = 0, config: Stage3Config | None = None, packed=None,
+                   on_done=None):
     """Runs every mesh through the PaMO pipeline; returns ([(V, F)], seconds).
 
     ``ratio`` and ``use_stage3`` are scalars or per-mesh sequences.  Outputs
     are numpy float32 vertices and int32 faces.  ``packed`` may hold the
-    result of ``pack_csr(meshes)`` (device inputs already resident)."""
+    result of ``pack_csr(meshes)`` (device inputs already resident).
+    ``on_done(mesh, status)``, if given, is called from a worker thread as each
+    mesh finishes."""
     n = len(meshes) if meshes is not None else len(packed[2]) - 1
     ratio = np.broadcast_to(np.asarray(ratio, np.float64), (n,)).copy()
     s3 = np.broadcast_to(np.asarray(use_stage3, np.int32), (n,)).copy()
@@ -54,12 +57,14 @@ def simplify_batch(meshes, ratio, use_stage3=False, n_slots: int = 4, tolerance:
     onf = np.zeros(n, np.int32)
     secs = ctypes.c_double(0)
     params = _s3_params(s3, config)
+    cb = _hip.BATCH_DONE_FN(lambda mesh, status, user: on_done(mesh, status)) if on_done else None
     torch.cuda.current_stream().synchronize()
     _hip.check(_hip.lib().pamo_batch_run(
         torch.cuda.current_device(), int(n_slots), ctypes.byref(params) if params is not None else None, n,
         Vd.data_ptr(), Fd.data_ptr(), vo.ctypes.data, fo.ctypes.data, ratio.ctypes.data, s3.ctypes.data,
         int(tolerance), float(threshold), int(iters), int(min_verts), ov.data_ptr(), of.data_ptr(),
-        onv.ctypes.data, onf.ctypes.data, ctypes.byref(secs)), "pamo_batch_run")
+        onv.ctypes.data, onf.ctypes.data, ctypes.byref(secs), ctypes.cast(cb, ctypes.c_void_p) if cb else None, None),
+        "pamo_batch_run")
     ovh, ofh = ov.cpu().numpy(), of.cpu().numpy()
     out = [(ovh[vo[i]:vo[i] + onv[i]].copy(), ofh[fo[i]:fo[i] + onf[i]].copy()) for i in range(n)]
     return out, secs.value

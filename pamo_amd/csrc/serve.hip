@@ -44,6 +44,7 @@ __global__ void shift_k(const float* __restrict__ v, long long n3, float mx, flo
 
 struct Job {
     long long id = 0;
+    int index = -1;  // position in a batch (-1: served job)
     const float* dv = nullptr;
     const int* df = nullptr;
     int nv = 0, nf = 0;
@@ -92,6 +93,9 @@ struct Engine {
     bool has_s3 = false;
     pamo_s3_params s3p{};
     std::vector<Slot> slots;
+    hipStream_t io = nullptr;  // result copies of pamo_serve_fetch
+    pamo_batch_done_fn on_done = nullptr;
+    void* on_done_user = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job*> queue;
@@ -211,6 +215,7 @@ static void worker(Engine* e, int si) {
             --e->running;
             j->done = true;
         }
+        if (e->on_done && j->index >= 0) e->on_done(j->index, j->status, e->on_done_user);
         e->cv_done.notify_all();
         e->cv_work.notify_all();
     }
@@ -225,6 +230,7 @@ static void destroy(Engine* e) {
     for (Slot& s : e->slots)
         if (s.th.joinable()) s.th.join();
     (void)hipSetDevice(e->device);
+    if (e->io) (void)hipStreamDestroy(e->io);
     for (Slot& s : e->slots) {
         if (s.c2) pamo_ctx_destroy(s.c2);
         if (s.c3) pamo_s3_destroy(s.c3);
@@ -253,6 +259,10 @@ static int create(Engine** out, int device, int n_slots, long long budget, const
     if (s3) e->s3p = *s3;
     e->slots.resize(n_slots);
     PAMO_HIP(hipSetDevice(device));
+    if (hipStreamCreateWithFlags(&e->io, hipStreamNonBlocking) != hipSuccess) {
+        destroy(e);
+        PAMO_CHECK(false, "pamo serve/batch: stream creation failed");
+    }
     for (Slot& s : e->slots) {
         int rc = pamo_ctx_create(&s.c2, device);
         if (rc == 0 && e->has_s3) rc = pamo_s3_create(&s.c3, device, &e->s3p);
@@ -290,7 +300,8 @@ extern "C" {
 int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_meshes, const float* d_verts,
                    const int* d_faces, const long long* vert_off, const long long* face_off, const double* ratio,
                    const int* use_stage3, int tolerance, float threshold, long long iters, long long min_verts,
-                   float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf, double* seconds) {
+                   float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf, double* seconds,
+                   pamo_batch_done_fn on_done, void* user) {
     pamo::clear_error();
     PAMO_CHECK(n_meshes >= 0 && d_verts && d_faces && vert_off && face_off && ratio && use_stage3 && d_out_verts &&
                    d_out_faces && out_nv && out_nf,
@@ -301,6 +312,8 @@ int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_mesh
     }
     Engine* e = nullptr;
     PAMO_TRY(pamo::serve::create(&e, device, n_slots, 0, s3));
+    e->on_done = on_done;
+    e->on_done_user = user;
     const double t0 = pamo::serve::now_s();
     // longest first (LPT on face count)
     std::vector<int> order(n_meshes);
@@ -312,6 +325,7 @@ int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_mesh
     for (int k = 0; k < n_meshes; ++k) {
         const int i = order[k];
         Job* j = new Job();
+        j->index = i;
         j->dv = d_verts + 3 * vert_off[i];
         j->df = d_faces + 3 * face_off[i];
         j->nv = (int)(vert_off[i + 1] - vert_off[i]);
@@ -428,8 +442,10 @@ int pamo_serve_fetch(pamo_server* srv, long long job_id, float* d_out_verts, int
         e->jobs.erase(it);
     }
     PAMO_HIP(hipSetDevice(e->device));
-    hipError_t er = hipMemcpy(d_out_verts, j->out_v, 12 * (size_t)j->out_nv, hipMemcpyDeviceToDevice);
-    if (er == hipSuccess) er = hipMemcpy(d_out_faces, j->out_f, 12 * (size_t)j->out_nf, hipMemcpyDeviceToDevice);
+    hipError_t er = hipMemcpyAsync(d_out_verts, j->out_v, 12 * (size_t)j->out_nv, hipMemcpyDeviceToDevice, e->io);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(d_out_faces, j->out_f, 12 * (size_t)j->out_nf, hipMemcpyDeviceToDevice, e->io);
+    if (er == hipSuccess) er = hipStreamSynchronize(e->io);
     const int status = j->status;
     const std::string err = j->err;
     (void)hipFree(j->out_v);

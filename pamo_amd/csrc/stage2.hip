@@ -134,6 +134,8 @@ struct Ctx {
     int n_pts = 0, n_tris = 0, n_edges = 0, n_rec = 0, n_regular = 0, n_vertices_undo = 0;
     int undo_iters = 0;
     int guard_small_stack = 0;  // test option: 32-entry guard frontier
+    double* cent_part = nullptr;  // per-block partial sums of pamo_centroid
+    size_t alloc_cent = 0;
     Profiler prof;
 };
 
@@ -145,11 +147,14 @@ struct ProfScope {
 };
 
 template <typename T>
-static int dalloc(T** q, size_t n) {
+static int dalloc(T** q, size_t n, hipStream_t st) {
     if (*q) PAMO_HIP(hipFree(*q));
     *q = nullptr;
     PAMO_HIP(hipMalloc((void**)q, sizeof(T) * std::max<size_t>(n, 1)));
-    PAMO_HIP(hipMemset(*q, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    // zeroed in order on the owning stream: a null-stream hipMemset is not
+    // ordered with the context's non-blocking stream and may complete after
+    // the kernels that fill the buffer
+    PAMO_HIP(hipMemsetAsync(*q, 0, sizeof(T) * std::max<size_t>(n, 1), st));
     return 0;
 }
 
@@ -157,7 +162,7 @@ static int ensure_parts(Ctx& c, size_t n_items) {
     size_t need = 6 * (n_items / 256 + 4);
     if (need > c.alloc_parts) {
         c.alloc_parts = need;
-        PAMO_TRY(dalloc(&c.box_part, need));
+        PAMO_TRY(dalloc(&c.box_part, need, c.stream));
     }
     return 0;
 }
@@ -166,7 +171,7 @@ static int ensure_ws(Ctx& c, long long n_items) {
     long long need = std::max(radix_workspace(n_items), scan_workspace(n_items + 1)) + 256;
     if ((size_t)need > c.alloc_ws) {
         c.alloc_ws = (size_t)need + need / 4;
-        PAMO_TRY(dalloc(&c.ws, c.alloc_ws));
+        PAMO_TRY(dalloc(&c.ws, c.alloc_ws, c.stream));
     }
     return 0;
 }
@@ -202,33 +207,33 @@ static int grow_pts(Ctx& c, size_t n) {
         return 0;
     }
     c.alloc_pts = a;
-    PAMO_TRY(dalloc(&c.p, a));
-    PAMO_TRY(dalloc(&c.op, a));
-    PAMO_TRY(dalloc(&c.np, a));
-    PAMO_TRY(dalloc(&c.Q, 3 * a));
-    PAMO_TRY(dalloc(&c.occ, a + 1));
-    PAMO_TRY(dalloc(&c.pts_map, a + 1));
-    PAMO_TRY(dalloc(&c.ring_off, a + 2));
-    PAMO_TRY(dalloc(&c.ring_fill, a + 1));
-    PAMO_TRY(dalloc(&c.undo_list, 4 * a));
-    PAMO_TRY(dalloc(&c.tmp_undo, 4 * a));
-    PAMO_TRY(dalloc(&c.undo_prev, 4 * a));
-    PAMO_TRY(dalloc(&c.invalid, a));
-    PAMO_TRY(dalloc(&c.vmin, a));
+    PAMO_TRY(dalloc(&c.p, a, c.stream));
+    PAMO_TRY(dalloc(&c.op, a, c.stream));
+    PAMO_TRY(dalloc(&c.np, a, c.stream));
+    PAMO_TRY(dalloc(&c.Q, 3 * a, c.stream));
+    PAMO_TRY(dalloc(&c.occ, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.pts_map, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.ring_off, a + 2, c.stream));
+    PAMO_TRY(dalloc(&c.ring_fill, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.undo_list, 4 * a, c.stream));
+    PAMO_TRY(dalloc(&c.tmp_undo, 4 * a, c.stream));
+    PAMO_TRY(dalloc(&c.undo_prev, 4 * a, c.stream));
+    PAMO_TRY(dalloc(&c.invalid, a, c.stream));
+    PAMO_TRY(dalloc(&c.vmin, a, c.stream));
     PAMO_TRY(ensure_parts(c, a));
     return ensure_ws(c, (long long)a + 1);
 }
 
 static int alloc_edges(Ctx& c, size_t ae) {
     c.alloc_edges = ae;
-    PAMO_TRY(dalloc(&c.eu, ae));
-    PAMO_TRY(dalloc(&c.ev, ae));
-    PAMO_TRY(dalloc(&c.esel, ae + 1));
-    PAMO_TRY(dalloc(&c.recs, ae + 1));
-    PAMO_TRY(dalloc(&c.pflag, ae + 1));
-    PAMO_TRY(dalloc(&c.pushed, ae + 1));
-    PAMO_TRY(dalloc(&c.edge_cost, ae + 1));
-    PAMO_TRY(dalloc(&c.eord, ae + 1));
+    PAMO_TRY(dalloc(&c.eu, ae, c.stream));
+    PAMO_TRY(dalloc(&c.ev, ae, c.stream));
+    PAMO_TRY(dalloc(&c.esel, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.recs, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.pflag, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.pushed, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.edge_cost, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.eord, ae + 1, c.stream));
     return 0;
 }
 
@@ -253,34 +258,34 @@ static int grow_tris(Ctx& c, size_t n) {
         return 0;
     }
     c.alloc_tris = a;
-    PAMO_TRY(dalloc(&c.ti, a)); PAMO_TRY(dalloc(&c.tj, a)); PAMO_TRY(dalloc(&c.tk, a));
-    PAMO_TRY(dalloc(&c.oi, a)); PAMO_TRY(dalloc(&c.oj, a)); PAMO_TRY(dalloc(&c.ok, a));
-    PAMO_TRY(dalloc(&c.ni, a)); PAMO_TRY(dalloc(&c.nj, a)); PAMO_TRY(dalloc(&c.nk, a));
-    PAMO_TRY(dalloc(&c.ring, 3 * a));
-    PAMO_TRY(dalloc(&c.ring_raw, 3 * a));
-    PAMO_TRY(dalloc(&c.ring_own, 3 * a));
-    PAMO_TRY(dalloc(&c.rv, 3 * a));
-    PAMO_TRY(dalloc(&c.edge_off, a + 1));
-    PAMO_TRY(dalloc(&c.changed, a));
-    PAMO_TRY(dalloc(&c.flagged, a));
-    PAMO_TRY(dalloc(&c.face_scan, a + 1));
-    PAMO_TRY(dalloc(&c.qlist, a + 1));
-    PAMO_TRY(dalloc(&c.bkey, a)); PAMO_TRY(dalloc(&c.bval, a));
-    PAMO_TRY(dalloc(&c.bkey2, a)); PAMO_TRY(dalloc(&c.bval2, a));
-    PAMO_TRY(dalloc(&c.bleft, a)); PAMO_TRY(dalloc(&c.bright, a));
-    PAMO_TRY(dalloc(&c.bparent, 2 * a));
-    PAMO_TRY(dalloc(&c.brange, a));
-    PAMO_TRY(dalloc(&c.bbox, 4 * a));  // 2n-1 nodes, 2 float4 each
-    PAMO_TRY(dalloc(&c.fbox, 2 * a));
-    PAMO_TRY(dalloc(&c.bpre, 2 * a));
-    PAMO_TRY(dalloc(&c.bsuf, 2 * a));
+    PAMO_TRY(dalloc(&c.ti, a, c.stream)); PAMO_TRY(dalloc(&c.tj, a, c.stream)); PAMO_TRY(dalloc(&c.tk, a, c.stream));
+    PAMO_TRY(dalloc(&c.oi, a, c.stream)); PAMO_TRY(dalloc(&c.oj, a, c.stream)); PAMO_TRY(dalloc(&c.ok, a, c.stream));
+    PAMO_TRY(dalloc(&c.ni, a, c.stream)); PAMO_TRY(dalloc(&c.nj, a, c.stream)); PAMO_TRY(dalloc(&c.nk, a, c.stream));
+    PAMO_TRY(dalloc(&c.ring, 3 * a, c.stream));
+    PAMO_TRY(dalloc(&c.ring_raw, 3 * a, c.stream));
+    PAMO_TRY(dalloc(&c.ring_own, 3 * a, c.stream));
+    PAMO_TRY(dalloc(&c.rv, 3 * a, c.stream));
+    PAMO_TRY(dalloc(&c.edge_off, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.changed, a, c.stream));
+    PAMO_TRY(dalloc(&c.flagged, a, c.stream));
+    PAMO_TRY(dalloc(&c.face_scan, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.qlist, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.bkey, a, c.stream)); PAMO_TRY(dalloc(&c.bval, a, c.stream));
+    PAMO_TRY(dalloc(&c.bkey2, a, c.stream)); PAMO_TRY(dalloc(&c.bval2, a, c.stream));
+    PAMO_TRY(dalloc(&c.bleft, a, c.stream)); PAMO_TRY(dalloc(&c.bright, a, c.stream));
+    PAMO_TRY(dalloc(&c.bparent, 2 * a, c.stream));
+    PAMO_TRY(dalloc(&c.brange, a, c.stream));
+    PAMO_TRY(dalloc(&c.bbox, 4 * a, c.stream));  // 2n-1 nodes, 2 float4 each
+    PAMO_TRY(dalloc(&c.fbox, 2 * a, c.stream));
+    PAMO_TRY(dalloc(&c.bpre, 2 * a, c.stream));
+    PAMO_TRY(dalloc(&c.bsuf, 2 * a, c.stream));
     const size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
-    PAMO_TRY(dalloc(&c.blk, 2 * nb));
-    PAMO_TRY(dalloc(&c.blkpre, 2 * nb));
-    PAMO_TRY(dalloc(&c.blksuf, 2 * nb));
-    PAMO_TRY(dalloc(&c.sblk, 2 * ns));
+    PAMO_TRY(dalloc(&c.blk, 2 * nb, c.stream));
+    PAMO_TRY(dalloc(&c.blkpre, 2 * nb, c.stream));
+    PAMO_TRY(dalloc(&c.blksuf, 2 * nb, c.stream));
+    PAMO_TRY(dalloc(&c.sblk, 2 * ns, c.stream));
     PAMO_TRY(ensure_parts(c, a));
-    PAMO_TRY(dalloc(&c.shuf, 4 * a + 64));
+    PAMO_TRY(dalloc(&c.shuf, 4 * a + 64, c.stream));
     if (2 * a + 1 > c.alloc_edges) PAMO_TRY(alloc_edges(c, 2 * a + 1));  // at most 2 edges per face
     return ensure_ws(c, (long long)(4 * a + 64));
 }
@@ -304,7 +309,7 @@ static void free_all(Ctx& c) {
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
                     c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
-                    c.shuf, c.ws, c.dcnt};
+                    c.shuf, c.ws, c.dcnt, c.cent_part};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
@@ -588,7 +593,7 @@ int pamo_ctx_create(pamo_ctx** out, int device) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->dcnt, sizeof(int) * pamo::C_COUNT);
-    if (e == hipSuccess) e = hipMemset(c->dcnt, 0, sizeof(int) * pamo::C_COUNT);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dcnt, 0, sizeof(int) * pamo::C_COUNT, c->stream);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcnt, sizeof(int) * pamo::C_COUNT, 0);
     if (e != hipSuccess) {
         pamo::set_error(std::string("pamo_ctx_create: ") + hipGetErrorString(e));
@@ -690,13 +695,15 @@ int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tri
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     PAMO_HIP(hipSetDevice(c.device));
     const int nb = (int)pamo::grid_for(n_tris, pamo::kBlock);
-    double* part = nullptr;
-    PAMO_HIP(hipMallocAsync((void**)&part, sizeof(double) * 3 * nb, c.stream));
+    if ((size_t)(3 * nb) > c.alloc_cent) {
+        PAMO_TRY(pamo::dalloc(&c.cent_part, 3 * (size_t)nb, c.stream));
+        c.alloc_cent = 3 * (size_t)nb;
+    }
+    double* part = c.cent_part;
     hipLaunchKernelGGL(pamo::centroid_part_k, dim3(nb), dim3(pamo::kBlock), 0, c.stream, d_pts, d_tris, n_tris, part);
     PAMO_HIP(hipGetLastError());
     std::vector<double> h(3 * (size_t)nb);
     PAMO_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
-    PAMO_HIP(hipFreeAsync(part, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
     for (int d = 0; d < 3; ++d) {
         double acc = 0.0;

@@ -48,30 +48,33 @@ struct Tree {
 };
 
 template <typename T>
-static int dalloc(T** q, size_t n) {
+static int dalloc(T** q, size_t n, hipStream_t st) {
     if (*q) PAMO_HIP(hipFree(*q));
     *q = nullptr;
     PAMO_HIP(hipMalloc((void**)q, sizeof(T) * std::max<size_t>(n, 1)));
-    PAMO_HIP(hipMemset(*q, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    // zeroed in order on the owning stream: a null-stream hipMemset is not
+    // ordered with the context's non-blocking stream and may complete after
+    // the kernels that fill the buffer
+    PAMO_HIP(hipMemsetAsync(*q, 0, sizeof(T) * std::max<size_t>(n, 1), st));
     return 0;
 }
 
-static int tree_alloc(Tree& t, int n) {
+static int tree_alloc(Tree& t, int n, hipStream_t st) {
     if (n <= t.cap) return 0;
     size_t a = (size_t)n + n / 4 + 16;
     t.cap = (int)a;
-    PAMO_TRY(dalloc(&t.item, 2 * a));
-    PAMO_TRY(dalloc(&t.key, a)); PAMO_TRY(dalloc(&t.val, a));
-    PAMO_TRY(dalloc(&t.key2, a)); PAMO_TRY(dalloc(&t.val2, a));
-    PAMO_TRY(dalloc(&t.left, a)); PAMO_TRY(dalloc(&t.right, a)); PAMO_TRY(dalloc(&t.parent, 2 * a));
-    PAMO_TRY(dalloc(&t.range, a));
-    PAMO_TRY(dalloc(&t.box, 4 * a));
-    PAMO_TRY(dalloc(&t.pre, 2 * a)); PAMO_TRY(dalloc(&t.suf, 2 * a));
+    PAMO_TRY(dalloc(&t.item, 2 * a, st));
+    PAMO_TRY(dalloc(&t.key, a, st)); PAMO_TRY(dalloc(&t.val, a, st));
+    PAMO_TRY(dalloc(&t.key2, a, st)); PAMO_TRY(dalloc(&t.val2, a, st));
+    PAMO_TRY(dalloc(&t.left, a, st)); PAMO_TRY(dalloc(&t.right, a, st)); PAMO_TRY(dalloc(&t.parent, 2 * a, st));
+    PAMO_TRY(dalloc(&t.range, a, st));
+    PAMO_TRY(dalloc(&t.box, 4 * a, st));
+    PAMO_TRY(dalloc(&t.pre, 2 * a, st)); PAMO_TRY(dalloc(&t.suf, 2 * a, st));
     size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
-    PAMO_TRY(dalloc(&t.blk, 2 * nb)); PAMO_TRY(dalloc(&t.blkpre, 2 * nb)); PAMO_TRY(dalloc(&t.blksuf, 2 * nb));
-    PAMO_TRY(dalloc(&t.sblk, 2 * ns));
-    PAMO_TRY(dalloc(&t.part, 6 * (a / kBlock + 4)));
-    if (!t.cnt) PAMO_TRY(dalloc(&t.cnt, C_COUNT));
+    PAMO_TRY(dalloc(&t.blk, 2 * nb, st)); PAMO_TRY(dalloc(&t.blkpre, 2 * nb, st)); PAMO_TRY(dalloc(&t.blksuf, 2 * nb, st));
+    PAMO_TRY(dalloc(&t.sblk, 2 * ns, st));
+    PAMO_TRY(dalloc(&t.part, 6 * (a / kBlock + 4), st));
+    if (!t.cnt) PAMO_TRY(dalloc(&t.cnt, C_COUNT, st));
     return 0;
 }
 
@@ -272,7 +275,7 @@ static int ensure_ws(Ctx& c, long long n) {
     long long need = std::max(radix_workspace(n), scan_workspace(n + 1)) + 256;
     if (need > c.cap_ws) {
         c.cap_ws = need + need / 4;
-        PAMO_TRY(dalloc(&c.ws, (size_t)c.cap_ws));
+        PAMO_TRY(dalloc(&c.ws, (size_t)c.cap_ws, c.stream));
     }
     return 0;
 }
@@ -281,7 +284,7 @@ static int ensure_part(Ctx& c, long long items) {
     int need = (int)(items / kB + 8);
     if (need > c.cap_part) {
         c.cap_part = need + need / 4;
-        PAMO_TRY(dalloc(&c.part, (size_t)c.cap_part));
+        PAMO_TRY(dalloc(&c.part, (size_t)c.cap_part, c.stream));
     }
     return 0;
 }
@@ -1272,10 +1275,10 @@ static int compute_energy(Ctx& c, int nC, int slot, int mask) {
 static int ensure_sort(Ctx& c, long long n) {
     if (n > c.cap_sort) {
         c.cap_sort = n + n / 4 + 1024;
-        PAMO_TRY(dalloc(&c.skey, (size_t)c.cap_sort));
-        PAMO_TRY(dalloc(&c.sval, (size_t)c.cap_sort));
-        PAMO_TRY(dalloc(&c.skey2, (size_t)c.cap_sort));
-        PAMO_TRY(dalloc(&c.sval2, (size_t)c.cap_sort));
+        PAMO_TRY(dalloc(&c.skey, (size_t)c.cap_sort, c.stream));
+        PAMO_TRY(dalloc(&c.sval, (size_t)c.cap_sort, c.stream));
+        PAMO_TRY(dalloc(&c.skey2, (size_t)c.cap_sort, c.stream));
+        PAMO_TRY(dalloc(&c.sval2, (size_t)c.cap_sort, c.stream));
     }
     return ensure_ws(c, n + 64);
 }
@@ -1363,16 +1366,16 @@ static int ensure_contacts(Ctx& c, int n) {
     if (n <= c.capC) return 0;
     int a = n + n / 4 + 1024;
     c.capC = a;
-    PAMO_TRY(dalloc(&c.cidx, a));
-    PAMO_TRY(dalloc(&c.ctype1, a));
-    PAMO_TRY(dalloc(&c.ctype2, a));
-    PAMO_TRY(dalloc(&c.cd, a));
-    PAMO_TRY(dalloc(&c.cdd, 4 * (size_t)a));
-    PAMO_TRY(dalloc(&c.act, a));
-    PAMO_TRY(dalloc(&c.act_flag, (size_t)a + 1));
-    PAMO_TRY(dalloc(&c.act_pos, (size_t)a + 1));
-    PAMO_TRY(dalloc(&c.sA, a));
-    PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a));
+    PAMO_TRY(dalloc(&c.cidx, a, c.stream));
+    PAMO_TRY(dalloc(&c.ctype1, a, c.stream));
+    PAMO_TRY(dalloc(&c.ctype2, a, c.stream));
+    PAMO_TRY(dalloc(&c.cd, a, c.stream));
+    PAMO_TRY(dalloc(&c.cdd, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.act, a, c.stream));
+    PAMO_TRY(dalloc(&c.act_flag, (size_t)a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.act_pos, (size_t)a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.sA, a, c.stream));
+    PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a, c.stream));
     return 0;
 }
 
@@ -1382,8 +1385,8 @@ static int ensure_contacts(Ctx& c, int n) {
 static int detect_contacts(Ctx& c, int& nC) {
     nC = 0;
     if (c.P.mask & PAMO_S3_E_COLLISION) {
-        PAMO_TRY(tree_alloc(c.tri_tree, c.nF));
-        PAMO_TRY(tree_alloc(c.edge_tree, c.nE));
+        PAMO_TRY(tree_alloc(c.tri_tree, c.nF, c.stream));
+        PAMO_TRY(tree_alloc(c.edge_tree, c.nE, c.stream));
         L3(tri_items_k, c.nF, (const float4*)c.q, (const int4*)c.tri, c.nF, 0.0f, c.tri_tree.item);
         PAMO_TRY(tree_build(c.tri_tree, c.nF, c.ws, c.stream));
         L3(edge_items_k, c.nE, (const float4*)c.q, (const int2*)c.edge, c.nE, c.edge_tree.item);
@@ -1660,9 +1663,9 @@ int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->ccount, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void**)&c->n_act, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(c->n_act, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->n_act, 0, sizeof(int), c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->sc, sizeof(double) * pamo::s3::SC_N);
-    if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(double) * pamo::s3::SC_N);
+    if (e == hipSuccess) e = hipMemsetAsync(c->sc, 0, sizeof(double) * pamo::s3::SC_N, c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, sizeof(float));
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcount, sizeof(int), 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hsc, sizeof(double) * pamo::s3::SC_N, 0);
@@ -1696,8 +1699,8 @@ int pamo_s3_set_target(pamo_s3* s, const float* d_gt_v, int n_gt_v, const int* d
     PAMO_HIP(hipSetDevice(c.device));
     if (n_gt_v > c.capG || n_gt_f > c.capG) {
         c.capG = std::max(n_gt_v, n_gt_f) + 1024;
-        PAMO_TRY(pamo::s3::dalloc(&c.gt_v, c.capG));
-        PAMO_TRY(pamo::s3::dalloc(&c.gt_f, c.capG));
+        PAMO_TRY(pamo::s3::dalloc(&c.gt_v, c.capG, c.stream));
+        PAMO_TRY(pamo::s3::dalloc(&c.gt_f, c.capG, c.stream));
     }
     c.nGV = n_gt_v;
     c.nGF = n_gt_f;
@@ -1707,7 +1710,7 @@ int pamo_s3_set_target(pamo_s3* s, const float* d_gt_v, int n_gt_v, const int* d
     hipLaunchKernelGGL(pamo::s3::load_faces4_k, dim3(pamo::grid_for(n_gt_f, 256)), dim3(256), 0, c.stream, d_gt_f,
                        n_gt_f, c.gt_f);
     PAMO_HIP(hipGetLastError());
-    PAMO_TRY(pamo::s3::tree_alloc(c.gt_tree, n_gt_f));
+    PAMO_TRY(pamo::s3::tree_alloc(c.gt_tree, n_gt_f, c.stream));
     hipLaunchKernelGGL(pamo::s3::tri_items_k, dim3(pamo::grid_for(n_gt_f, 256)), dim3(256), 0, c.stream,
                        (const float4*)c.gt_v, (const int4*)c.gt_f, n_gt_f, 0.0f, c.gt_tree.item);
     PAMO_HIP(hipGetLastError());
@@ -1730,37 +1733,37 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
     if (n_v > c.capV) {
         c.capV = n_v + n_v / 4 + 64;
         float4** vv[] = {&c.q, &c.qn, &c.qd, &c.qrest, &c.p, &c.target, &c.grad, &c.diag, &c.r, &c.v, &c.Av, &c.z};
-        for (float4** q : vv) PAMO_TRY(dalloc(q, c.capV));
-        PAMO_TRY(dalloc(&c.vor, c.capV));
-        PAMO_TRY(dalloc(&c.vf_off, (size_t)c.capV + 1));
-        PAMO_TRY(dalloc(&c.vh_off, (size_t)c.capV + 1));
-        PAMO_TRY(dalloc(&c.vc_off, (size_t)c.capV + 1));
-        PAMO_TRY(dalloc(&c.dn, 3 * (size_t)c.capV));
+        for (float4** q : vv) PAMO_TRY(dalloc(q, c.capV, c.stream));
+        PAMO_TRY(dalloc(&c.vor, c.capV, c.stream));
+        PAMO_TRY(dalloc(&c.vf_off, (size_t)c.capV + 1, c.stream));
+        PAMO_TRY(dalloc(&c.vh_off, (size_t)c.capV + 1, c.stream));
+        PAMO_TRY(dalloc(&c.vc_off, (size_t)c.capV + 1, c.stream));
+        PAMO_TRY(dalloc(&c.dn, 3 * (size_t)c.capV, c.stream));
         const size_t np = pamo::grid_for(c.capV, pamo::s3::kB) + 2;
-        PAMO_TRY(dalloc(&c.zr_part, 2 * np));
-        PAMO_TRY(dalloc(&c.vav_part, pamo::grid_for((long long)c.capV * pamo::s3::kGather, pamo::s3::kB) + 2));
+        PAMO_TRY(dalloc(&c.zr_part, 2 * np, c.stream));
+        PAMO_TRY(dalloc(&c.vav_part, pamo::grid_for((long long)c.capV * pamo::s3::kGather, pamo::s3::kB) + 2, c.stream));
     }
     if (n_f > c.capF) {
         c.capF = n_f + n_f / 4 + 64;
-        PAMO_TRY(dalloc(&c.tri, c.capF));
-        PAMO_TRY(dalloc(&c.area, c.capF));
-        PAMO_TRY(dalloc(&c.inv_dm, c.capF));
-        PAMO_TRY(dalloc(&c.vf_ent, 3 * (size_t)c.capF));
-        PAMO_TRY(dalloc(&c.fc, 3 * (size_t)c.capF));
+        PAMO_TRY(dalloc(&c.tri, c.capF, c.stream));
+        PAMO_TRY(dalloc(&c.area, c.capF, c.stream));
+        PAMO_TRY(dalloc(&c.inv_dm, c.capF, c.stream));
+        PAMO_TRY(dalloc(&c.vf_ent, 3 * (size_t)c.capF, c.stream));
+        PAMO_TRY(dalloc(&c.fc, 3 * (size_t)c.capF, c.stream));
     }
     if (n_e > c.capE) {
         c.capE = n_e + n_e / 4 + 64;
-        PAMO_TRY(dalloc(&c.edge, c.capE));
+        PAMO_TRY(dalloc(&c.edge, c.capE, c.stream));
     }
     if (n_h > c.capH) {
         c.capH = n_h + n_h / 4 + 64;
-        PAMO_TRY(dalloc(&c.hinge, c.capH));
-        PAMO_TRY(dalloc(&c.rest_angle, c.capH));
-        PAMO_TRY(dalloc(&c.rest_elen, c.capH));
-        PAMO_TRY(dalloc(&c.hblock, 144 * (size_t)c.capH));
-        PAMO_TRY(dalloc(&c.hsym, 45 * (size_t)c.capH));
-        PAMO_TRY(dalloc(&c.vh_ent, 4 * (size_t)c.capH));
-        PAMO_TRY(dalloc(&c.hc, 4 * (size_t)c.capH));
+        PAMO_TRY(dalloc(&c.hinge, c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.rest_angle, c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.rest_elen, c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.hblock, 144 * (size_t)c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.hsym, 45 * (size_t)c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.vh_ent, 4 * (size_t)c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.hc, 4 * (size_t)c.capH, c.stream));
     }
     c.nV = n_v;
     c.nF = n_f;
@@ -1820,7 +1823,8 @@ int pamo_s3_get_vertices(pamo_s3* s, float* d_out, double scale, const double* t
         o[3 * (size_t)i + 1] = (float)(((double)h[i].y - t3[1]) / scale);
         o[3 * (size_t)i + 2] = (float)(((double)h[i].z - t3[2]) / scale);
     }
-    PAMO_HIP(hipMemcpy(d_out, o.data(), sizeof(float) * o.size(), hipMemcpyHostToDevice));
+    PAMO_HIP(hipMemcpyAsync(d_out, o.data(), sizeof(float) * o.size(), hipMemcpyHostToDevice, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
     return 0;
 }
 
@@ -1847,10 +1851,11 @@ int pamo_s3_set_dense(pamo_s3* s, const double* h_A, const double* h_b, double c
     PAMO_CHECK(n_v == c.nV && n_v > 0 && n_v <= 1024, "pamo_s3_set_dense: n_v must equal the mesh's (<= 1024)");
     PAMO_HIP(hipSetDevice(c.device));
     const size_t n3 = 3 * (size_t)n_v;
-    PAMO_TRY(pamo::s3::dalloc(&c.dense_A, n3 * n3));
-    PAMO_TRY(pamo::s3::dalloc(&c.dense_b, n3));
-    PAMO_HIP(hipMemcpy(c.dense_A, h_A, sizeof(double) * n3 * n3, hipMemcpyHostToDevice));
-    PAMO_HIP(hipMemcpy(c.dense_b, h_b, sizeof(double) * n3, hipMemcpyHostToDevice));
+    PAMO_TRY(pamo::s3::dalloc(&c.dense_A, n3 * n3, c.stream));
+    PAMO_TRY(pamo::s3::dalloc(&c.dense_b, n3, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.dense_A, h_A, sizeof(double) * n3 * n3, hipMemcpyHostToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.dense_b, h_b, sizeof(double) * n3, hipMemcpyHostToDevice, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
     c.dense_c = c0;
     return 0;
 }

@@ -260,9 +260,17 @@ _FACES_PER_RES2 = 2006528 / 296 ** 2
 
 def sized_union(seed: int, target_faces: float):
     """The headline generator at the grid resolution whose face count is
-    closest to target_faces (for this seed's shapes)."""
+    closest to target_faces (for this seed's shapes).  A contour whose sheets
+    meet at a single grid point has a pinch vertex (not manifold); the
+    resolution then steps to res+1, res-1, res+2, ... until the mesh is
+    watertight and manifold."""
     res = max(8, int(round((target_faces / _FACES_PER_RES2) ** 0.5)))
-    return ellipsoid_box_union(seed, res)
+    for step in range(16):
+        r = res + ((step + 1) // 2 if step % 2 else -(step // 2))
+        V, F = ellipsoid_box_union(seed, max(8, r))
+        if is_watertight_manifold(F, len(V)):
+            return V, F
+    raise RuntimeError(f"no watertight manifold contour near grid {res} for seed {seed}")
 
 
 def batch_workload(n: int = 96, seed0: int = 100, f_lo: float = 20e3, f_hi: float = 800e3, rng_seed: int = 3):
