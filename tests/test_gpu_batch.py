@@ -91,3 +91,21 @@ def test_served_results_same_alone_and_in_the_mix():
     for (V, F), s3, m, a in zip(meshes, stage3, mixed, alone):
         assert m[2]["state"] == "done" and m[2]["latency_s"] > 0
         _same(m[:2], a[:2], V, s3)
+
+
+def test_many_slots_first_allocations_concurrent():
+    """Twelve meshes of mixed size on six slots: every slot allocates its
+    buffers while the others run (the allocation fills must be ordered on the
+    slot's own streams), then reuses them for smaller meshes."""
+    specs = [(5, 10), (3, 11), (4, 12), (5, 13), (3, 14), (4, 15), (5, 16), (3, 17), (4, 18), (3, 19), (4, 20),
+             (5, 21)]
+    meshes = []
+    for level, seed in specs:
+        V, F = synth.icosphere_sh(level, seed)
+        meshes.append((V.astype(np.float32), F.astype(np.int32)))
+    ratios = [0.05, 0.1, 0.02] * 4
+    stage3 = [True, False] * 6
+    print("case: batch of 12 meshes (icosphere L3-L5), 6 slots", flush=True)
+    out, secs = simplify_batch(meshes, ratios, stage3, n_slots=6)
+    for (V, F), r, s3, got in zip(meshes, ratios, stage3, out):
+        _same(got, _alone(V, F, r, s3), V, s3)
