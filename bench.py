@@ -22,6 +22,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HBM bytes per launch over the whole headline run (FETCH_SIZE + WRITE_SIZE,
+# each scaled by what it reads for a 512 MiB copy), from rocprofv3 --pmc runs
+# of this workload; re-measure after changing these kernels.
+TRAFFIC_SOURCE = "profiles/counters_traffic_occupancy_lds.md"
+HBM_TRAFFIC_PER_LAUNCH = {"guard_query": 55.6e6, "edge_cost": 118.0e6}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -232,8 +238,10 @@ def main():
     # roofline of the dominant kernel (live HIP-event timing inside the library)
     dom = [s for s in prof.stats() if s["name"] == dom_name][0]
     achieved = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9 if dom["total_ms"] > 0 else 0.0
+    traffic = HBM_TRAFFIC_PER_LAUNCH.get(dom["name"])
     roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": 8000.0,
-                "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+                "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
                 "avg_launch_us": round(1e3 * dom["total_ms"] / max(dom["launches"], 1), 2),
                 "launches": dom["launches"]}
 

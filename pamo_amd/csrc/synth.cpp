@@ -7,7 +7,9 @@
 // zero set of the piecewise-linear interpolant is a closed 2-manifold, and
 // sharing one output vertex per crossed grid edge makes the mesh watertight.
 // Triangles are oriented so their normal points toward increasing f
-// (outward, since f < 0 inside).
+// (outward, since f < 0 inside), decided combinatorially from grid
+// coordinates, so the orientation is consistent whatever the rounding of the
+// float32 positions.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -55,10 +57,14 @@ inline int edge_type(int dx, int dy, int dz) {
 extern "C" {
 
 // shapes: n_shapes records of 16 doubles: kind, c[3], a[3], R[9].
-// Grid: (n+1)^3 vertices on [lo, hi]^3.  Outputs are allocated by the
-// library; free with synth_free.  Returns 0 on success.
-int synth_mt(const double* shapes, int n_shapes, int n, double lo, double hi, float** out_v, long long* out_nv,
-             int** out_f, long long* out_nf) {
+// Grid: (n+1)^3 vertices on [lo, hi]^3; the surface is the level set
+// {min over shapes = level}.  Grid values with |f| < min_abs are moved to
+// +-min_abs (0 goes to +), which keeps every crossing point at least about
+// min_abs / |grad f| away from the grid nodes, so that no two crossing points
+// round to the same float32 position.  Outputs are allocated by the library;
+// free with synth_free.  Returns 0 on success.
+int synth_mt(const double* shapes, int n_shapes, int n, double lo, double hi, double level, double min_abs,
+             float** out_v, long long* out_nv, int** out_f, long long* out_nf) {
     std::vector<Shape> S(n_shapes);
     for (int s = 0; s < n_shapes; ++s) {
         const double* r = shapes + 16 * s;
@@ -78,7 +84,9 @@ int synth_mt(const double* shapes, int n_shapes, int n, double lo, double hi, fl
                 double x = lo + h * i;
                 double v = 1e30;
                 for (const Shape& s : S) v = std::min(v, eval_shape(s, x, y, z));
-                float fv = (float)v;
+                double w = v - level;
+                if (std::fabs(w) < min_abs) w = w < 0.0 ? -min_abs : min_abs;
+                float fv = (float)w;
                 if (fv == 0.0f) fv = 1e-12f;  // keep every vertex off the zero set
                 f[(size_t)((k * N1 + j) * N1 + i)] = fv;
             }
@@ -184,34 +192,51 @@ int synth_mt(const double* shapes, int n_shapes, int n, double lo, double hi, fl
                             if (fv[s] < 0) in[ni++] = s;
                             else ou[no++] = s;
                         }
-                        int poly[4], np = 0;
+                        // surface vertices with the tet edge each lies on
+                        int poly[4], pe[4][2], np = 0;
                         if (ni == 1 || no == 1) {
                             int lone = ni == 1 ? in[0] : ou[0];
                             for (int s = 0; s < 4; ++s)
-                                if (s != lone) poly[np++] = eid(q[lone], q[s]);
+                                if (s != lone) {
+                                    pe[np][0] = lone;
+                                    pe[np][1] = s;
+                                    poly[np++] = eid(q[lone], q[s]);
+                                }
                         } else {
                             // quad: a0-b0, a0-b1, a1-b1, a1-b0 (cyclic)
-                            poly[0] = eid(q[in[0]], q[ou[0]]);
-                            poly[1] = eid(q[in[0]], q[ou[1]]);
-                            poly[2] = eid(q[in[1]], q[ou[1]]);
-                            poly[3] = eid(q[in[1]], q[ou[0]]);
+                            const int ends[4][2] = {{in[0], ou[0]}, {in[0], ou[1]}, {in[1], ou[1]}, {in[1], ou[0]}};
+                            for (int e = 0; e < 4; ++e) {
+                                pe[e][0] = ends[e][0];
+                                pe[e][1] = ends[e][1];
+                                poly[e] = eid(q[ends[e][0]], q[ends[e][1]]);
+                            }
                             np = 4;
                         }
-                        // orientation from one positive vertex
-                        int pos = ou[0];
-                        double pp[3] = {lo + h * q[pos][0], lo + h * q[pos][1], lo + h * q[pos][2]};
+                        // Orientation toward a positive tet corner, decided exactly.  A
+                        // surface triangle's corners sit on three tet edges; for any
+                        // interpolation parameters in (0,1) the triangle neither
+                        // degenerates nor passes through a tet corner, so the sign
+                        // taken at the edge midpoints (integer grid coordinates,
+                        // doubled) is the sign at the actual crossing points.
+                        const int pos = ou[0];
                         auto emit = [&](int a, int b, int c) {
-                            double A[3], B[3], C[3];
-                            for (int d = 0; d < 3; ++d) { A[d] = V[3 * a + d]; B[d] = V[3 * b + d]; C[d] = V[3 * c + d]; }
-                            double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-                            double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-                            double nn[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-                            double s = nn[0] * (pp[0] - A[0]) + nn[1] * (pp[1] - A[1]) + nn[2] * (pp[2] - A[2]);
-                            if (s >= 0) { out.push_back(a); out.push_back(b); out.push_back(c); }
-                            else { out.push_back(a); out.push_back(c); out.push_back(b); }
+                            long long m[3][3];
+                            const int sel[3] = {a, b, c};
+                            for (int r = 0; r < 3; ++r)
+                                for (int d = 0; d < 3; ++d) m[r][d] = (long long)q[pe[sel[r]][0]][d] + q[pe[sel[r]][1]][d];
+                            long long u[3], w[3], z[3];
+                            for (int d = 0; d < 3; ++d) {
+                                u[d] = m[1][d] - m[0][d];
+                                w[d] = m[2][d] - m[0][d];
+                                z[d] = 2LL * q[pos][d] - m[0][d];
+                            }
+                            const long long det = z[0] * (u[1] * w[2] - u[2] * w[1]) + z[1] * (u[2] * w[0] - u[0] * w[2]) +
+                                                  z[2] * (u[0] * w[1] - u[1] * w[0]);
+                            if (det >= 0) { out.push_back(poly[a]); out.push_back(poly[b]); out.push_back(poly[c]); }
+                            else { out.push_back(poly[a]); out.push_back(poly[c]); out.push_back(poly[b]); }
                         };
-                        if (np == 3) emit(poly[0], poly[1], poly[2]);
-                        else { emit(poly[0], poly[1], poly[2]); emit(poly[0], poly[2], poly[3]); }
+                        if (np == 3) emit(0, 1, 2);
+                        else { emit(0, 1, 2); emit(0, 2, 3); }
                     }
                 }
             tabA.swap(tabB);

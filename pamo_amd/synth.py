@@ -34,8 +34,8 @@ def _synth_lib():
         lib = ctypes.CDLL(path)
         lib.synth_mt.restype = ctypes.c_int
         lib.synth_mt.argtypes = [
-            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-            ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(ctypes.c_longlong),
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(ctypes.c_longlong),
             ctypes.POINTER(ctypes.POINTER(ctypes.c_int)), ctypes.POINTER(ctypes.c_longlong),
         ]
         lib.synth_free.argtypes = [ctypes.c_void_p]
@@ -130,15 +130,21 @@ def union_shapes(seed: int, n_ell: int = 16, n_box: int = 4):
     return np.ascontiguousarray(np.stack(recs), dtype=np.float64)
 
 
-def contour_shapes(shapes: np.ndarray, res: int, lo: float = -1.0, hi: float = 1.0):
+def contour_shapes(shapes: np.ndarray, res: int, lo: float = -1.0, hi: float = 1.0, level: float = 0.0,
+                   min_abs: float = 0.0):
+    """Marching tetrahedra of {min over shapes = level} on a res^3 grid over
+    [lo, hi]^3.  The surface is exactly the level set of the grid's
+    piecewise-linear interpolant, so contours at different levels are
+    disjoint.  Grid values are kept at least min_abs from the level (so that
+    crossing points stay clear of grid nodes in float32)."""
     lib = _synth_lib()
     pv = ctypes.POINTER(ctypes.c_float)()
     pf = ctypes.POINTER(ctypes.c_int)()
     nv = ctypes.c_longlong()
     nf = ctypes.c_longlong()
     shapes = np.ascontiguousarray(shapes, dtype=np.float64)
-    rc = lib.synth_mt(shapes.ctypes.data, len(shapes), int(res), lo, hi, ctypes.byref(pv), ctypes.byref(nv),
-                      ctypes.byref(pf), ctypes.byref(nf))
+    rc = lib.synth_mt(shapes.ctypes.data, len(shapes), int(res), lo, hi, float(level), float(min_abs), ctypes.byref(pv),
+                      ctypes.byref(nv), ctypes.byref(pf), ctypes.byref(nf))
     if rc != 0:
         raise RuntimeError("synth_mt failed")
     V = np.ctypeslib.as_array(pv, shape=(nv.value, 3)).copy()
@@ -160,6 +166,26 @@ HEADLINE_RES = 296
 def headline_mesh(seed: int = 1):
     """The 2M-face single-mesh workload (stages 2+3, ratio 0.01)."""
     return ellipsoid_box_union(seed, HEADLINE_RES)
+
+
+def stress_sheets(seed: int = 500, target_faces: float = 8e6, level: float = -1e-3, min_abs: float = 1e-5):
+    """Closely separated parallel sheets (intersection stress): the union's
+    contour and, inside it, the contour at `level`, with reversed
+    orientation, together about target_faces faces.  Being level sets of
+    the same piecewise-linear function, the sheets are disjoint.  The grid
+    resolution steps (res, res+1, res-1, ...) until both contours are
+    watertight and manifold.  Returns (V, F, res)."""
+    shapes = union_shapes(seed)
+    res = max(8, int(round((0.5 * target_faces / _FACES_PER_RES2) ** 0.5)))
+    for step in range(16):
+        r = res + ((step + 1) // 2 if step % 2 else -(step // 2))
+        Vo, Fo = contour_shapes(shapes, r, min_abs=min_abs)
+        Vi, Fi = contour_shapes(shapes, r, level=level, min_abs=min_abs)
+        if is_watertight_manifold(Fo, len(Vo)) and is_watertight_manifold(Fi, len(Vi)):
+            V = np.concatenate([Vo, Vi], 0).astype(np.float32)
+            F = np.concatenate([Fo, Fi[:, ::-1] + len(Vo)], 0).astype(np.int32)
+            return V, F, r
+    raise RuntimeError(f"no watertight manifold contours near grid {res} for seed {seed}")
 
 
 def vertex_normals(V: np.ndarray, F: np.ndarray):

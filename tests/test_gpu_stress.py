@@ -1,0 +1,74 @@
+"""Full-size runs checked through size-independent guarantees.
+
+The workloads are the 2M-face headline mesh (ellipsoid/box union, ratio 0.01,
+stages 2+3) and the intersection stress case: two closely separated
+parallel sheets of about 8M faces in total (the union's contour and the
+contour one level inside it, reversed), ratio 0.0025, tolerance 8, stage 2
+capped at 400 rounds.  At these sizes the oracle is too slow to run in a test, so the outputs
+are checked for what the reference's method guarantees: watertight and
+manifold, no degenerate face, the Euler characteristic of the input, and
+(when the loop is not capped) the face target.  The inputs are checked with
+the exhaustive exact-predicate check (oracle/exact_check.cpp) first.  The
+reference's guard predicate is not exact: on the stress sheets it misses
+some crossings between nearly parallel faces that the exact check finds, so
+the exact count of the outputs is reported, not required to be zero.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact
+from pamo_amd import synth
+from pamo_amd.pamo import PaMO
+
+pytestmark = pytest.mark.gpu
+
+
+def _guarantees(V, F, V_in, F_in, target, what):
+    """Checks the reference's guarantees; returns the exact intersection count."""
+    if target is not None:
+        assert len(F) <= target, f"{what}: {len(F)} faces, target {target}"
+    assert synth.is_watertight_manifold(F, len(V)), f"{what}: not watertight and manifold"
+    assert synth.euler_characteristic(F, len(V)) == synth.euler_characteristic(F_in, len(V_in))
+    pairs, cand, degen = exact.self_intersections(V, F)
+    assert degen == 0, f"{what}: {degen} degenerate faces"
+    print(f"{what}: {len(F)} faces, exact check: {len(pairs)} intersecting pairs of {cand} candidates", flush=True)
+    return len(pairs)
+
+
+def _run(V, F, ratio, tolerance, stage3, iters=100000):
+    pm = PaMO((V, F), use_stage1=False, use_stage3=stage3)
+    v, f = pm.run(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda(), ratio, tolerance, 1e-3, iters, 0)
+    return np.asarray(v, np.float32), np.asarray(f, np.int32), len(pm.face_hist)
+
+
+def test_headline_full_size_guarantees():
+    V, F = synth.headline_mesh(1)
+    print(f"case: headline input {len(F)} faces, exact input check", flush=True)
+    pairs, _, degen = exact.self_intersections(V, F)
+    assert len(pairs) == 0 and degen == 0
+    target = int(0.01 * len(F))
+    print("case: headline stage 2 only", flush=True)
+    V2, F2, _ = _run(V, F, 0.01, 4, False)
+    n2 = _guarantees(V2, F2, V, F, target, "stage-2 output")
+    print("case: headline stages 2+3", flush=True)
+    V3, F3, _ = _run(V, F, 0.01, 4, True)
+    assert np.array_equal(F3.astype(np.int64), F2.astype(np.int64))  # stage 3 moves vertices only
+    n3 = _guarantees(V3, F3, V, F, target, "stage-3 output")
+    assert n3 <= n2  # the safe projection adds no crossing
+
+
+def test_stress_sheets_8m_guarantees():
+    V, F, res = synth.stress_sheets(500, 8e6)
+    print(f"case: stress sheets seed 500, grid {res}, {len(F)} faces, exact input check", flush=True)
+    pairs, cand, degen = exact.self_intersections(V, F)
+    assert len(pairs) == 0 and degen == 0, f"input: {len(pairs)} pairs, {degen} degenerate"
+    assert len(F) > 7.5e6
+    # The guard undoes most collapses between sheets this close (the sheet
+    # gap is about a fifth of an edge), so the loop is capped (the
+    # reference's iter argument) long before the face target; the capped
+    # output still has millions of faces, too many for a stage-3 test.
+    print("case: stress sheets stage 2, ratio 0.0025, tolerance 8, 400 rounds", flush=True)
+    V2, F2, rounds = _run(V, F, 0.0025, 8, False, iters=400)
+    assert len(F2) < 0.5 * len(F)
+    _guarantees(V2, F2, V, F, None if rounds >= 400 else int(0.0025 * len(F)), "stress output")
