@@ -22,8 +22,9 @@ int pamo_ctx_create(pamo_ctx** out, int device);
 int pamo_ctx_destroy(pamo_ctx* ctx);
 
 /* Returns a context to the state of a freshly created one, keeping its
- * device allocations (no counterpart in the reference, which constructs a
- * new CUDSP_Free per PaMO object, simp_cuda/pamo/__init__.py:21). */
+ * device allocations and the options set with pamo_ctx_set_option (no
+ * counterpart in the reference, which constructs a new CUDSP_Free per PaMO
+ * object, simp_cuda/pamo/__init__.py:21). */
 int pamo_ctx_reset(pamo_ctx* ctx);
 
 /* Test option (no counterpart in the reference): with value 1 the guard
@@ -31,6 +32,14 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * of the frontier traversal is exercised; 0 restores the default.  The
  * result is identical either way. */
 #define PAMO_OPT_GUARD_SMALL_STACK 1
+/* Guard predicate (no counterpart in the reference, whose guard predicate
+ * is not exact): 0 (default) the reference's predicate and skip rules,
+ * bit-exact with the reference; 1 exact -- a candidate pair is flagged
+ * exactly when the two faces intersect beyond their shared vertices, decided
+ * with FP64-filtered orientation predicates and an exact pass for the pairs
+ * the filter cannot decide; 2 as 1 with every pair sent through the exact
+ * pass (test setting).  A context keeps the setting across calls. */
+#define PAMO_OPT_EXACT_GUARD 2
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).

@@ -50,6 +50,8 @@
 #include <omp.h>
 #endif
 
+#include "exact_pred.h"
+
 namespace orc {
 
 struct V3 {
@@ -766,7 +768,17 @@ static void guard(Ctx& c, const std::vector<uint8_t>& changed, float epsilon) {
                            c.P[tb.k].x, c.P[tb.k].y, c.P[tb.k].z};
             int ib[3] = {tb.i, tb.j, tb.k};
             pairs += 2;
-            bool hit = ref_guard(fb, ib, fa, ia, epsilon) || ref_guard(fa, ia, fb, ib, epsilon);
+            bool hit;
+            if (c.pred_mode == 1) {  // exact mode: the exact, adjacency-aware intersection test
+                exact::P3 pa[3], pb[3];
+                for (int k = 0; k < 3; ++k) {
+                    pa[k] = exact::P3{fa[3 * k], fa[3 * k + 1], fa[3 * k + 2]};
+                    pb[k] = exact::P3{fb[3 * k], fb[3 * k + 1], fb[3 * k + 2]};
+                }
+                hit = exact::faces_intersect(ia, pa, ib, pb);
+            } else {
+                hit = ref_guard(fb, ib, fa, ia, epsilon) || ref_guard(fa, ia, fb, ib, epsilon);
+            }
             if (hit) hits[qi].push_back(bf);
         }
     }

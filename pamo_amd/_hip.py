@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpamo_hip.so")
 c_int, c_float, c_double, c_ll, c_vp = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
 
 PAMO_OPT_GUARD_SMALL_STACK = 1
+PAMO_OPT_EXACT_GUARD = 2
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3

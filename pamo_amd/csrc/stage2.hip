@@ -136,6 +136,9 @@ struct Ctx {
     int guard_small_stack = 0;  // test option: 32-entry guard frontier
     double* cent_part = nullptr;  // per-block partial sums of pamo_centroid
     size_t alloc_cent = 0;
+    int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
+    int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
+    size_t alloc_ulist = 0;
     Profiler prof;
 };
 
@@ -309,7 +312,7 @@ static void free_all(Ctx& c) {
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
                     c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
-                    c.shuf, c.ws, c.dcnt, c.cent_part};
+                    c.shuf, c.ws, c.dcnt, c.cent_part, c.ulist};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
@@ -398,12 +401,30 @@ static int run_guard(Ctx& c, int nT) {
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
     const long long guard_threads = (((long long)nT + kGuardQ - 1) / kGuardQ) * kWave;
     const double guard_bytes = 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT;
-    if (c.guard_small_stack)
-        LAUNCHP(K_GUARD_QUERY, guard_bytes, guard_query_k<kGuardStackSmall>, guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged);
-    else
-        LAUNCHP(K_GUARD_QUERY, guard_bytes, guard_query_k<kGuardStack>, guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged);
+    Undecided u{nullptr, c.dcnt + C_UNDECIDED, 0, c.exact_guard == 2 ? 1 : 0};
+    if (c.exact_guard) {
+        // list capacity: undecided pairs are near-degenerate ones (a few per
+        // changed face at most); the test setting lists every candidate pair
+        const size_t cap = c.exact_guard == 2 ? 48 * (size_t)c.alloc_tris : std::max<size_t>(1 << 16, 2 * c.alloc_tris);
+        if (cap > c.alloc_ulist) {
+            PAMO_TRY(dalloc(&c.ulist, cap, c.stream));
+            c.alloc_ulist = cap;
+        }
+        u.list = c.ulist;
+        u.cap = (int)std::min<size_t>(c.alloc_ulist, 0x7fffffff);
+        PAMO_HIP(hipMemsetAsync(c.dcnt + C_UNDECIDED, 0, sizeof(int), c.stream));
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, true>), guard_threads, m, c.qlist, c.dcnt,
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+        hipLaunchKernelGGL(exact_pairs_k, dim3(1024), dim3(kBlock), 0, c.stream, m, (const int2*)c.ulist,
+                           (const int*)(c.dcnt + C_UNDECIDED), u.cap, c.flagged);
+        PAMO_HIP(hipGetLastError());
+    } else if (c.guard_small_stack) {
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStackSmall, false>), guard_threads, m, c.qlist,
+                c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+    } else {
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, false>), guard_threads, m, c.qlist, c.dcnt,
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+    }
     return 0;
 }
 
@@ -414,6 +435,8 @@ static int undo_candidates(Ctx& c, int& npushed) {
     LAUNCH(undo_compact_k, std::max(c.n_rec, 1), c.recs, c.n_rec, c.pflag, c.esel, c.pushed, c.dcnt);
     PAMO_TRY(sync_counters(c));
     npushed = c.hcnt[C_PUSHED];
+    PAMO_CHECK(!c.exact_guard || c.hcnt[C_UNDECIDED] <= (long long)c.alloc_ulist,
+               "exact guard: more pairs left undecided than the list holds");
     return 0;
 }
 
@@ -626,9 +649,14 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     pamo::clear_error();
     PAMO_CHECK(ctx != nullptr, "pamo_ctx_set_option: null context");
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
-    PAMO_CHECK(option == PAMO_OPT_GUARD_SMALL_STACK, "pamo_ctx_set_option: unknown option");
-    PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: value must be 0 or 1");
-    c.guard_small_stack = (int)value;
+    if (option == PAMO_OPT_GUARD_SMALL_STACK) {
+        PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: value must be 0 or 1");
+        c.guard_small_stack = (int)value;
+        return 0;
+    }
+    PAMO_CHECK(option == PAMO_OPT_EXACT_GUARD, "pamo_ctx_set_option: unknown option");
+    PAMO_CHECK(value >= 0 && value <= 2, "pamo_ctx_set_option: exact guard value must be 0, 1 or 2");
+    c.exact_guard = (int)value;
     return 0;
 }
 

@@ -8,6 +8,7 @@
 //               ring face with one 16-byte load;
 //   quadrics    3 float4 per vertex (10 symmetric entries + 2 pad).
 #pragma once
+#include "exact_guard.h"
 #include "guard_ref.h"
 #include "hip_common.h"
 
@@ -65,6 +66,7 @@ enum Counter {
     C_NF = 5,       // next-round faces
     C_NEDGE = 6,
     C_NQUERY = 7,   // guard queries
+    C_UNDECIDED = 8,  // exact guard: pairs the filtered predicates left undecided
     C_COUNT = 16
 };
 
@@ -960,30 +962,99 @@ constexpr int kGuardStackSmall = 32;   // test variant: forces the overflow walk
 
 // Candidate filter for queried face a (vertex ids ia0..ia2) and a live face b
 // whose box overlaps a's (closed intervals): b != a; a pair of queried faces
-// is taken from the larger index only; pairs sharing two vertex indices are
-// dropped before any arithmetic (the reference rejects them after its
-// coplanarity test in every outcome).
+// is taken from the larger index only; with the reference predicate, pairs
+// sharing two vertex indices are dropped before any arithmetic (the
+// reference rejects them after its coplanarity test in every outcome).
+template <bool kExact = false>
 __device__ __forceinline__ bool guard_candidate(Mesh m, const uint8_t* __restrict__ changed, int a, int ia0,
                                                 int ia1, int ia2, int b) {
     if (b == a || (changed[b] && b < a)) return false;
+    if (kExact) return true;  // the exact test also judges pairs sharing an edge (flat folds)
     const int b0 = m.i[b], b1 = m.j[b], b2 = m.k[b];
     const int sh = (ia0 == b0) + (ia0 == b1) + (ia0 == b2) + (ia1 == b0) + (ia1 == b1) + (ia1 == b2) +
                    (ia2 == b0) + (ia2 == b1) + (ia2 == b2);
     return sh != 2;
 }
 
+// Exact mode: where the undecided pairs go.
+struct Undecided {
+    int2* list;
+    int* count;
+    int cap;
+    int force;  // test setting: every pair goes to the exact pass
+};
+
+__device__ __forceinline__ void load_xtri(Mesh m, int f, XP3* p, int* idx) {
+    idx[0] = m.i[f]; idx[1] = m.j[f]; idx[2] = m.k[f];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const float4 q = m.p[idx[v]];
+        p[v] = XP3{q.x, q.y, q.z};
+    }
+}
+
+// Exact mode, first pass: the decision with filtered predicates; a pair they
+// cannot decide is listed for exact_pairs_k.
+__device__ __noinline__ void eval_pair_filtered(Mesh m, int a, int b, uint8_t* __restrict__ flagged, Undecided u) {
+    if (!u.force) {
+        XP3 pa[3], pb[3];
+        int ia[3], ib[3];
+        load_xtri(m, a, pa, ia);
+        load_xtri(m, b, pb, ib);
+        XPred<false> pr;
+        const bool hit = x_faces_intersect(pr, ia, pa, ib, pb);
+        if (!pr.undecided) {
+            if (hit) {
+                flagged[a] = 1;
+                flagged[b] = 1;
+            }
+            return;
+        }
+    }
+    const int k = atomicAdd(u.count, 1);
+    if (k < u.cap) u.list[k] = make_int2(a, b);
+}
+
+template <bool kExact>
+__device__ __forceinline__ void guard_eval(Mesh m, int a, int b, float epsilon, uint8_t* __restrict__ flagged,
+                                           Undecided u) {
+    if (kExact) eval_pair_filtered(m, a, b, flagged, u);
+    else eval_pair(m, a, b, epsilon, flagged);
+}
+
+// Exact mode, second pass: the listed pairs with exact predicates.
+__global__ void __launch_bounds__(kBlock) exact_pairs_k(Mesh m, const int2* __restrict__ list,
+                                                        const int* __restrict__ count, int cap,
+                                                        uint8_t* __restrict__ flagged) {
+    const int n = min(*count, cap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int2 e = list[k];
+        XP3 pa[3], pb[3];
+        int ia[3], ib[3];
+        load_xtri(m, e.x, pa, ia);
+        load_xtri(m, e.y, pb, ib);
+        XPred<true> pr;
+        if (x_faces_intersect(pr, ia, pa, ib, pb)) {
+            flagged[e.x] = 1;
+            flagged[e.y] = 1;
+        }
+    }
+}
+
 // Stackless walk of the subtree below internal node x: from its left child
 // until x's escape, evaluating candidates in this lane.
+template <bool kExact>
 __device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restrict__ changed,
                                                 const float4* __restrict__ node, int n, int x, int stop,
                                                 float4 qlo, float4 qhi, int a, int ia0, int ia1, int ia2,
-                                                float epsilon, uint8_t* __restrict__ flagged) {
+                                                float epsilon, uint8_t* __restrict__ flagged, Undecided u) {
     while (x != stop) {
         const float4 lo = node[2 * (long long)x], hi = node[2 * (long long)x + 1];
         const int lw = __float_as_int(lo.w), esc = __float_as_int(hi.w);
         if (overlap(qlo, qhi, lo, hi)) {
             if (x >= n - 1) {
-                if (guard_candidate(m, changed, a, ia0, ia1, ia2, lw)) eval_pair(m, a, lw, epsilon, flagged);
+                if (guard_candidate<kExact>(m, changed, a, ia0, ia1, ia2, lw))
+                    guard_eval<kExact>(m, a, lw, epsilon, flagged, u);
                 x = esc;
             } else {
                 x = lw;
@@ -1006,14 +1077,14 @@ __device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restric
 // When the frontier is full, a lane walks the rest of its subtree by escape
 // links (exact, only slower).  The flags are idempotent, so the visiting
 // order does not change the result.
-template <int kStackCap>
+template <int kStackCap, bool kExact>
 __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
                                                         const int* __restrict__ cnt,
                                                         const uint8_t* __restrict__ changed,
                                                         const float4* __restrict__ fbox,
                                                         const float4* __restrict__ node,
                                                         const int* __restrict__ right, float epsilon,
-                                                        uint8_t* __restrict__ flagged) {
+                                                        uint8_t* __restrict__ flagged, Undecided u) {
     constexpr int kW = kBlock / kWave;
     __shared__ int2 stk[kW][kStackCap];
     __shared__ int2 pq[kW][kQueue];
@@ -1073,7 +1144,8 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
                 stk[w][size + 2 * below + 1] = make_int2(l, c1);
             } else {
                 const int4 id = qid[w][l];
-                guard_walk_subtree(m, changed, node, n, c0, esc, qlo, qhi, id.x, id.y, id.z, id.w, epsilon, flagged);
+                guard_walk_subtree<kExact>(m, changed, node, n, c0, esc, qlo, qhi, id.x, id.y, id.z, id.w, epsilon,
+                                           flagged, u);
             }
         }
         size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));
@@ -1081,7 +1153,8 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
         if (qn >= kWave) {
             const int2 e = pq[w][qn - kWave + lane];
             const int4 id = qid[w][e.x];
-            if (guard_candidate(m, changed, id.x, id.y, id.z, id.w, e.y)) eval_pair(m, id.x, e.y, epsilon, flagged);
+            if (guard_candidate<kExact>(m, changed, id.x, id.y, id.z, id.w, e.y))
+                guard_eval<kExact>(m, id.x, e.y, epsilon, flagged, u);
             qn -= kWave;
             wave_sync();
         }
@@ -1089,7 +1162,8 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
     if (lane < qn) {
         const int2 e = pq[w][lane];
         const int4 id = qid[w][e.x];
-        if (guard_candidate(m, changed, id.x, id.y, id.z, id.w, e.y)) eval_pair(m, id.x, e.y, epsilon, flagged);
+        if (guard_candidate<kExact>(m, changed, id.x, id.y, id.z, id.w, e.y))
+            guard_eval<kExact>(m, id.x, e.y, epsilon, flagged, u);
     }
 }
 

@@ -122,9 +122,13 @@ class PaMO(torch.nn.Module):
     ``input_mesh`` is a trimesh-like object (``vertices``, ``faces`` and,
     optionally, ``bounding_box.bounds``) or a ``(vertices, faces)`` pair.
     Stage 1 (SDF remeshing) is not part of this package: asking for it raises.
+    ``exact_guard=True`` (not in the reference's signature) replaces the
+    reference's guard predicate by the exact intersection test, so that no
+    collapse that makes two faces intersect survives; the default keeps the
+    reference's predicate and results.
     """
 
-    def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True):
+    def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True, exact_guard: bool = False):
         super().__init__()
         if use_stage1:
             raise NotImplementedError(
@@ -141,6 +145,8 @@ class PaMO(torch.nn.Module):
         bb = getattr(input_mesh, "bounding_box", None)
         self.bbox = bb.bounds if bb is not None else _Bounds(self.gt_vertices).bounds
         self.ctx = _hip.Context(torch.cuda.current_device())
+        if exact_guard:
+            _hip.check(_hip.lib().pamo_ctx_set_option(self.ctx.ptr, _hip.PAMO_OPT_EXACT_GUARD, 1), "set_option")
         if use_stage3:
             from . import stage3
             self.config = stage3.Stage3Config()

@@ -139,6 +139,36 @@ def test_guard_frontier_matches_oracle(torch_cuda, oracle_lib, small_stack):
     assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32))
 
 
+@pytest.mark.parametrize("setting", [1, 2])
+@pytest.mark.parametrize("case", ["stress sheets", "union 96"])
+def test_exact_guard_matches_oracle(torch_cuda, oracle_lib, case, setting):
+    """Exact guard mode (setting 1: filtered predicates with the exact pass
+    for undecided pairs; 2: every pair through the exact pass) against the
+    oracle's exact mode, bit for bit; the output has no intersecting pair
+    under the exhaustive exact check."""
+    from oracle import exact
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    if case == "stress sheets":
+        V, F, _ = synth.stress_sheets(500, 1e5)
+        ratio, tol, iters = 0.0025, 8, 80
+    else:
+        V, F = synth.ellipsoid_box_union(1, 96)
+        ratio, tol, iters = 0.01, 4, 100000
+    Vc = centred(V, F)
+    print(f"case: exact guard setting {setting}, {case}, {len(F)} faces", flush=True)
+    ctx = _hip.Context(0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_EXACT_GUARD, setting), "set_option")
+    vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), ratio,
+                                 tolerance=tol, iters=iters, min_verts=0)
+    vo, fo, ho = oracle_lib.run(Vc, F, ratio, tolerance=tol, iters=iters, min_verts=0, pred_mode=1)
+    assert np.array_equal(hg, ho)
+    assert np.array_equal(fg.cpu().numpy(), fo)
+    assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32))
+    assert len(exact.self_intersections(vo, fo)[0]) == 0
+
+
 def test_fused_loop_reuses_context(torch_cuda, oracle_lib):
     """Two runs on one context (as PaMO reuses one CUDSP_Free) keep the
     reference's persistent-buffer semantics."""
