@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
     p.add_argument("--stage3", type=int, default=1, help="1/0: stage 3 on/off")
+    p.add_argument("--exact-guard", action="store_true",
+                   help="exact intersection guard instead of the reference's predicate (results differ from the reference)")
     p.add_argument("--iters", type=int, default=100000,
                    help="cap on stage-2 rounds (the reference's iter; the workload uses 100000)")
     p.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
@@ -194,7 +196,7 @@ def main():
     Vd = torch.from_numpy(V).to(dev)
     Fd = torch.from_numpy(F).to(dev)
     use_s3 = bool(args.stage3)
-    pm = PaMO((V, F), use_stage1=False, use_stage3=use_s3)
+    pm = PaMO((V, F), use_stage1=False, use_stage3=use_s3, exact_guard=args.exact_guard)
     ctx = pm.ctx
     prof = _hip.Profiler(ctx)
 
@@ -289,7 +291,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ellipsoid/box union, marching tetrahedra grid 296", "faces_in": nF0,
                    "verts_in": nV0, "faces_out": int(len(fo)), "ratio": ratio, "tolerance": tol,
-                   "threshold": thr, "rounds": int(len(hist)), "stage3": use_s3},
+                   "threshold": thr, "rounds": int(len(hist)), "stage3": use_s3,
+                   "guard": "exact" if args.exact_guard else "reference predicate"},
         "stage2_s": round(s2 / args.steps, 4),
         "stage3_s": round(s3 / args.steps, 4),
         "faces_per_s": round(nF0 / per_step, 1),
