@@ -215,6 +215,11 @@ struct Ctx {
     // incidence lists (sorted by element) and per-slot HVP outputs
     int *vf_off = nullptr, *vf_ent = nullptr, *vh_off = nullptr, *vh_ent = nullptr;
     int *vc_off = nullptr, *vc_ent = nullptr;
+    // inverse incidence: position in the vertex lists of face slot 3 f + k /
+    // hinge slot 4 h + a, so element contributions land in list order
+    int *vf_dst = nullptr, *vh_dst = nullptr;
+    float4* cdv = nullptr;  // active-contact corner gradients in vertex-list order
+    int* cj = nullptr;      // their active-contact index
     float4 *fc = nullptr, *hc = nullptr;
     float* dn = nullptr;
     // active contacts (d < d_hat) of the current Newton iteration
@@ -261,6 +266,7 @@ static void ctx_free(Ctx& c) {
                     c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
                     c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
                     c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.sA, c.skey, c.sval, c.skey2,
+                    c.vf_dst, c.vh_dst, c.cdv, c.cj,
                     c.sval2, c.zr_part, c.vav_part};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -889,13 +895,15 @@ __device__ __forceinline__ V3 cg_dx(const CgVec& cv, float beta, int v) {
     return cv.first ? v3(cv.z[v]) : v3(cv.z[v]) + beta * v3(cv.v_old[v]);
 }
 
-// items: [0, nF) faces -> fc[3f + k]; [nF, +nH) hinges -> hc[4h + a];
+// items: [0, nF) faces -> fc[f_dst[3f + k]]; [nF, +nH) hinges -> hc[h_dst[4h + a]]
+// (vertex-list order, so the gather reads them contiguously);
 // [.., +nA) active contacts -> sA[j] = c2 (g . dx); [.., +3 nV_dense) dense rows
 __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, Topo tp, Params P, int nF, int nH,
                                                  int nA_max, const int* __restrict__ n_act, int n_dense,
                                                  CgVec cv, const float* __restrict__ cd, const int* __restrict__ act,
                                                  const float* __restrict__ hsym, const float4* __restrict__ cdd,
-                                                 const double* __restrict__ dA, float4* __restrict__ fc,
+                                                 const double* __restrict__ dA, const int* __restrict__ f_dst,
+                                                 const int* __restrict__ h_dst, float4* __restrict__ fc,
                                                  float4* __restrict__ hc, float* __restrict__ sA,
                                                  float* __restrict__ dn) {
     __shared__ double red[kB];
@@ -935,9 +943,9 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
         for (int k = 0; k < 4; ++k) dH[k] *= W;
         V3 df1 = dH[0] * m.b1 + dH[2] * m.b2;
         V3 df2 = dH[1] * m.b1 + dH[3] * m.b2;
-        fc[3 * i] = f4(-df1 - df2);
-        fc[3 * i + 1] = f4(df1);
-        fc[3 * i + 2] = f4(df2);
+        fc[f_dst[3 * i]] = f4(-df1 - df2);
+        fc[f_dst[3 * i + 1]] = f4(df1);
+        fc[f_dst[3 * i + 2]] = f4(df2);
     } else if ((i -= nF) < nH) {
         // projected hinge Hessian: rows / columns of vertex 3 follow from the
         // 9x9 block by translation invariance, so y = A (d_a - d_3) and the
@@ -959,10 +967,10 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
                 if (c != r) y[c] += a * u[r];
             }
         V3 y0 = mk(y[0], y[1], y[2]), y1 = mk(y[3], y[4], y[5]), y2 = mk(y[6], y[7], y[8]);
-        hc[4 * i] = f4(y0);
-        hc[4 * i + 1] = f4(y1);
-        hc[4 * i + 2] = f4(y2);
-        hc[4 * i + 3] = f4(-(y0 + y1 + y2));
+        hc[h_dst[4 * i]] = f4(y0);
+        hc[h_dst[4 * i + 1]] = f4(y1);
+        hc[h_dst[4 * i + 2]] = f4(y2);
+        hc[h_dst[4 * i + 3]] = f4(-(y0 + y1 + y2));
     } else if ((i -= nH) < nA_max) {
         if (i >= n_act[0]) return;
         const int c = act[i];
@@ -984,9 +992,11 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
 }
 
 struct Incid {
-    const int *vf_off, *vf_ent;  // vertex -> face corner slots (3 f + k)
-    const int *vh_off, *vh_ent;  // vertex -> hinge slots (4 h + a)
-    const int *vc_off, *vc_ent;  // vertex -> active contact slots (4 j + a)
+    const int* vf_off;  // vertex -> face corner entries of fc (list order)
+    const int* vh_off;  // vertex -> hinge corner entries of hc (list order)
+    const int* vc_off;  // vertex -> active contact corners: gradient cdv[e], contact cj[e]
+    const float4* cdv;
+    const int* cj;
 };
 
 // Av = H dx per vertex; the direction dx is stored to v_out; block partials
@@ -1010,14 +1020,11 @@ __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2g
     V3 acc = mk(0.0f, 0.0f, 0.0f);
     if (v < nV) {
         if (use_elem)
-            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) acc = acc + v3(fc[ic.vf_ent[e]]);
+            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) acc = acc + v3(fc[e]);
         if (use_hinge)
-            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) acc = acc + v3(hc[ic.vh_ent[e]]);
+            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) acc = acc + v3(hc[e]);
         if (use_contact)
-            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
-                const int s = ic.vc_ent[e];
-                acc = acc + v3(cdd[s]) * sA[s >> 2];
-            }
+            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) acc = acc + v3(ic.cdv[e]) * sA[ic.cj[e]];
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
@@ -1087,6 +1094,20 @@ __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, 
 }
 
 // ---- incidence lists
+// dst[ent[e]] = e: the list position of every slot
+__global__ void invert_k(const int* __restrict__ ent, int n, int* __restrict__ dst) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[ent[e]] = e;
+}
+// active-contact corner gradients and contact indices in vertex-list order
+__global__ void contact_permute_k(const int* __restrict__ ent, int n, const float4* __restrict__ cdd,
+                                  float4* __restrict__ cdv, int* __restrict__ cj) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int s = ent[e];
+    cdv[e] = cdd[s];
+    cj[e] = s >> 2;
+}
 // keys[s] = vertex of slot s of element s / width, vals[s] = s
 __global__ void slot_keys_k(const int* __restrict__ elem, int width, int stride, int n_slots,
                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -1337,6 +1358,7 @@ static int compute_diff(Ctx& c, int nC, int mask) {
         PAMO_TRY(ensure_sort(c, n));
         L3(contact_keys_k, n, (const int4*)c.cidx, (const int*)c.act, (const int*)c.n_act, c.skey, c.sval);
         PAMO_TRY(finish_incidence(c, n, c.vc_off, c.vc_ent));
+        L3(contact_permute_k, n, (const int*)c.vc_ent, (int)n, (const float4*)c.cdd, c.cdv, c.cj);
     } else {
         PAMO_TRY(fill(c.vc_off, 0, c.nV + 1, c.stream));
     }
@@ -1354,8 +1376,8 @@ static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4*
     if (total > 0)
         L3(hvp_elem_k, total, (const float4*)c.q, topo(c), c.P, nF, nH, nA, (const int*)c.n_act, nd, cv,
            (const float*)c.cd, (const int*)c.act, (const float*)c.hsym, (const float4*)c.cdd,
-           (const double*)c.dense_A, c.fc, c.hc, c.sA, c.dn);
-    Incid ic{c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent};
+           (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, c.sA, c.dn);
+    Incid ic{c.vf_off, c.vh_off, c.vc_off, (const float4*)c.cdv, (const int*)c.cj};
     L3(hvp_gather_k, (long long)c.nV * kGather, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
        (int)(nA > 0), (int)(nd > 0), (const float*)c.vor, cv, ic, (const float4*)c.fc, (const float4*)c.hc,
        (const float*)c.sA, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
@@ -1376,6 +1398,8 @@ static int ensure_contacts(Ctx& c, int n) {
     PAMO_TRY(dalloc(&c.act_pos, (size_t)a + 1, c.stream));
     PAMO_TRY(dalloc(&c.sA, a, c.stream));
     PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.cdv, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.cj, 4 * (size_t)a, c.stream));
     return 0;
 }
 
@@ -1749,6 +1773,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         PAMO_TRY(dalloc(&c.area, c.capF, c.stream));
         PAMO_TRY(dalloc(&c.inv_dm, c.capF, c.stream));
         PAMO_TRY(dalloc(&c.vf_ent, 3 * (size_t)c.capF, c.stream));
+        PAMO_TRY(dalloc(&c.vf_dst, 3 * (size_t)c.capF, c.stream));
         PAMO_TRY(dalloc(&c.fc, 3 * (size_t)c.capF, c.stream));
     }
     if (n_e > c.capE) {
@@ -1763,6 +1788,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         PAMO_TRY(dalloc(&c.hblock, 144 * (size_t)c.capH, c.stream));
         PAMO_TRY(dalloc(&c.hsym, 45 * (size_t)c.capH, c.stream));
         PAMO_TRY(dalloc(&c.vh_ent, 4 * (size_t)c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.vh_dst, 4 * (size_t)c.capH, c.stream));
         PAMO_TRY(dalloc(&c.hc, 4 * (size_t)c.capH, c.stream));
     }
     c.nV = n_v;
@@ -1794,6 +1820,13 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
     // vertex -> face corner / hinge slot incidence, fixed for this mesh
     PAMO_TRY(pamo::s3::build_incidence(c, (const int*)c.tri, n_f, 3, 4, c.vf_off, c.vf_ent));
     PAMO_TRY(pamo::s3::build_incidence(c, (const int*)c.hinge, n_h, 4, 4, c.vh_off, c.vh_ent));
+    if (n_f > 0)
+        hipLaunchKernelGGL(pamo::s3::invert_k, dim3(pamo::grid_for(3LL * n_f, 256)), dim3(256), 0, c.stream,
+                           (const int*)c.vf_ent, 3 * n_f, c.vf_dst);
+    if (n_h > 0)
+        hipLaunchKernelGGL(pamo::s3::invert_k, dim3(pamo::grid_for(4LL * n_h, 256)), dim3(256), 0, c.stream,
+                           (const int*)c.vh_ent, 4 * n_h, c.vh_dst);
+    PAMO_HIP(hipGetLastError());
     PAMO_HIP(hipStreamSynchronize(c.stream));
     return 0;
 }
