@@ -157,15 +157,33 @@ __global__ void __launch_bounds__(kSortBlock) radix_hist_k(const uint32_t* __res
     counts[(long long)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// raw = 0: offs holds the exclusive scan of the per-block digit counts
+// (digit-major); raw = 1: offs holds the counts themselves and each block
+// forms its own offsets -- all blocks' totals of smaller digits plus the
+// counts of earlier blocks for its digit (used for small grids, where it
+// replaces two scan launches).
 __global__ void __launch_bounds__(kSortBlock) radix_scatter_k(const uint32_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint32_t* __restrict__ kout,
                                                               uint32_t* __restrict__ vout, int n, int shift,
-                                                              const int* __restrict__ offs, int nb) {
+                                                              const int* __restrict__ offs, int nb, int raw) {
     __shared__ int base[256];
     __shared__ int wcnt[4][256];
+    __shared__ int lds4[4];
     const int t = threadIdx.x, w = t >> 6;
-    base[t] = offs[(long long)t * nb + blockIdx.x];
+    if (raw) {
+        int before = 0, total = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int c = offs[(long long)t * nb + b];
+            if (b < (int)blockIdx.x) before += c;
+            total += c;
+        }
+        int all;
+        const int smaller = block_exclusive_scan(total, lds4, all);
+        base[t] = smaller + before;
+    } else {
+        base[t] = offs[(long long)t * nb + blockIdx.x];
+    }
     const long long tile = (long long)blockIdx.x * kSortTile;
     for (int c = 0; c < kSortIpt; ++c) {
         const long long idx = tile + c * kSortBlock + t;
@@ -204,6 +222,10 @@ __global__ void __launch_bounds__(kSortBlock) radix_scatter_k(const uint32_t* __
     }
 }
 
+// grids of at most this many tiles form their digit offsets inside the
+// scatter kernel (each block reads 256 x nb counts)
+constexpr int kRawOffsetBlocks = 64;
+
 inline long long radix_workspace(long long n) {
     long long nb = (n + kSortTile - 1) / kSortTile;
     return 256 * nb + scan_workspace(256 * nb) + 16;
@@ -223,10 +245,11 @@ inline int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* k2, uint32
         hipLaunchKernelGGL(radix_hist_k, dim3(nb), dim3(kSortBlock), 0, st, ka, n, shift, counts, nb);
         prof_end(K_RADIX_HIST, st);
         PAMO_HIP(hipGetLastError());
-        PAMO_TRY(scan_exclusive(counts, counts, 256LL * nb, sws, st));
+        const int raw = nb <= kRawOffsetBlocks ? 1 : 0;
+        if (!raw) PAMO_TRY(scan_exclusive(counts, counts, 256LL * nb, sws, st));
         prof_begin(K_RADIX_SCATTER, 16.0 * n + 1024.0 * nb, st);
         hipLaunchKernelGGL(radix_scatter_k, dim3(nb), dim3(kSortBlock), 0, st, ka, va, kb, vb, n, shift,
-                           (const int*)counts, nb);
+                           (const int*)counts, nb, raw);
         prof_end(K_RADIX_SCATTER, st);
         PAMO_HIP(hipGetLastError());
         std::swap(ka, kb);

@@ -815,6 +815,27 @@ __device__ __forceinline__ void grow(float4& lo, float4& hi, const float4* bx, l
     hi.x = fmaxf(hi.x, b.x); hi.y = fmaxf(hi.y, b.y); hi.z = fmaxf(hi.z, b.z);
 }
 
+// Union of boxes bx[base + q] for q in [q0, q1]: eight independent loads per
+// step, so a long range costs range/8 memory round trips instead of range
+// (min / max are exact and order-free, so the result is the same).
+__device__ __forceinline__ void grow_range(float4& lo, float4& hi, const float4* bx, long long base, int q0, int q1) {
+    int q = q0;
+    for (; q + 8 <= q1 + 1; q += 8) {
+        float4 a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a[k] = bx[2 * (base + q + k)];
+            b[k] = bx[2 * (base + q + k) + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            lo.x = fminf(lo.x, a[k].x); lo.y = fminf(lo.y, a[k].y); lo.z = fminf(lo.z, a[k].z);
+            hi.x = fmaxf(hi.x, b[k].x); hi.y = fmaxf(hi.y, b[k].y); hi.z = fmaxf(hi.z, b[k].z);
+        }
+    }
+    for (; q <= q1; ++q) grow(lo, hi, bx, base + q);
+}
+
 // Box of every internal node from its leaf range [a, b]: direct loop inside
 // one 256-leaf block, otherwise suffix(a) + prefix(b) + the full blocks in
 // between (again via block-level suffix / prefix and full superblocks).
@@ -832,7 +853,7 @@ __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ ra
     float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
     const int A = r.x >> 8, B = r.y >> 8;
     if (A == B) {
-        for (int s = r.x; s <= r.y; ++s) grow(lo, hi, leaf, (long long)(n - 1) + s);
+        grow_range(lo, hi, leaf, (long long)(n - 1), r.x, r.y);
     } else {
         grow(lo, hi, suf, r.x);
         grow(lo, hi, pre, r.y);
@@ -840,11 +861,11 @@ __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ ra
         if (A1 <= B1) {
             const int SA = A1 >> 8, SB = B1 >> 8;
             if (SA == SB) {
-                for (int q = A1; q <= B1; ++q) grow(lo, hi, blk, q);
+                grow_range(lo, hi, blk, 0, A1, B1);
             } else {
                 grow(lo, hi, bsuf, A1);
                 grow(lo, hi, bpre, B1);
-                for (int q = SA + 1; q <= SB - 1; ++q) grow(lo, hi, sblk, q);
+                grow_range(lo, hi, sblk, 0, SA + 1, SB - 1);
             }
         }
     }
