@@ -448,26 +448,92 @@ __global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__
 // Each lane walks the rope-linked tree for its query one node per loop
 // iteration and queues box hits in LDS; the wave classifies 64 queued pairs
 // at a time and reserves output slots with one atomic per wave.
-constexpr int kDetQ = 128;
-constexpr int kDetLog = 4;  // 16 lanes per query, one subtree each
+constexpr int kDetQ = 128;     // per-wave pair queue (< 64 pending + one append of <= 64)
+constexpr int kDetW = 16;      // queries per wave
+constexpr int kDetStack = 512; // frontier entries per wave
 
+// One candidate (query item qa, tree item qb): the classified distance within
+// thr keeps it as a contact block (contact_detection.py:120-214).
+template <bool kPT>
+__device__ __forceinline__ bool det_pair(const float4* __restrict__ x, const int4* __restrict__ tri,
+                                         const int2* __restrict__ edge, int qa, int qb, float thr, float ee_thres,
+                                         int4& ids) {
+    if (kPT) {
+        const int4 t = tri[qb];
+        if (qa == t.x || qa == t.y || qa == t.z) return false;
+        const V3 x0 = v3(x[qa]), x1 = v3(x[t.x]), x2 = v3(x[t.y]), x3 = v3(x[t.z]);
+        const int ty = pt_classify(x0, x1, x2, x3);
+        ids = make_int4(qa, t.x, t.y, t.z);
+        return pt_distance(x0, x1, x2, x3, ty) <= thr;
+    }
+    const int2 e0 = edge[qa], e1 = edge[qb];
+    if (e0.x == e1.x || e0.x == e1.y || e0.y == e1.x || e0.y == e1.y) return false;
+    const V3 x0 = v3(x[e0.x]), x1 = v3(x[e0.y]), x2 = v3(x[e1.x]), x3 = v3(x[e1.y]);
+    const int ty = ee_classify(x0, x1, x2, x3, ee_thres);
+    ids = make_int4(e0.x, e0.y, e1.x, e1.y);
+    return ee_distance(x0, x1, x2, x3, ty) <= thr;
+}
+
+// Stackless walk below internal node x (from its left child until x's
+// escape), classifying and appending in this lane.
+template <bool kPT>
+__device__ __noinline__ void det_walk_subtree(const float4* __restrict__ x, const int4* __restrict__ tri,
+                                              const int2* __restrict__ edge, const float4* __restrict__ node,
+                                              int n_leaves, int cur, int stop, V3 lo, V3 hi, int q, float thr,
+                                              float ee_thres, int* counter, int cap, int4* __restrict__ cidx,
+                                              int* __restrict__ ctype1) {
+    while (cur != stop) {
+        const float4 blo = node[2 * (long long)cur], bhi = node[2 * (long long)cur + 1];
+        const int lw = __float_as_int(blo.w), esc = __float_as_int(bhi.w);
+        if (box_hit(lo, hi, blo, bhi)) {
+            if (cur >= n_leaves - 1) {
+                int4 ids;
+                if ((kPT || lw > q) && det_pair<kPT>(x, tri, edge, q, lw, thr, ee_thres, ids)) {
+                    const int slot = atomicAdd(counter, 1);
+                    if (slot < cap) {
+                        cidx[slot] = ids;
+                        ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
+                    }
+                }
+                cur = esc;
+            } else {
+                cur = lw;
+            }
+        } else {
+            cur = esc;
+        }
+    }
+}
+
+// PT (query = vertex, tree over triangles) or EE (query = edge, tree over
+// edges, pairs taken from the larger edge index) candidates within
+// thr = 2 r + d_hat of the query's box; classified contacts are appended at
+// counter (contact_detection.py:120-214).
+//
+// A wave serves kDetW queries with one shared LIFO frontier of
+// (query, node) entries in LDS: each step pops up to 64 entries, one per
+// lane, pushes both children of overlapping internal nodes and queues
+// overlapping leaves; the queue is classified 64 pairs at a time, and output
+// slots are reserved with one atomic per wave.  When the frontier is full, a
+// lane walks the rest of its subtree by escape links.
 template <bool kPT>
 __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int nq, const int4* __restrict__ tri,
                                                const int2* __restrict__ edge, const float4* __restrict__ node,
-                                               int n_leaves, float d_hat, float radius, float ee_thres,
-                                               int* counter, int cap, int4* __restrict__ cidx,
-                                               int* __restrict__ ctype1) {
-    __shared__ int qa[kB / kWave][kDetQ], qb[kB / kWave][kDetQ];
+                                               const int* __restrict__ right, int n_leaves, float d_hat,
+                                               float radius, float ee_thres, int* counter, int cap,
+                                               int4* __restrict__ cidx, int* __restrict__ ctype1) {
+    constexpr int kW = kB / kWave;
+    __shared__ int2 stk[kW][kDetStack];
+    __shared__ int2 pq[kW][kDetQ];
+    __shared__ float4 qbox[kW][kDetW][2];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = (int)(tid >> kDetLog), sub = (int)(tid & ((1 << kDetLog) - 1));
-    if (__ballot(q < nq) == 0 || n_leaves <= 0) return;  // uniform per wave
-    int cur = -1, stop = -1;
-    if (q < nq) cur = subtree_start<kDetLog>(node, n_leaves, sub, &stop);
-    bool active = cur >= 0;
+    const long long q0 = ((long long)blockIdx.x * kW + w) * kDetW;
+    if (q0 >= nq || n_leaves <= 0) return;  // uniform per wave
+    const int nl = nq - q0 < kDetW ? (int)(nq - q0) : kDetW;
     const float thr = 2.0f * radius + d_hat;
-    V3 lo = mk(0.f, 0.f, 0.f), hi = lo;
-    if (active) {
+    if (lane < nl) {
+        const int q = (int)q0 + lane;
+        V3 lo, hi;
         if (kPT) {
             const V3 x0 = v3(x[q]);
             lo = x0 - mk(thr, thr, thr);
@@ -478,76 +544,85 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
             lo = vmin(a, b) - mk(thr, thr, thr);
             hi = vmax(a, b) + mk(thr, thr, thr);
         }
+        qbox[w][lane][0] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        qbox[w][lane][1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        stk[w][lane] = make_int2(lane, 0);  // root
     }
-    int qn = 0;
-    while (true) {
-        int found = -1;
-        if (active) {
+    wave_sync();
+    // emits the classified pairs of pq[w][qn - take, qn) with one atomic per wave
+    auto classify = [&](int qn, int take) {
+        bool keep = false;
+        int4 ids = make_int4(0, 0, 0, 0);
+        if (lane < take) {
+            const int2 e = pq[w][qn - take + lane];
+            keep = det_pair<kPT>(x, tri, edge, (int)q0 + e.x, e.y, thr, ee_thres, ids);
+        }
+        const uint64_t km = __ballot(keep);
+        int base = 0;
+        if (lane == 0 && km) base = atomicAdd(counter, (int)__popcll(km));
+        base = __shfl(base, 0);
+        if (keep) {
+            const int slot = base + lanes_below(km);
+            if (slot < cap) {
+                cidx[slot] = ids;
+                ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
+            }
+        }
+    };
+    int size = nl, qn = 0;  // wave-uniform
+    while (size > 0) {
+        const int take = size < kWave ? size : kWave;
+        int l = 0, cur = -1;
+        if (lane < take) {
+            const int2 e = stk[w][size - take + lane];
+            l = e.x;
+            cur = e.y;
+        }
+        size -= take;
+        int c0 = -1, c1 = -1, esc = -1, leaf = -1;
+        V3 lo = mk(0.f, 0.f, 0.f), hi = lo;
+        if (cur >= 0) {
             const float4 blo = node[2 * (long long)cur], bhi = node[2 * (long long)cur + 1];
-            const int lw = __float_as_int(blo.w), esc = __float_as_int(bhi.w);
+            const int r = cur < n_leaves - 1 ? right[cur] : -1;
+            const float4 ql = qbox[w][l][0], qh = qbox[w][l][1];
+            lo = mk(ql.x, ql.y, ql.z);
+            hi = mk(qh.x, qh.y, qh.z);
             if (box_hit(lo, hi, blo, bhi)) {
+                const int lw = __float_as_int(blo.w);
                 if (cur >= n_leaves - 1) {
-                    if (kPT || lw > q) found = lw;
-                    cur = esc;
+                    if (kPT || lw > (int)q0 + l) leaf = lw;
                 } else {
-                    cur = lw;
+                    c0 = lw;
+                    c1 = r;
+                    esc = __float_as_int(bhi.w);
                 }
+            }
+        }
+        wave_sync();  // every pop has read its entry before pushes reuse the slots
+        const uint64_t lm = __ballot(leaf >= 0);
+        if (leaf >= 0) pq[w][qn + lanes_below(lm)] = make_int2(l, leaf);
+        qn += (int)__popcll(lm);
+        const uint64_t pm = __ballot(c0 >= 0);
+        const int below = lanes_below(pm);
+        const bool fits = size + 2 * (below + 1) <= kDetStack;  // pushes that fit form a prefix
+        if (c0 >= 0) {
+            if (fits) {
+                stk[w][size + 2 * below] = make_int2(l, c0);
+                stk[w][size + 2 * below + 1] = make_int2(l, c1);
             } else {
-                cur = esc;
+                det_walk_subtree<kPT>(x, tri, edge, node, n_leaves, c0, esc, lo, hi, (int)q0 + l, thr, ee_thres,
+                                      counter, cap, cidx, ctype1);
             }
-            if (cur == stop) active = false;
         }
-        const uint64_t fm = __ballot(found >= 0);
-        if (found >= 0) {
-            const int slot = qn + lanes_below(fm);
-            qa[w][slot] = q;
-            qb[w][slot] = found;
-        }
-        qn += (int)__popcll(fm);
-        const bool any_active = __ballot(active) != 0;
-        if (qn >= kWave || (!any_active && qn > 0)) {
-            wave_sync();
-            while (qn >= kWave || (!any_active && qn > 0)) {
-                const int take = qn < kWave ? qn : kWave;
-                bool keep = false;
-                int4 ids = make_int4(0, 0, 0, 0);
-                if (lane < take) {
-                    const int qa_ = qa[w][qn - take + lane], qb_ = qb[w][qn - take + lane];
-                    if (kPT) {
-                        const int4 t = tri[qb_];
-                        if (qa_ != t.x && qa_ != t.y && qa_ != t.z) {
-                            const V3 x0 = v3(x[qa_]), x1 = v3(x[t.x]), x2 = v3(x[t.y]), x3 = v3(x[t.z]);
-                            const int ty = pt_classify(x0, x1, x2, x3);
-                            keep = pt_distance(x0, x1, x2, x3, ty) <= thr;
-                            ids = make_int4(qa_, t.x, t.y, t.z);
-                        }
-                    } else {
-                        const int2 e0 = edge[qa_], e1 = edge[qb_];
-                        if (e0.x != e1.x && e0.x != e1.y && e0.y != e1.x && e0.y != e1.y) {
-                            const V3 x0 = v3(x[e0.x]), x1 = v3(x[e0.y]), x2 = v3(x[e1.x]), x3 = v3(x[e1.y]);
-                            const int ty = ee_classify(x0, x1, x2, x3, ee_thres);
-                            keep = ee_distance(x0, x1, x2, x3, ty) <= thr;
-                            ids = make_int4(e0.x, e0.y, e1.x, e1.y);
-                        }
-                    }
-                }
-                const uint64_t km = __ballot(keep);
-                int base = 0;
-                if (lane == 0 && km) base = atomicAdd(counter, (int)__popcll(km));
-                base = __shfl(base, 0);
-                if (keep) {
-                    const int slot = base + lanes_below(km);
-                    if (slot < cap) {
-                        cidx[slot] = ids;
-                        ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
-                    }
-                }
-                qn -= take;
-            }
+        size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));
+        wave_sync();
+        if (qn >= kWave) {
+            classify(qn, kWave);
+            qn -= kWave;
             wave_sync();
         }
-        if (!any_active) break;
     }
+    if (qn > 0) classify(qn, qn);
 }
 
 // ------------------------------------------------------------ energies (B7)
@@ -1418,12 +1493,14 @@ static int detect_contacts(Ctx& c, int& nC) {
         while (true) {
             PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
             const int cap = std::min(c.capC, c.P.max_blocks);
-            L3(detect_k<true>, (long long)c.nV << kDetLog, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
-               (const float4*)c.tri_tree.box, c.nF, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
-               c.ctype1);
-            L3(detect_k<false>, (long long)c.nE << kDetLog, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
-               (const float4*)c.edge_tree.box, c.nE, c.P.d_hat, c.radius, c.P.ee_thres, c.ccount, cap, c.cidx,
-               c.ctype1);
+            const long long pt_threads = ((long long)c.nV + kDetW - 1) / kDetW * kWave;
+            const long long ee_threads = ((long long)c.nE + kDetW - 1) / kDetW * kWave;
+            L3(detect_k<true>, pt_threads, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
+               (const float4*)c.tri_tree.box, (const int*)c.tri_tree.right, c.nF, c.P.d_hat, c.radius, c.P.ee_thres,
+               c.ccount, cap, c.cidx, c.ctype1);
+            L3(detect_k<false>, ee_threads, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
+               (const float4*)c.edge_tree.box, (const int*)c.edge_tree.right, c.nE, c.P.d_hat, c.radius,
+               c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1);
             PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, sizeof(int), hipMemcpyDeviceToHost, c.stream));
             PAMO_HIP(hipStreamSynchronize(c.stream));
             const int n = c.hcount[0];
