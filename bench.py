@@ -247,6 +247,15 @@ def main():
                 "avg_launch_us": round(1e3 * dom["total_ms"] / max(dom["launches"], 1), 2),
                 "launches": dom["launches"]}
 
+    # output quality, untimed: igl-semantics Hausdorff and sampled Chamfer
+    # distances of the output to the input surface (GPU metrics)
+    from pamo_amd import metrics
+    tq = time.perf_counter()
+    cd, hd = metrics.compute_igl_CD_HD(V, F, vo, fo)
+    quality = {"hausdorff": float(hd), "chamfer": float(cd), "bbox_diag": float(np.linalg.norm(V.max(0) - V.min(0))),
+               "metric_s": round(time.perf_counter() - tq, 3)}
+    print(f"[bench] quality: Hausdorff {hd:.4e}, Chamfer {cd:.4e}", file=sys.stderr, flush=True)
+
     # CPU baseline: the oracle on the first rounds of the same workload,
     # scaled to the whole job by faces processed.
     cpu = None
@@ -299,6 +308,7 @@ def main():
         "face_rounds": face_rounds,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "quality": quality,
         "kernels": stats,
     }
     if args.calib_copy_mb > 0:

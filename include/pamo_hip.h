@@ -181,6 +181,12 @@ int pamo_s3_set_mesh_device(pamo_s3* s3, const float* d_v, int n_v, const int* d
  * targets, then n_newton_iters of {contacts, gradient, Jacobi PCG, clamp,
  * ACCD, halving line search}. */
 int pamo_s3_step(pamo_s3* s3, int n_steps);
+/* Replaces Stage3System.step(eval=True, ...) (system.py:411-470): one step,
+ * calling on_newton(newton_iter, user) after each Newton iteration once its
+ * device work has finished, so the caller can read the vertices and
+ * evaluate them (system.py:466-470). */
+typedef void (*pamo_s3_newton_fn)(int newton_iter, void* user);
+int pamo_s3_step_eval(pamo_s3* s3, pamo_s3_newton_fn on_newton, void* user);
 
 /* Current positions mapped back to the input frame, (x' - t) / scale, into a
  * device float32 (n_v,3) buffer (processing.py:96-104). */
@@ -260,6 +266,30 @@ int pamo_serve_poll(pamo_server* srv, long long job_id, int wait, int* state, in
 /* Copies a finished job's output (n_verts_out x 3 f32, n_faces_out x 3 i32)
  * to device buffers and releases the job. */
 int pamo_serve_fetch(pamo_server* srv, long long job_id, float* d_out_verts, int* d_out_faces);
+
+/* ---------------------------------------------------------------- surface metrics
+ * Point-to-mesh distances and the Hausdorff distance with the semantics of
+ * the libigl calls in simp_cuda/safe_project/src/pamo_safe_project/metrics.py
+ * (libigl 2.5.1), in FP64 on the device.  All arrays are device pointers:
+ * points and vertices (n, 3) f64, faces (n, 3) i32.  A metrics context owns
+ * one HIP stream and its tree buffers; calls return when the results are
+ * written.  Faces indexing outside the vertices are rejected. */
+typedef struct pamo_metrics pamo_metrics;
+int pamo_metrics_create(pamo_metrics** out, int device);
+int pamo_metrics_destroy(pamo_metrics* m);
+
+/* Replaces igl.point_mesh_squared_distance(P, V, F) -> (sqrD, I, C)
+ * (metrics.py:41-42): squared distance of each point to the nearest face,
+ * that face's index (ties: smaller index) and the closest point.
+ * face_index and closest may be NULL. */
+int pamo_point_mesh_sqdist(pamo_metrics* m, const double* points, long long n_points, const double* verts, int n_v,
+                           const int* faces, int n_f, double* sqr_d, int* face_index, double* closest);
+
+/* Replaces igl.hausdorff(VA, FA, VB, FB) (metrics.py:49): the larger of the
+ * largest distance of a vertex of A to the surface of B and of a vertex of B
+ * to the surface of A, written to host *d_out. */
+int pamo_hausdorff(pamo_metrics* m, const double* va, int n_va, const int* fa, int n_fa, const double* vb, int n_vb,
+                   const int* fb, int n_fb, double* d_out);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,7 @@ SIGNATURES = [
     ("pamo_s3_set_mesh", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double), c_vp, c_int,
                                  c_vp, c_int, c_vp, c_int]),
     ("pamo_s3_step", c_int, [c_vp, c_int]),
+    ("pamo_s3_step_eval", c_int, [c_vp, c_vp, c_vp]),
     ("pamo_s3_get_vertices", c_int, [c_vp, c_vp, c_double, ctypes.POINTER(c_double)]),
     ("pamo_s3_get_normalized", c_int, [c_vp, c_vp]),
     ("pamo_s3_set_dense", c_int, [c_vp, c_vp, c_vp, c_double, c_int]),
@@ -102,10 +103,17 @@ SIGNATURES = [
     ("pamo_serve_poll", c_int, [c_vp, c_ll, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                 ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     ("pamo_serve_fetch", c_int, [c_vp, c_ll, c_vp, c_vp]),
+    ("pamo_metrics_create", c_int, [ctypes.POINTER(c_vp), c_int]),
+    ("pamo_metrics_destroy", c_int, [c_vp]),
+    ("pamo_point_mesh_sqdist", c_int, [c_vp, c_vp, c_ll, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
+    ("pamo_hausdorff", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_int,
+                               ctypes.POINTER(ctypes.c_double)]),
 ]
 
 # pamo_batch_run's per-mesh completion callback: (mesh, status, user)
 BATCH_DONE_FN = ctypes.CFUNCTYPE(None, c_int, c_int, c_vp)
+# pamo_s3_step_eval's per-Newton-iteration callback: (newton_iter, user)
+S3_NEWTON_FN = ctypes.CFUNCTYPE(None, c_int, c_vp)
 
 _lib = None
 

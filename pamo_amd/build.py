@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-HIP_SOURCES = ["stage2.hip", "stage3.hip", "serve.hip"]
+HIP_SOURCES = ["stage2.hip", "stage3.hip", "serve.hip", "metrics.hip"]
 
 
 def _run(cmd):

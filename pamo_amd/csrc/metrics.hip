@@ -1,0 +1,321 @@
+// Surface metrics on gfx950 behind pamo_metrics_* of include/pamo_hip.h:
+// point-to-mesh squared distances and the two-sided Hausdorff distance
+// between triangle meshes, with the semantics of the libigl calls the
+// reference makes (simp_cuda/safe_project/src/pamo_safe_project/metrics.py:31-51):
+//   igl.point_mesh_squared_distance(P, V, F) -> (sqrD, I, C)
+//   igl.hausdorff(VA, FA, VB, FB) = sqrt(max(max_{a in VA} d(a, B)^2,
+//                                            max_{b in VB} d(b, A)^2))
+// (libigl 2.5.1 hausdorff: vertex-to-surface both ways).
+//
+// Distances are FP64.  A point's squared distance to a triangle is the one of
+// Ericson's closest point (Real-Time Collision Detection 5.1.5, the region
+// test libigl's point_simplex_squared_distance uses), evaluated in double
+// without contraction.  The nearest triangle is found by branch and bound
+// over an LBVH of float boxes rounded outward from the double corners: a
+// subtree is skipped only when its box is farther than the best distance so
+// far, so the result is the minimum over all triangles up to rounding at
+// exact ties, with ties broken by the smaller face index.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "pamo_hip.h"
+#include "lbvh_tree.h"
+
+namespace pamo {
+namespace metrics {
+
+using s3::kB;
+using s3::Tree;
+using s3::TreeView;
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 load3(const double* __restrict__ p, long long i) {
+    return D3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+}
+
+// Ericson 5.1.5: closest point of triangle abc to p by Voronoi regions.
+__device__ inline D3 closest_on_tri_d(D3 p, D3 a, D3 b, D3 c) {
+    const D3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    const D3 bp = p - b;
+    const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return a + (d1 / (d1 - d3)) * ab;
+    const D3 cp = p - c;
+    const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return a + (d2 / (d2 - d6)) * ac;
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return b + ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b);
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    return a + v * ab + w * ac;
+}
+
+// Largest float <= x and smallest float >= x.
+__device__ __forceinline__ float f_down(double x) {
+    float f = (float)x;
+    return (double)f > x ? nextafterf(f, -__builtin_huge_valf()) : f;
+}
+__device__ __forceinline__ float f_up(double x) {
+    float f = (float)x;
+    return (double)f < x ? nextafterf(f, __builtin_huge_valf()) : f;
+}
+
+// Number of faces with a corner outside [0, nV).
+__global__ void __launch_bounds__(kB) check_faces_k(const int* __restrict__ F, int nF, int nV, int* bad) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const int a = F[3 * (long long)f], b = F[3 * (long long)f + 1], c = F[3 * (long long)f + 2];
+    if (a < 0 || a >= nV || b < 0 || b >= nV || c < 0 || c >= nV) atomicAdd(bad, 1);
+}
+
+// Float box of each face, rounded outward from its double corners.
+__global__ void __launch_bounds__(kB) face_box_d_k(const double* __restrict__ V, const int* __restrict__ F, int nF,
+                                                  float4* __restrict__ item) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const D3 a = load3(V, F[3 * (long long)f]), b = load3(V, F[3 * (long long)f + 1]),
+             c = load3(V, F[3 * (long long)f + 2]);
+    item[2 * (long long)f] = make_float4(f_down(fmin(fmin(a.x, b.x), c.x)), f_down(fmin(fmin(a.y, b.y), c.y)),
+                                         f_down(fmin(fmin(a.z, b.z), c.z)), 0.0f);
+    item[2 * (long long)f + 1] = make_float4(f_up(fmax(fmax(a.x, b.x), c.x)), f_up(fmax(fmax(a.y, b.y), c.y)),
+                                             f_up(fmax(fmax(a.z, b.z), c.z)), 0.0f);
+}
+
+__device__ __forceinline__ double box_d2(D3 p, float4 lo, float4 hi) {
+    const double dx = fmax(fmax((double)lo.x - p.x, 0.0), p.x - (double)hi.x);
+    const double dy = fmax(fmax((double)lo.y - p.y, 0.0), p.y - (double)hi.y);
+    const double dz = fmax(fmax((double)lo.z - p.z, 0.0), p.z - (double)hi.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Nearest face of each point: ordered stackless descent (nearer child
+// first, parent links to climb), subtrees farther than the best skipped.
+// Writes sqd[i] and, when given, I[i] and the closest point C[i].
+__global__ void __launch_bounds__(kB) point_mesh_k(const double* __restrict__ P, long long nP, TreeView T,
+                                                  const double* __restrict__ V, const int* __restrict__ F,
+                                                  double* __restrict__ sqd, int* __restrict__ I,
+                                                  double* __restrict__ C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nP) return;
+    const D3 p = load3(P, i);
+    double best = __builtin_huge_val();
+    int best_f = -1;
+    D3 best_c = D3{0.0, 0.0, 0.0};
+    const int n = T.n;
+    auto leaf = [&](int f) {
+        const D3 c = closest_on_tri_d(p, load3(V, F[3 * (long long)f]), load3(V, F[3 * (long long)f + 1]),
+                                      load3(V, F[3 * (long long)f + 2]));
+        const D3 d = c - p;
+        const double d2 = ddot(d, d);
+        if (d2 < best || (d2 == best && f < best_f)) {
+            best = d2;
+            best_f = f;
+            best_c = c;
+        }
+    };
+    auto bd = [&](int node) { return box_d2(p, T.box[2 * (long long)node], T.box[2 * (long long)node + 1]); };
+    if (n == 1) {
+        leaf((int)T.val[0]);
+    } else if (n > 1) {
+        int cur = 0, prev = -1;
+        while (cur >= 0) {
+            const int par = cur == 0 ? -1 : T.parent[cur];
+            int next;
+            if (prev == par) {
+                if (bd(cur) > best) {
+                    next = par;
+                } else if (cur >= n - 1) {
+                    leaf((int)T.val[cur - (n - 1)]);
+                    next = par;
+                } else {
+                    const int l = T.left[cur], r = T.right[cur];
+                    next = bd(l) <= bd(r) ? l : r;
+                }
+            } else {
+                const int l = T.left[cur], r = T.right[cur];
+                const bool left_first = bd(l) <= bd(r);
+                const int first = left_first ? l : r, second = left_first ? r : l;
+                next = prev == first ? second : par;
+            }
+            prev = cur;
+            cur = next;
+        }
+    }
+    sqd[i] = best;
+    if (I) I[i] = best_f;
+    if (C) {
+        C[3 * i] = best_c.x;
+        C[3 * i + 1] = best_c.y;
+        C[3 * i + 2] = best_c.z;
+    }
+}
+
+// Maximum of non-negative doubles (their bit patterns order like the values;
+// a NaN orders above every number and propagates).
+__global__ void __launch_bounds__(kB) max_bits_k(const double* __restrict__ x, long long n,
+                                                unsigned long long* out) {
+    __shared__ unsigned long long red[kB];
+    unsigned long long m = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        m = max(m, (unsigned long long)__double_as_longlong(x[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Tree tree;
+    int* ws = nullptr;
+    long long cap_ws = 0;
+    double* sqd = nullptr;  // Hausdorff: per-vertex squared distances
+    long long cap_sqd = 0;
+    int* bad = nullptr;                 // [1]
+    unsigned long long* mx = nullptr;   // [1]
+    int* h_bad = nullptr;               // pinned
+    unsigned long long* h_mx = nullptr; // pinned
+};
+
+static void ctx_free(Ctx& c) {
+    s3::tree_free(c.tree);
+    void* ptrs[] = {c.ws, c.sqd, c.bad, c.mx};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c.h_bad) (void)hipHostFree(c.h_bad);
+    if (c.h_mx) (void)hipHostFree(c.h_mx);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+// Builds the tree over the faces of (V, F) after checking every index.
+static int build_tree(Ctx& c, const double* V, int nV, const int* F, int nF, const char* who) {
+    PAMO_CHECK(V && F && nV > 0 && nF > 0, std::string(who) + ": empty or null mesh");
+    PAMO_HIP(hipMemsetAsync(c.bad, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(check_faces_k, dim3(grid_for(nF, kB)), dim3(kB), 0, c.stream, F, nF, nV, c.bad);
+    PAMO_HIP(hipGetLastError());
+    PAMO_HIP(hipMemcpyAsync(c.h_bad, c.bad, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    PAMO_CHECK(*c.h_bad == 0, std::string(who) + ": " + std::to_string(*c.h_bad) + " faces index outside the vertices");
+    const long long need = std::max(radix_workspace(nF + 64LL), scan_workspace(nF + 65LL)) + 256;
+    if (need > c.cap_ws) {
+        c.cap_ws = need + need / 4;
+        PAMO_TRY(s3::dalloc(&c.ws, (size_t)c.cap_ws, c.stream));
+    }
+    PAMO_TRY(s3::tree_alloc(c.tree, nF, c.stream));
+    hipLaunchKernelGGL(face_box_d_k, dim3(grid_for(nF, kB)), dim3(kB), 0, c.stream, V, F, nF, c.tree.item);
+    PAMO_HIP(hipGetLastError());
+    return s3::tree_build(c.tree, nF, c.ws, c.stream);
+}
+
+static int query(Ctx& c, const double* P, long long nP, const double* V, const int* F, double* sqd, int* I,
+                 double* C) {
+    if (nP <= 0) return 0;
+    hipLaunchKernelGGL(point_mesh_k, dim3(grid_for(nP, kB)), dim3(kB), 0, c.stream, P, nP, s3::view(c.tree), V, F,
+                       sqd, I, C);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+// max over sqd of the one-sided distances of (VA -> B), written to c.mx
+static int one_side(Ctx& c, const double* VA, int nVA, const double* VB, int nVB, const int* FB, int nFB) {
+    PAMO_TRY(build_tree(c, VB, nVB, FB, nFB, "pamo_hausdorff"));
+    if (nVA > c.cap_sqd) {
+        c.cap_sqd = nVA + nVA / 4;
+        PAMO_TRY(s3::dalloc(&c.sqd, (size_t)c.cap_sqd, c.stream));
+    }
+    PAMO_TRY(query(c, VA, nVA, VB, FB, c.sqd, nullptr, nullptr));
+    const unsigned nb = std::min<unsigned>(grid_for(nVA, kB), 1024u);
+    hipLaunchKernelGGL(max_bits_k, dim3(nb), dim3(kB), 0, c.stream, (const double*)c.sqd, (long long)nVA, c.mx);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace metrics
+}  // namespace pamo
+
+using MCtx = pamo::metrics::Ctx;
+
+extern "C" {
+
+int pamo_metrics_create(pamo_metrics** out, int device) {
+    pamo::clear_error();
+    PAMO_CHECK(out != nullptr, "pamo_metrics_create: null argument");
+    MCtx* c = new MCtx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->bad, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->mx, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_bad, sizeof(int), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_mx, sizeof(unsigned long long), 0);
+    if (e != hipSuccess) {
+        pamo::set_error(std::string("pamo_metrics_create: ") + hipGetErrorString(e));
+        pamo::metrics::ctx_free(*c);
+        delete c;
+        return 1;
+    }
+    *out = reinterpret_cast<pamo_metrics*>(c);
+    return 0;
+}
+
+int pamo_metrics_destroy(pamo_metrics* m) {
+    pamo::clear_error();
+    if (!m) return 0;
+    MCtx* c = reinterpret_cast<MCtx*>(m);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    pamo::metrics::ctx_free(*c);
+    delete c;
+    return 0;
+}
+
+int pamo_point_mesh_sqdist(pamo_metrics* m, const double* P, long long n_points, const double* V, int n_v,
+                           const int* F, int n_f, double* sqr_d, int* face_index, double* closest) {
+    pamo::clear_error();
+    PAMO_CHECK(m != nullptr, "pamo_point_mesh_sqdist: null context");
+    PAMO_CHECK(n_points >= 0 && (n_points == 0 || (P && sqr_d)), "pamo_point_mesh_sqdist: null argument");
+    MCtx& c = *reinterpret_cast<MCtx*>(m);
+    PAMO_HIP(hipSetDevice(c.device));
+    PAMO_TRY(pamo::metrics::build_tree(c, V, n_v, F, n_f, "pamo_point_mesh_sqdist"));
+    PAMO_TRY(pamo::metrics::query(c, P, n_points, V, F, sqr_d, face_index, closest));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int pamo_hausdorff(pamo_metrics* m, const double* VA, int n_va, const int* FA, int n_fa, const double* VB, int n_vb,
+                   const int* FB, int n_fb, double* d_out) {
+    pamo::clear_error();
+    PAMO_CHECK(m != nullptr && d_out != nullptr, "pamo_hausdorff: null argument");
+    MCtx& c = *reinterpret_cast<MCtx*>(m);
+    PAMO_HIP(hipSetDevice(c.device));
+    PAMO_HIP(hipMemsetAsync(c.mx, 0, sizeof(unsigned long long), c.stream));
+    // vertices of B against the surface of A, then vertices of A against B
+    PAMO_TRY(pamo::metrics::one_side(c, VB, n_vb, VA, n_va, FA, n_fa));
+    PAMO_TRY(pamo::metrics::one_side(c, VA, n_va, VB, n_vb, FB, n_fb));
+    PAMO_HIP(hipMemcpyAsync(c.h_mx, c.mx, sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    double d2;
+    std::memcpy(&d2, c.h_mx, sizeof(double));
+    *d_out = std::sqrt(d2);
+    return 0;
+}
+
+}  // extern "C"

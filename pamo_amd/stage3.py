@@ -10,11 +10,14 @@ trimesh / igl; all per-iteration work runs in the library.
 from __future__ import annotations
 
 import ctypes
+import logging
 
 import numpy as np
 import torch
 
 from . import _hip
+
+logger = logging.getLogger(__name__)
 
 _ENERGY_BITS = {
     "Mesh2GTDistanceBvhEnergyCalculator": _hip.PAMO_S3_E_M2GT,
@@ -266,9 +269,31 @@ class Stage3System:
         self.set_target(V_gt, F_gt, 1.0, np.zeros(3))
         self.set_mesh(V, F, 1.0, np.zeros(3))
 
-    def step(self, n: int = 1):
-        """Stage3System.step (system.py:411-470), n times."""
-        _hip.check(_hip.lib().pamo_s3_step(self.ptr, int(n)), "pamo_s3_step")
+    def step(self, n: int = 1, eval=False, eval_CDs=None, eval_HDs=None, gt_V=None, gt_F=None, eval_F=None):
+        """Stage3System.step (system.py:411-470), n times.  With eval=True
+        the Chamfer and Hausdorff distances of the current vertices (with
+        faces eval_F) to (gt_V, gt_F), all normalised, are appended to
+        eval_CDs / eval_HDs after every Newton iteration (system.py:466-470),
+        computed by pamo_amd.metrics.compute_igl_CD_HD."""
+        if not eval:
+            _hip.check(_hip.lib().pamo_s3_step(self.ptr, int(n)), "pamo_s3_step")
+            return
+        from .metrics import compute_igl_CD_HD
+        failed = []
+
+        def on_newton(it, user):
+            try:
+                cd, hd = compute_igl_CD_HD(gt_V, gt_F, self.get_vertices(), eval_F)
+                eval_CDs.append(cd)
+                eval_HDs.append(hd)
+            except Exception as e:  # raised after the step returns
+                failed.append(e)
+
+        cb = _hip.S3_NEWTON_FN(on_newton)
+        for _ in range(int(n)):
+            _hip.check(_hip.lib().pamo_s3_step_eval(self.ptr, cb, None), "pamo_s3_step_eval")
+            if failed:
+                raise failed[0]
 
     def get_vertices(self) -> np.ndarray:
         """Normalised positions (float32 values as float64), system.py:164-165."""
@@ -320,10 +345,13 @@ def normalization_transform_device(system: "Stage3System", V: torch.Tensor):
 
 
 def process(gt_V, gt_F, stage2_V, stage2_F, n_iters, system: Stage3System | None = None,
-            config: Stage3Config | None = None):
+            config: Stage3Config | None = None, eval=False, return_curve=False):
     """processing.py:23-108: normalise by the input surface, n_iters steps,
     map back.  Inputs may be numpy arrays or CUDA tensors (kept on the
-    device).  Returns (float64 vertices, faces) as numpy."""
+    device).  Returns (float64 vertices, faces) as numpy, and with
+    return_curve also the Chamfer / Hausdorff distances after every Newton
+    iteration (normalised frame).  eval logs the distances before the first
+    step and after each step (processing.py:66-97)."""
     if system is None:
         system = Stage3System(config if config is not None else Stage3Config())
     else:
@@ -334,7 +362,33 @@ def process(gt_V, gt_F, stage2_V, stage2_F, n_iters, system: Stage3System | None
         scale, t = normalization_transform(np.asarray(gt_V))
     system.set_target(gt_V, gt_F, scale, t)
     system.set_mesh(stage2_V, stage2_F, scale, t)
-    system.step(int(n_iters))
-    V = (system.get_vertices() - t) / scale
     F = stage2_F.cpu().numpy() if isinstance(stage2_F, torch.Tensor) else np.asarray(stage2_F)
+    if not (eval or return_curve):
+        system.step(int(n_iters))
+        V = (system.get_vertices() - t) / scale
+        return V, F
+    from .metrics import compute_igl_CD_HD
+    gt_n = _host64(gt_V) * scale + t
+    gt_Fn = _host64(gt_F).astype(np.int64)
+    if eval:
+        cd, hd = compute_igl_CD_HD(gt_n, gt_Fn, system.get_vertices(), F)
+        logger.info(f"Initial CD: {cd:.3e}, HD: {hd:.3e}")
+    CDs, HDs = [], []
+    for i in range(int(n_iters)):
+        if return_curve:
+            system.step(1, eval=True, eval_CDs=CDs, eval_HDs=HDs, gt_V=gt_n, gt_F=gt_Fn, eval_F=F)
+        else:
+            system.step(1)
+        if eval:
+            cd, hd = compute_igl_CD_HD(gt_n, gt_Fn, system.get_vertices(), F)
+            logger.info(f"Iter {i}: CD: {cd:.3e}, HD: {hd:.3e}")
+    V = (system.get_vertices() - t) / scale
+    if return_curve:
+        return V, F, CDs, HDs
     return V, F
+
+
+def _host64(a):
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    return np.asarray(a, dtype=np.float64)
