@@ -41,7 +41,9 @@ def parse():
                    help="exact intersection guard instead of the reference's predicate (results differ from the reference)")
     p.add_argument("--iters", type=int, default=100000,
                    help="cap on stage-2 rounds (the reference's iter; the workload uses 100000)")
-    p.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
+    p.add_argument("--stress-iters", type=int, default=400,
+                   help="stress: cap on stage-2 rounds (the sheets crawl under the guard; uncapped takes minutes)")
+    p.add_argument("--workload", choices=["headline", "batch", "serve", "stress"], default="headline",
                    help="headline: one 2M-face mesh; batch: many meshes, batched vs plain; serve: a request stream")
     p.add_argument("--batch-n", type=int, default=96, help="batch: number of meshes")
     p.add_argument("--slots", type=int, default=8, help="batch / serve: concurrent slots")
@@ -173,6 +175,70 @@ def bench_serve(args):
     print(json.dumps(res))
 
 
+def bench_stress(args):
+    """Intersection stress: two closely separated sheets (about 8M faces,
+    ratio 0.0025, tolerance 8, stages 2+3), one mesh in the plain form and
+    four (seeds 500..503) batched against the same four run one by one.
+    Stage 2 is capped at --stress-iters rounds: the guard undoes most
+    collapses between sheets this close, so the loop reaches the face target
+    only after tens of thousands of rounds."""
+    import numpy as np
+    import torch
+    from pamo_amd import synth
+    from pamo_amd.batch import pack_csr, simplify_batch
+    from pamo_amd.pamo import PaMO
+
+    ratio, tol, it = 0.0025, 8, args.stress_iters
+    s3 = bool(args.stage3)
+    meshes = []
+    for seed in (500, 501, 502, 503):
+        V, F, res = synth.stress_sheets(seed, 8e6)
+        meshes.append((V, F))
+        print(f"[bench] stress sheets seed {seed}: grid {res}, {len(F)} faces", file=sys.stderr, flush=True)
+    dev = [(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()) for V, F in meshes]
+
+    def plain(k):
+        pm = PaMO(meshes[k], use_stage1=False, use_stage3=s3, exact_guard=args.exact_guard)
+        out = pm.run(dev[k][0], dev[k][1], ratio, tol, 1e-3, it, 0)
+        return out, pm
+
+    plain(0)  # warm-up
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    (vo, fo), pm = plain(0)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    print(f"[bench] single: {t_single:.2f}s, {len(fo)} faces", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    outs = [plain(k)[0] for k in range(4)]
+    torch.cuda.synchronize()
+    t_plain4 = time.perf_counter() - t0
+    print(f"[bench] plain x4: {t_plain4:.2f}s", file=sys.stderr, flush=True)
+    packed = pack_csr(meshes)
+    t0 = time.perf_counter()
+    bout, _ = simplify_batch(None, ratio, s3, n_slots=4, tolerance=tol, iters=it, packed=packed,
+                             exact_guard=args.exact_guard)
+    torch.cuda.synchronize()
+    t_batch4 = time.perf_counter() - t0
+    print(f"[bench] batched x4: {t_batch4:.2f}s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a[1].astype(np.int64), b[1].astype(np.int64)) for a, b in zip(outs, bout))
+    nF = len(meshes[0][1])
+    res = {"metric": "stress sheets wall-time (stages 2+3)" if s3 else "stress sheets wall-time (stage 2)",
+           "value": round(t_single, 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(t_single * 1e3, 2), "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "two level-set sheets of the ellipsoid/box union (level 0 and -1e-3), seed 500",
+                      "faces_in": nF, "faces_out": int(len(fo)), "ratio": ratio, "tolerance": tol,
+                      "stage2_round_cap": it, "rounds": int(pm.rounds), "stage3": s3,
+                      "guard": "exact" if args.exact_guard else "reference predicate"},
+           "stage2_s": round(pm.timings["stage2_s"], 4), "stage3_s": round(pm.timings.get("stage3_s", 0.0), 4),
+           "batched_x4_s": round(t_batch4, 3), "plain_x4_s": round(t_plain4, 3),
+           "batched_x4_meshes_per_s": round(4 / t_batch4, 3), "plain_x4_meshes_per_s": round(4 / t_plain4, 3),
+           "batched_x4_faces_identical_to_plain": same}
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -183,6 +249,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "stress":
+        return bench_stress(args)
     import numpy as np
     import torch
     from pamo_amd import synth, _hip

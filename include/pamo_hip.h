@@ -238,23 +238,25 @@ int pamo_s3_stats(pamo_s3* s3, double* out8);
  * host per-mesh ratio and use_stage3.  Outputs go to d_out_verts /
  * d_out_faces at the input offsets (capacity = input sizes), counts to host
  * out_nv / out_nf; *seconds receives the wall time.  s3 may be NULL when no
- * mesh uses stage 3.  Meshes are started longest first.  on_done (may be
+ * mesh uses stage 3.  exact_guard is the PAMO_OPT_EXACT_GUARD value of every
+ * slot's stage-2 context.  Meshes are started longest first.  on_done (may be
  * NULL) is called from a worker thread as each mesh finishes, with the
  * mesh's index, its status (0 = ok) and user. */
 typedef void (*pamo_batch_done_fn)(int mesh, int status, void* user);
 int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_meshes, const float* d_verts,
                    const int* d_faces, const long long* vert_off, const long long* face_off, const double* ratio,
                    const int* use_stage3, int tolerance, float threshold, long long iters, long long min_verts,
-                   float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf, double* seconds,
-                   pamo_batch_done_fn on_done, void* user);
+                   int exact_guard, float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf,
+                   double* seconds, pamo_batch_done_fn on_done, void* user);
 
 /* Request loop: jobs are admitted in arrival order when a slot is free and
  * the estimated HBM footprint of the running jobs plus the new one fits
  * hbm_budget_bytes (<= 0: 200 GiB); a job larger than the budget runs alone.
- * The caller keeps a job's inputs alive until it is done. */
+ * exact_guard is the PAMO_OPT_EXACT_GUARD value of every slot.  The caller
+ * keeps a job's inputs alive until it is done. */
 typedef struct pamo_server pamo_server;
 int pamo_serve_create(pamo_server** out, int device, int n_slots, long long hbm_budget_bytes,
-                      const pamo_s3_params* s3);
+                      const pamo_s3_params* s3, int exact_guard);
 int pamo_serve_destroy(pamo_server* srv);
 int pamo_serve_submit(pamo_server* srv, const float* d_verts, int n_verts, const int* d_faces, int n_faces,
                       double ratio, int use_stage3, int tolerance, float threshold, long long iters,

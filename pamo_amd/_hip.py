@@ -94,9 +94,9 @@ SIGNATURES = [
     ("pamo_s3_normalization", c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     ("pamo_s3_set_mesh_device", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double)]),
     ("pamo_batch_run", c_int, [c_int, c_int, ctypes.POINTER(S3Params), c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                               c_int, c_float, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_double),
-                               c_vp, c_vp]),
-    ("pamo_serve_create", c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_ll, ctypes.POINTER(S3Params)]),
+                               c_int, c_float, c_ll, c_ll, c_int, c_vp, c_vp, c_vp, c_vp,
+                               ctypes.POINTER(c_double), c_vp, c_vp]),
+    ("pamo_serve_create", c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_ll, ctypes.POINTER(S3Params), c_int]),
     ("pamo_serve_destroy", c_int, [c_vp]),
     ("pamo_serve_submit", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_int, c_float, c_ll, c_ll,
                                   ctypes.POINTER(c_ll)]),

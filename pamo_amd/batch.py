@@ -39,14 +39,15 @@ def pack_csr(meshes):
 
 def simplify_batch(meshes, ratio, use_stage3=False, n_slots: int = 4, tolerance: int = 4, threshold: float = 1e-3,
                    iters: int = 100000, min_verts: int = 0, config: Stage3Config | None = None, packed=None,
-                   on_done=None):
+                   on_done=None, exact_guard: bool = False):
     """Runs every mesh through the PaMO pipeline; returns ([(V, F)], seconds).
 
     ``ratio`` and ``use_stage3`` are scalars or per-mesh sequences.  Outputs
     are numpy float32 vertices and int32 faces.  ``packed`` may hold the
     result of ``pack_csr(meshes)`` (device inputs already resident).
     ``on_done(mesh, status)``, if given, is called from a worker thread as each
-    mesh finishes."""
+    mesh finishes.  ``exact_guard`` selects the exact guard predicate for
+    every mesh, as ``PaMO(..., exact_guard=True)`` does."""
     n = len(meshes) if meshes is not None else len(packed[2]) - 1
     ratio = np.broadcast_to(np.asarray(ratio, np.float64), (n,)).copy()
     s3 = np.broadcast_to(np.asarray(use_stage3, np.int32), (n,)).copy()
@@ -62,7 +63,8 @@ def simplify_batch(meshes, ratio, use_stage3=False, n_slots: int = 4, tolerance:
     _hip.check(_hip.lib().pamo_batch_run(
         torch.cuda.current_device(), int(n_slots), ctypes.byref(params) if params is not None else None, n,
         Vd.data_ptr(), Fd.data_ptr(), vo.ctypes.data, fo.ctypes.data, ratio.ctypes.data, s3.ctypes.data,
-        int(tolerance), float(threshold), int(iters), int(min_verts), ov.data_ptr(), of.data_ptr(),
+        int(tolerance), float(threshold), int(iters), int(min_verts), int(bool(exact_guard)), ov.data_ptr(),
+        of.data_ptr(),
         onv.ctypes.data, onf.ctypes.data, ctypes.byref(secs), ctypes.cast(cb, ctypes.c_void_p) if cb else None, None),
         "pamo_batch_run")
     ovh, ofh = ov.cpu().numpy(), of.cpu().numpy()
@@ -75,12 +77,13 @@ class Server:
     free slot and estimated HBM footprint and run concurrently."""
 
     def __init__(self, n_slots: int = 4, hbm_budget_bytes: int = 0, stage3: bool = True,
-                 config: Stage3Config | None = None):
+                 config: Stage3Config | None = None, exact_guard: bool = False):
         self.ptr = ctypes.c_void_p()
         self._params = (config or Stage3Config()).to_params() if stage3 else None
         _hip.check(_hip.lib().pamo_serve_create(
             ctypes.byref(self.ptr), torch.cuda.current_device(), int(n_slots), int(hbm_budget_bytes),
-            ctypes.byref(self._params) if self._params is not None else None), "pamo_serve_create")
+            ctypes.byref(self._params) if self._params is not None else None, int(bool(exact_guard))),
+            "pamo_serve_create")
         self._keep = {}
 
     def close(self):

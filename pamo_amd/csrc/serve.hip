@@ -250,8 +250,10 @@ static void destroy(Engine* e) {
     delete e;
 }
 
-static int create(Engine** out, int device, int n_slots, long long budget, const pamo_s3_params* s3) {
+static int create(Engine** out, int device, int n_slots, long long budget, const pamo_s3_params* s3,
+                  int exact_guard) {
     PAMO_CHECK(out && n_slots > 0 && n_slots <= 64, "pamo serve/batch: 1..64 slots");
+    PAMO_CHECK(exact_guard >= 0 && exact_guard <= 2, "pamo serve/batch: exact_guard must be 0, 1 or 2");
     Engine* e = new Engine();
     e->device = device;
     e->budget = budget > 0 ? budget : (long long)200 << 30;
@@ -265,6 +267,7 @@ static int create(Engine** out, int device, int n_slots, long long budget, const
     }
     for (Slot& s : e->slots) {
         int rc = pamo_ctx_create(&s.c2, device);
+        if (rc == 0 && exact_guard) rc = pamo_ctx_set_option(s.c2, PAMO_OPT_EXACT_GUARD, exact_guard);
         if (rc == 0 && e->has_s3) rc = pamo_s3_create(&s.c3, device, &e->s3p);
         if (rc == 0 && hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) rc = 1;
         if (rc != 0) {
@@ -300,8 +303,8 @@ extern "C" {
 int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_meshes, const float* d_verts,
                    const int* d_faces, const long long* vert_off, const long long* face_off, const double* ratio,
                    const int* use_stage3, int tolerance, float threshold, long long iters, long long min_verts,
-                   float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf, double* seconds,
-                   pamo_batch_done_fn on_done, void* user) {
+                   int exact_guard, float* d_out_verts, int* d_out_faces, int* out_nv, int* out_nf,
+                   double* seconds, pamo_batch_done_fn on_done, void* user) {
     pamo::clear_error();
     PAMO_CHECK(n_meshes >= 0 && d_verts && d_faces && vert_off && face_off && ratio && use_stage3 && d_out_verts &&
                    d_out_faces && out_nv && out_nf,
@@ -311,7 +314,7 @@ int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_mesh
         PAMO_CHECK(!use_stage3[i] || s3, "pamo_batch_run: stage 3 requested without stage-3 parameters");
     }
     Engine* e = nullptr;
-    PAMO_TRY(pamo::serve::create(&e, device, n_slots, 0, s3));
+    PAMO_TRY(pamo::serve::create(&e, device, n_slots, 0, s3, exact_guard));
     e->on_done = on_done;
     e->on_done_user = user;
     const double t0 = pamo::serve::now_s();
@@ -367,10 +370,10 @@ int pamo_batch_run(int device, int n_slots, const pamo_s3_params* s3, int n_mesh
 
 // ------------------------------------------------------------------ served form
 int pamo_serve_create(pamo_server** out, int device, int n_slots, long long hbm_budget_bytes,
-                      const pamo_s3_params* s3) {
+                      const pamo_s3_params* s3, int exact_guard) {
     pamo::clear_error();
     Engine* e = nullptr;
-    PAMO_TRY(pamo::serve::create(&e, device, n_slots, hbm_budget_bytes, s3));
+    PAMO_TRY(pamo::serve::create(&e, device, n_slots, hbm_budget_bytes, s3, exact_guard));
     *out = reinterpret_cast<pamo_server*>(e);
     return 0;
 }

@@ -28,8 +28,8 @@ def _meshes():
     return out
 
 
-def _alone(V, F, ratio, stage3):
-    pm = PaMO((V, F), use_stage1=False, use_stage3=stage3)
+def _alone(V, F, ratio, stage3, exact_guard=False):
+    pm = PaMO((V, F), use_stage1=False, use_stage3=stage3, exact_guard=exact_guard)
     v, f = pm.run(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda(), ratio, 4, 1e-3, 100000, 0)
     return np.asarray(v), np.asarray(f)
 
@@ -70,6 +70,24 @@ def test_batch_matches_each_mesh_alone(n_slots):
     out, secs = simplify_batch(meshes, ratios, stage3, n_slots=n_slots)
     for (V, F), r, s3, got in zip(meshes, ratios, stage3, out):
         _same(got, _alone(V, F, r, s3), V, s3)
+
+
+def test_batch_and_server_exact_guard_match_plain_form():
+    """The exact guard setting reaches every slot: batched and served results
+    equal PaMO(..., exact_guard=True) run alone."""
+    meshes = _meshes()
+    ratios = [0.05, 0.1, 0.02, 0.05, 0.02]
+    stage3 = [False, True, False, False, True]
+    print("case: batch of 5 meshes, exact guard, 2 slots", flush=True)
+    out, _ = simplify_batch(meshes, ratios, stage3, n_slots=2, exact_guard=True)
+    for (V, F), r, s3, got in zip(meshes, ratios, stage3, out):
+        _same(got, _alone(V, F, r, s3, exact_guard=True), V, s3)
+    print("case: server, 2 jobs, exact guard, 2 slots", flush=True)
+    srv = Server(n_slots=2, stage3=True, exact_guard=True)
+    ids = [srv.submit(V, F, r, s3) for (V, F), r, s3 in list(zip(meshes, ratios, stage3))[3:]]
+    for i, (V, F), r, s3 in zip(ids, meshes[3:], ratios[3:], stage3[3:]):
+        _same(srv.result(i)[:2], _alone(V, F, r, s3, exact_guard=True), V, s3)
+    srv.close()
 
 
 def test_served_results_same_alone_and_in_the_mix():
