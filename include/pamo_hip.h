@@ -90,6 +90,15 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
                   int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
                   int* rounds_out);
 
+/* The loop of the reference's Blender library (pamo_blender/cusimpinterface.cu:57-161)
+ * on host arrays: in_out_nv_ni = {n_v, n_t} in, result counts out; no
+ * centring, no face target, stop after four rounds without progress.  The
+ * context keeps its persistent state across calls as the library's static
+ * CUSimp_Free does.  libcusimp.so exports it under the add-on's name
+ * pamo_simplify (include/pamo_blender.h). */
+int pamo_blender_simplify(pamo_ctx* ctx, const float* in_verts, const int* in_tris, float* out_verts, int* out_tris,
+                          int* in_out_nv_ni, float threshold);
+
 /* Launch profiler (no counterpart in the reference, which only prints host
  * wall clock, simp_cuda/pamo/__init__.py:139-148,187-190): HIP events are
  * recorded around launches of the kernels selected by kernel_mask (bit k =

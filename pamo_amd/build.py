@@ -2,6 +2,7 @@
 
 * ``lib/libpamo_hip.so``   -- HIP kernels for gfx950 + the C ABI (include/pamo_hip.h)
 * ``lib/libpamo_synth.so`` -- host C++ workload generator
+* ``lib/libcusimp.so``     -- the Blender add-on's C ABI (include/pamo_blender.h)
 
 The HIP flags keep IEEE single-precision division and square root correctly
 rounded and disable FMA contraction (the reference builds with
@@ -55,8 +56,19 @@ def build_synth():
     return out
 
 
+def build_blender():
+    """libcusimp.so: the Blender add-on's library name and signature
+    (include/pamo_blender.h), forwarding to libpamo_hip.so."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcusimp.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"), "-o", out + ".tmp",
+          os.path.join(HERE, "csrc", "blender_abi.cpp"), "-ldl"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_hip(), build_synth()]
+    return [build_hip(), build_synth(), build_blender()]
 
 
 if __name__ == "__main__":

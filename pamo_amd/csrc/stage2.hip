@@ -854,19 +854,18 @@ int pamo_round_debug(pamo_ctx* ctx, int what, void* d_out, long long cap, long l
     return 0;
 }
 
-int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d_faces, int n_faces, double ratio,
-                  int tolerance, float threshold, long long iters, long long min_verts, float* d_out_verts,
-                  int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
-                  int* rounds_out) {
-    pamo::clear_error();
-    PAMO_CHECK(ctx != nullptr, "pamo_simplify: null context");
-    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
-    pamo::ProfScope ps(c);
-    PAMO_CHECK(n_verts > 0 && n_faces > 0, "pamo_simplify: empty mesh");
-    PAMO_CHECK(d_verts && d_faces && d_out_verts && d_out_faces, "pamo_simplify: null buffer");
-    PAMO_HIP(hipSetDevice(c.device));
-    long long target = (long long)(ratio * (double)n_faces);
-    if (min_verts > target) target = min_verts;
+}  // extern "C"
+
+namespace pamo {
+
+// The round loop shared by pamo_simplify (PaMO.run's stage 2) and
+// pamo_blender_simplify: stop at the face target, at 10 faces when
+// small_stop is set (PaMO.run only), or after `tolerance` rounds without
+// progress.
+static int simplify_loop(Ctx& c, const float* d_verts, int n_verts, const int* d_faces, int n_faces, long long target,
+                         bool small_stop, int tolerance, float threshold, long long iters, float* d_out_verts,
+                         int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
+                         int* rounds_out) {
     PAMO_TRY(pamo::grow_pts(c, (size_t)n_verts));
     PAMO_TRY(pamo::grow_tris(c, (size_t)n_faces));
     PAMO_TRY(pamo::load_mesh(c, d_verts, n_verts, d_faces, n_faces));
@@ -908,7 +907,7 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
         nV = c.hcnt[pamo::C_NV];
         nF = c.hcnt[pamo::C_NF];
         if (face_hist && rounds <= hist_cap) face_hist[rounds - 1] = nF;
-        if (nF <= target || nF <= 10) break;
+        if (nF <= target || (small_stop && nF <= 10)) break;
         if (nF == prev) ++count;
         else { count = 0; is_stuck = false; }
         if (count >= 2) is_stuck = true;
@@ -925,6 +924,81 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
     if (n_verts_out) *n_verts_out = nV;
     if (n_faces_out) *n_faces_out = nF;
     if (rounds_out) *rounds_out = rounds;
+    return 0;
+}
+
+}  // namespace pamo
+
+extern "C" {
+
+int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d_faces, int n_faces, double ratio,
+                  int tolerance, float threshold, long long iters, long long min_verts, float* d_out_verts,
+                  int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
+                  int* rounds_out) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_simplify: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    pamo::ProfScope ps(c);
+    PAMO_CHECK(n_verts > 0 && n_faces > 0, "pamo_simplify: empty mesh");
+    PAMO_CHECK(d_verts && d_faces && d_out_verts && d_out_faces, "pamo_simplify: null buffer");
+    PAMO_HIP(hipSetDevice(c.device));
+    long long target = (long long)(ratio * (double)n_faces);
+    if (min_verts > target) target = min_verts;
+    return pamo::simplify_loop(c, d_verts, n_verts, d_faces, n_faces, target, true, tolerance, threshold, iters,
+                               d_out_verts, d_out_faces, n_verts_out, n_faces_out, face_hist, hist_cap, rounds_out);
+}
+
+int pamo_blender_simplify(pamo_ctx* ctx, const float* in_verts, const int* in_tris, float* out_verts, int* out_tris,
+                          int* in_out_nv_ni, float threshold) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx && in_verts && in_tris && out_verts && out_tris && in_out_nv_ni,
+               "pamo_blender_simplify: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    const int nv = in_out_nv_ni[0], nt = in_out_nv_ni[1];
+    PAMO_CHECK(nv > 0 && nt > 0, "pamo_blender_simplify: empty mesh");
+    PAMO_HIP(hipSetDevice(c.device));
+    for (long long i = 0; i < 3LL * nt; ++i)
+        PAMO_CHECK(in_tris[i] >= 0 && in_tris[i] < nv, "pamo_blender_simplify: face index outside the vertices");
+    // host arrays staged through device buffers on the context's stream
+    float *dv = nullptr, *dvo = nullptr;
+    int *dt = nullptr, *dto = nullptr;
+    auto release = [&]() {
+        void* ptrs[] = {dv, dvo, dt, dto};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+    };
+    hipError_t e = hipMalloc((void**)&dv, sizeof(float) * 3 * (size_t)nv);
+    if (e == hipSuccess) e = hipMalloc((void**)&dvo, sizeof(float) * 3 * (size_t)nv);
+    if (e == hipSuccess) e = hipMalloc((void**)&dt, sizeof(int) * 3 * (size_t)nt);
+    if (e == hipSuccess) e = hipMalloc((void**)&dto, sizeof(int) * 3 * (size_t)nt);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, in_verts, sizeof(float) * 3 * (size_t)nv, hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, in_tris, sizeof(int) * 3 * (size_t)nt, hipMemcpyHostToDevice, c.stream);
+    if (e != hipSuccess) {
+        release();
+        pamo::set_error(std::string("pamo_blender_simplify: ") + hipGetErrorString(e));
+        return 1;
+    }
+    int onv = 0, onf = 0;
+    // cusimpinterface.cu:57-161: no centring, no face target, stop after four
+    // rounds without progress (or 1e6 rounds)
+    int rc;
+    {
+        pamo::ProfScope ps(c);
+        rc = pamo::simplify_loop(c, dv, nv, dt, nt, 0, false, 4, threshold, 1000000, dvo, dto, &onv, &onf, nullptr,
+                                 0, nullptr);
+    }
+    if (rc == 0) e = hipMemcpyAsync(out_verts, dvo, sizeof(float) * 3 * (size_t)onv, hipMemcpyDeviceToHost, c.stream);
+    if (rc == 0 && e == hipSuccess)
+        e = hipMemcpyAsync(out_tris, dto, sizeof(int) * 3 * (size_t)onf, hipMemcpyDeviceToHost, c.stream);
+    if (rc == 0 && e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    release();
+    if (rc != 0) return rc;
+    if (e != hipSuccess) {
+        pamo::set_error(std::string("pamo_blender_simplify: ") + hipGetErrorString(e));
+        return 1;
+    }
+    in_out_nv_ni[0] = onv;
+    in_out_nv_ni[1] = onf;
     return 0;
 }
 
