@@ -90,6 +90,22 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
                   int* d_out_faces, int* n_verts_out, int* n_faces_out, int* face_hist, int hist_cap,
                   int* rounds_out);
 
+/* Legacy simplifier behind PaSP (reference simp_cuda/src/cusimp.cu,
+ * simp_cuda/pamo/__init__.py:208-253): midpoint collapses of edges whose
+ * legacy cost (cusimp.cu:283-482) is at most the threshold, chosen as
+ * independent sets; no intersection guard and no undo. */
+/* One round: cusimp::CUDSP.forward (pybind.cpp:140-183, cusimp.cu:587-720);
+ * outputs as pamo_round's (points (n_pts,3) with removed ones zeroed, faces
+ * (n_tris,3) with removed rows -1, occ and map (n_pts,)). */
+int pamo_pasp_round(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tris, int n_tris, float scale,
+                    float threshold, int init, float* d_out_pts, int* d_out_tris, int* d_out_occ, int* d_out_map);
+/* PaSP.run (pamo/__init__.py:224-253) on the device: scale = largest
+ * bounding-box extent, rounds with compaction until one removes no face or
+ * `iters` rounds. */
+int pamo_pasp_run(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d_faces, int n_faces, float threshold,
+                  long long iters, float* d_out_verts, int* d_out_faces, int* n_verts_out, int* n_faces_out,
+                  int* rounds_out);
+
 /* The loop of the reference's Blender library (pamo_blender/cusimpinterface.cu:57-161)
  * on host arrays: in_out_nv_ni = {n_v, n_t} in, result counts out; no
  * centring, no face target, stop after four rounds without progress.  The

@@ -61,6 +61,11 @@ def lib():
         L.orc_candidates.argtypes = [c_vp, c_int, c_vp, c_int, c_vp, c_ll]
         L.orc_ref_guard.restype = c_int
         L.orc_ref_guard.argtypes = [c_vp, c_vp, c_vp, c_vp, c_float]
+        L.orc_pasp_forward.restype = None
+        L.orc_pasp_forward.argtypes = [c_vp, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_vp, c_vp, c_vp,
+                                       c_vp]
+        L.orc_pasp_run.restype = c_int
+        L.orc_pasp_run.argtypes = [c_vp, c_vp, c_int, c_vp, c_int, c_float, c_ll, c_vp, c_vp, c_vp, c_vp]
         L.orc_run.restype = c_int
         L.orc_run.argtypes = [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_float, c_ll, c_ll, c_int, c_vp, c_vp,
                               c_vp, c_vp, c_vp, c_int]
@@ -211,3 +216,31 @@ def run(verts, faces, ratio, tolerance=4, threshold=1e-3, iters=1000000, min_ver
                            float(threshold), int(iters), int(min_verts), int(pred_mode), ov.ctypes.data,
                            ctypes.byref(nv), of.ctypes.data, ctypes.byref(nf), hist.ctypes.data, len(hist))
     return ov[:nv.value].copy(), of[:nf.value].copy(), hist[:min(rounds, len(hist))].copy()
+
+
+def pasp_forward(verts, faces, scale, threshold, init, ctx: Round | None = None):
+    """One legacy round (cusimp::CUDSP.forward); returns (points, faces, occ, map)."""
+    V = _f32(verts).reshape(-1, 3)
+    F = _i32(faces).reshape(-1, 3)
+    r = Round() if ctx is None else ctx
+    op = np.empty_like(V)
+    of = np.empty_like(F)
+    oo = np.empty(len(V), np.int32)
+    om = np.empty(len(V), np.int32)
+    lib().orc_pasp_forward(r.ctx, V.ctypes.data, len(V), F.ctypes.data, len(F), float(scale), float(threshold),
+                           int(bool(init)), op.ctypes.data, of.ctypes.data, oo.ctypes.data, om.ctypes.data)
+    return op, of, oo, om
+
+
+def pasp_run(verts, faces, threshold=0.001, iters=1000, ctx: Round | None = None):
+    """PaSP.run on the oracle; returns (vertices, faces, rounds)."""
+    V = _f32(verts).reshape(-1, 3)
+    F = _i32(faces).reshape(-1, 3)
+    r = Round() if ctx is None else ctx
+    ov = np.empty_like(V)
+    of = np.empty_like(F)
+    nv = c_int()
+    nf = c_int()
+    rounds = lib().orc_pasp_run(r.ctx, V.ctypes.data, len(V), F.ctypes.data, len(F), float(threshold), int(iters),
+                                ov.ctypes.data, ctypes.byref(nv), of.ctypes.data, ctypes.byref(nf))
+    return ov[:nv.value].copy(), of[:nf.value].copy(), rounds

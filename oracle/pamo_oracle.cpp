@@ -959,6 +959,157 @@ static void forward(Ctx& c, const float* pts, int nP, const int* tris, int nT, c
     }
 }
 
+// ---------------------------------------------------------------- legacy PaSP (cusimp.cu)
+// compute_edge_cost_kernel of the legacy simplifier, cusimp.cu:283-482: the
+// link test, edge-length, skinny and flip terms of edge_costs without the
+// invalid / stuck rules, with three differences: the skinny ratio is a
+// double expression (`4.0f * sqrt(3)` takes the double root; the squared
+// lengths are float squares) cast to float, it is not scaled by 5, and the
+// averaged edge length is counted twice (cusimp.cu:378,
+// `edge_len += edge_len + |v0 - v1| / s * t`).
+static void edge_costs_legacy(Ctx& c, float edge_s, float tres) {
+    const double k4s3 = 4.0 * std::sqrt(3.0);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int e = 0; e < c.n_edges; ++e) {
+        int u = c.eu[e], v = c.ev[e];
+        V3 p0 = c.P[u], p1 = c.P[v];
+        int dup = 0;
+        bool over = false;
+        for (int a = c.ring_off[u]; a < c.ring_off[u + 1] && !over; ++a) {
+            int su = succ_of(c.T[c.ring[a]], u);
+            for (int b = c.ring_off[v]; b < c.ring_off[v + 1]; ++b) {
+                int sv = succ_of(c.T[c.ring[b]], v);
+                if (su == sv) ++dup;
+                if (dup > 2) { over = true; break; }
+            }
+        }
+        if (over) { c.edge_cost[e] = 0xFFFFFFFFu; continue; }
+        if (dup != 2) continue;  // stale
+        float elen = 0.0f;
+        int nedge = 0;
+        for (int side = 0; side < 2; ++side) {
+            int w = side ? v : u;
+            for (int a = c.ring_off[w]; a < c.ring_off[w + 1]; ++a) {
+                const Tri t = c.T[c.ring[a]];
+                if (t.i > t.j) { elen += vnorm(vsub(c.P[t.i], c.P[t.j])); ++nedge; }
+                if (t.j > t.k) { elen += vnorm(vsub(c.P[t.j], c.P[t.k])); ++nedge; }
+                if (t.k > t.i) { elen += vnorm(vsub(c.P[t.k], c.P[t.i])); ++nedge; }
+            }
+        }
+        elen = elen / (float)nedge / edge_s * tres;
+        const float elen_edge = vnorm(vsub(p0, p1)) / edge_s * tres;
+        elen = elen + (elen + elen_edge);
+        V3 vm = vdivs(vadd(p0, p1), 2.0f);
+        float qa = 0.0f;
+        int ntri = 0;
+        bool flip = false;
+        for (int side = 0; side < 2 && !flip; ++side) {
+            int w = side ? v : u, other = side ? u : v;
+            for (int a = c.ring_off[w]; a < c.ring_off[w + 1]; ++a) {
+                const Tri t = c.T[c.ring[a]];
+                if (t.i == other || t.j == other || t.k == other) continue;
+                V3 o0 = c.P[t.i], o1 = c.P[t.j], o2 = c.P[t.k];
+                V3 n0 = o0, n1 = o1, n2 = o2;
+                if (t.i == w) n0 = vm;
+                if (t.j == w) n1 = vm;
+                if (t.k == w) n2 = vm;
+                V3 on = vcross(vsub(o1, o0), vsub(o2, o0));
+                on = vdivs(on, vnorm(on));
+                V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+                nn = vdivs(nn, vnorm(nn));
+                if (vdot(on, nn) < 0) { flip = true; break; }
+                float l01 = vnorm(vsub(n0, n1)), l12 = vnorm(vsub(n1, n2)), l20 = vnorm(vsub(n2, n0));
+                float sq01 = l01 * l01, sq12 = l12 * l12, sq20 = l20 * l20;
+                double qd = k4s3 * (double)heron(n0, n1, n2) / (double)sq01 + (double)sq12 + (double)sq20 +
+                            (double)0.0000001f;
+                qa += 1.0f - tclamp((float)qd, 0.0f, 1.0f);
+                ++ntri;
+            }
+        }
+        if (flip) { c.edge_cost[e] = 0xFFFFFFFFu; continue; }
+        float M[16];
+        const float* qu = &c.Q[(size_t)u * 10];
+        const float* qv = &c.Q[(size_t)v * 10];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) M[i * 4 + j] = qu[qidx(i, j)] + qv[qidx(i, j)];
+        float x[4] = {vm.x, vm.y, vm.z, 1.0f};
+        float r[4];
+        for (int j = 0; j < 4; ++j) r[j] = M[0 * 4 + j] * x[0] + M[1 * 4 + j] * x[1] + M[2 * 4 + j] * x[2] + M[3 * 4 + j] * x[3];
+        float cost = (r[0] * x[0] + r[1] * x[1] + r[2] * x[2] + r[3] * x[3]) / (edge_s * edge_s);
+        float tot = cost + elen + qa / (float)ntri * tres;
+        c.edge_cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    }
+}
+
+// CUSimp::forward, cusimp.cu:587-720: load (faces shuffled when init), rings
+// in ascending face order, edges, quadrics, legacy costs, per-face minimum
+// keys (propagate_edge_cost_kernel :484-509) and the midpoint collapse of
+// every edge at or below the threshold holding the minimum of all faces of
+// both rings (collapse_edge_kernel :511-585); pts_map = exclusive scan of
+// occupancy.
+static void pasp_forward(Ctx& c, const float* pts, int nP, const int* tris, int nT, float scale, float thr,
+                         bool init) {
+    const uint32_t collapse_t = sat_u32(tclamp(thr, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    c.n_pts = nP;
+    c.n_tris = nT;
+    grow_pts(c, nP);
+    c.P.resize(nP);
+    for (int v = 0; v < nP; ++v) c.P[v] = V3{pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+    c.occ.assign(nP + 1, 1);
+    c.occ[nP] = 0;
+    c.T.resize(nT);
+    if (init) {
+        std::vector<int64_t> perm;
+        shuffle_perm((uint64_t)nT, perm);
+        for (int f = 0; f < nT; ++f) {
+            int64_t s = perm[f];
+            c.T[f] = Tri{tris[3 * s], tris[3 * s + 1], tris[3 * s + 2]};
+        }
+    } else {
+        for (int f = 0; f < nT; ++f) c.T[f] = Tri{tris[3 * f], tris[3 * f + 1], tris[3 * f + 2]};
+    }
+    build_rings(c);
+    build_edges(c);
+    vertex_quadrics(c);
+    grow_edges(c, c.n_edges);
+    edge_costs_legacy(c, scale, thr);
+    c.tri_min.assign(nT, ~0ull);
+    for (int e = 0; e < c.n_edges; ++e) {
+        uint64_t k = edge_key(c.edge_cost[e], e);
+        for (int w : {c.eu[e], c.ev[e]})
+            for (int a = c.ring_off[w]; a < c.ring_off[w + 1]; ++a) {
+                uint64_t& m = c.tri_min[c.ring[a]];
+                if (k < m) m = k;
+            }
+    }
+    std::vector<char> sel(c.n_edges, 0);
+    for (int e = 0; e < c.n_edges; ++e) {
+        if (c.edge_cost[e] > collapse_t) continue;
+        uint64_t k = edge_key(c.edge_cost[e], e);
+        bool ok = true;
+        for (int w : {c.eu[e], c.ev[e]}) {
+            for (int a = c.ring_off[w]; a < c.ring_off[w + 1]; ++a)
+                if (c.tri_min[c.ring[a]] != k) { ok = false; break; }
+            if (!ok) break;
+        }
+        sel[e] = ok;
+    }
+    for (int e = 0; e < c.n_edges; ++e) {
+        if (!sel[e]) continue;
+        int u = c.eu[e], v = c.ev[e];
+        c.P[u] = vdivs(vadd(c.P[u], c.P[v]), 2.0f);
+        c.P[v] = V3{0, 0, 0};
+        c.occ[v] = 0;
+        rewrite_rings(c, u, v);
+    }
+    int acc = 0;
+    for (int v = 0; v <= nP; ++v) {
+        int o = c.occ[v];
+        c.pts_map[v] = acc;
+        acc += o;
+    }
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------- C ABI
@@ -1162,6 +1313,69 @@ int orc_run(void* ctx, const float* verts, int nV, const int* faces, int nF, dou
         else { count = 0; is_stuck = false; }
         if (count >= 2) is_stuck = true;
         if (count == tolerance) break;
+    }
+    std::memcpy(out_v, V.data(), sizeof(float) * V.size());
+    std::memcpy(out_f, F.data(), sizeof(int) * F.size());
+    *out_nv = (int)(V.size() / 3);
+    *out_nf = (int)(F.size() / 3);
+    return rounds;
+}
+
+// cusimp::CUDSP.forward (pybind.cpp:140-183) on the oracle.
+void orc_pasp_forward(void* ctx, const float* pts, int nP, const int* tris, int nT, float scale, float thr, int init,
+                      float* out_pts, int* out_tris, int* out_occ, int* out_map) {
+    orc::Ctx& c = *(orc::Ctx*)ctx;
+    orc::pasp_forward(c, pts, nP, tris, nT, scale, thr, init != 0);
+    for (int v = 0; v < nP; ++v) {
+        out_pts[3 * v] = c.P[v].x;
+        out_pts[3 * v + 1] = c.P[v].y;
+        out_pts[3 * v + 2] = c.P[v].z;
+        out_occ[v] = c.occ[v];
+        out_map[v] = c.pts_map[v];
+    }
+    for (int f = 0; f < nT; ++f) {
+        out_tris[3 * f] = c.T[f].i;
+        out_tris[3 * f + 1] = c.T[f].j;
+        out_tris[3 * f + 2] = c.T[f].k;
+    }
+}
+
+// PaSP.run (pamo/__init__.py:224-253): scale = largest extent, rounds with
+// compaction until one removes no face.  Returns the number of rounds.
+int orc_pasp_run(void* ctx, const float* verts, int nV, const int* faces, int nF, float thr, long long iters,
+                 float* out_v, int* out_nv, int* out_f, int* out_nf) {
+    orc::Ctx& c = *(orc::Ctx*)ctx;
+    std::vector<float> V(verts, verts + 3 * (size_t)nV);
+    std::vector<int> F(faces, faces + 3 * (size_t)nF);
+    float ext[3];
+    for (int d = 0; d < 3; ++d) {
+        float mn = std::numeric_limits<float>::infinity(), mx = -mn;
+        for (int v = 0; v < nV; ++v) { mn = std::fmin(mn, V[3 * v + d]); mx = std::fmax(mx, V[3 * v + d]); }
+        ext[d] = mx - mn;
+    }
+    float scale = ext[0];
+    if (ext[1] > scale) scale = ext[1];
+    if (ext[2] > scale) scale = ext[2];
+    int rounds = 0;
+    for (long long it = 0; it < iters; ++it) {
+        int nv = (int)(V.size() / 3), nf = (int)(F.size() / 3);
+        orc::pasp_forward(c, V.data(), nv, F.data(), nf, scale, thr, it == 0);
+        ++rounds;
+        std::vector<float> V2;
+        V2.reserve(V.size());
+        for (int v = 0; v < nv; ++v)
+            if (c.occ[v]) { V2.push_back(c.P[v].x); V2.push_back(c.P[v].y); V2.push_back(c.P[v].z); }
+        std::vector<int> F2;
+        F2.reserve(F.size());
+        for (int f = 0; f < nf; ++f) {
+            if (c.T[f].i < 0) continue;
+            F2.push_back(c.pts_map[c.T[f].i]);
+            F2.push_back(c.pts_map[c.T[f].j]);
+            F2.push_back(c.pts_map[c.T[f].k]);
+        }
+        V.swap(V2);
+        F.swap(F2);
+        if ((int)(F.size() / 3) == nf) break;
     }
     std::memcpy(out_v, V.data(), sizeof(float) * V.size());
     std::memcpy(out_f, F.data(), sizeof(int) * F.size());

@@ -104,6 +104,8 @@ SIGNATURES = [
                                 ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     ("pamo_serve_fetch", c_int, [c_vp, c_ll, c_vp, c_vp]),
     ("pamo_blender_simplify", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_float]),
+    ("pamo_pasp_round", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_vp, c_vp, c_vp, c_vp]),
+    ("pamo_pasp_run", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_float, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("pamo_metrics_create", c_int, [ctypes.POINTER(c_vp), c_int]),
     ("pamo_metrics_destroy", c_int, [c_vp]),
     ("pamo_point_mesh_sqdist", c_int, [c_vp, c_vp, c_ll, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),

@@ -442,26 +442,10 @@ static int undo_candidates(Ctx& c, int& npushed) {
 
 // One round on the mesh held in (p, ti, tj, tk) with nP points and nT faces
 // (cusimp_free.cu:891-1048); the undo-list prologue has already run.
-static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_stuck) {
-    const float tres = thr, edge_s = scale;
-    uint32_t collapse_t;  // u32(clamp(thr, 0, 10) / 10 * 2^32) with saturation (:923)
-    {
-        float x = thr > 0.0f ? thr : 0.0f;
-        x = x < 10.0f ? x : 10.0f;
-        float q = x / 10.0f * 4294967296.0f;
-        collapse_t = !(q < 4294967296.0f) ? 0xFFFFFFFFu : (uint32_t)q;
-    }
-    Mesh m = cur_mesh(c);
-    c.n_pts = nP;
-    c.n_tris = nT;
-    // snapshot for undo
-    PAMO_HIP(hipMemcpyAsync(c.op, c.p, sizeof(float4) * nP, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.oi, c.ti, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.oj, c.tj, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.ok, c.tk, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_TRY(fill(c.occ, 1, nP, c.stream));
-    PAMO_TRY(fill(c.occ + nP, 0, 1, c.stream));
-    // vertex -> face rings
+// Vertex -> face rings in ascending face order (:276-304), the edge list
+// (:306-335), the quadrics (:337-366) and the edge-cost launch order; sets
+// c.n_edges.  Shared by the round and the legacy round.
+static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     PAMO_HIP(hipMemsetAsync(c.ring_off, 0, sizeof(int) * (nP + 1), c.stream));
     PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
     LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP, ring_count_k, nT, m, nT, c.ring_off);
@@ -479,13 +463,64 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     c.n_edges = nE;
     PAMO_TRY(grow_edges(c, (size_t)nE));
     LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev);
-    // quadrics + cost
+    // quadrics
     LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
     // edge-cost launch order; ring_fill is free once the rings are sorted
     PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
     LAUNCH(edge_order_count_k, nE, c.eu, nE, c.ring_fill);
     PAMO_TRY(scan_exclusive(c.ring_fill, c.ring_fill, nP + 1, c.ws, c.stream));
     LAUNCH(edge_order_scatter_k, nE, c.eu, nE, c.ring_fill, c.eord);
+    return 0;
+}
+
+// u32(clamp(thr, 0, 10) / 10 * 2^32) with saturation (:923, cusimp.cu:589)
+static uint32_t collapse_threshold(float thr) {
+    float x = thr > 0.0f ? thr : 0.0f;
+    x = x < 10.0f ? x : 10.0f;
+    float q = x / 10.0f * 4294967296.0f;
+    return !(q < 4294967296.0f) ? 0xFFFFFFFFu : (uint32_t)q;
+}
+
+// One round of the legacy simplifier behind PaSP, CUSimp::forward
+// (cusimp.cu:587-720), on the mesh loaded in c: rings, edges and quadrics as
+// in the round, the legacy cost, the per-vertex minima and the midpoint
+// collapse; no guard, no undo, no line collapse.  Leaves occ and pts_map.
+static int pasp_round_device(Ctx& c, int nP, int nT, float scale, float thr) {
+    const uint32_t collapse_t = collapse_threshold(thr);
+    Mesh m = cur_mesh(c);
+    c.n_pts = nP;
+    c.n_tris = nT;
+    PAMO_TRY(fill(c.occ, 1, nP, c.stream));
+    PAMO_TRY(fill(c.occ + nP, 0, 1, c.stream));
+    PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
+    const int nE = c.n_edges;
+    LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 4.0 * nE, pasp_cost_k, nE, m,
+            c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.Q, scale, thr, c.edge_cost);
+    PAMO_TRY(fill<unsigned long long>((unsigned long long*)c.vmin, ~0ull, nP, c.stream));
+    LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
+            (unsigned long long*)c.vmin);
+    LAUNCHP(K_SELECT_COLLAPSE, 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP, pasp_collapse_k, nE, m, c.occ,
+            c.ring_off, c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin,
+            collapse_t);
+    PAMO_TRY(scan_exclusive(c.occ, c.pts_map, nP + 1, c.ws, c.stream));
+    return 0;
+}
+
+static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_stuck) {
+    const float tres = thr, edge_s = scale;
+    const uint32_t collapse_t = collapse_threshold(thr);
+    Mesh m = cur_mesh(c);
+    c.n_pts = nP;
+    c.n_tris = nT;
+    // snapshot for undo
+    PAMO_HIP(hipMemcpyAsync(c.op, c.p, sizeof(float4) * nP, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.oi, c.ti, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.oj, c.tj, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.ok, c.tk, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_TRY(fill(c.occ, 1, nP, c.stream));
+    PAMO_TRY(fill(c.occ + nP, 0, 1, c.stream));
+    PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
+    const int nE = c.n_edges;
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
     LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE,
             edge_cost_k, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid, c.Q, cp, c.edge_cost);
@@ -858,6 +893,25 @@ int pamo_round_debug(pamo_ctx* ctx, int what, void* d_out, long long cap, long l
 
 namespace pamo {
 
+// Largest bounding-box extent of the loaded points (pamo/__init__.py:153,
+// :227 for PaSP).
+static int input_scale(Ctx& c, int n_verts, float& scale) {
+    const int nparts = (int)pamo::grid_for(n_verts, pamo::kBlock);
+    hipLaunchKernelGGL(pamo::vert_box_k, dim3(nparts), dim3(pamo::kBlock), 0, c.stream, (const float4*)c.p, n_verts,
+                       c.box_part);
+    PAMO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pamo::whole_box_k, dim3(1), dim3(pamo::kBlock), 0, c.stream, c.box_part, nparts);
+    PAMO_HIP(hipGetLastError());
+    float hb[6];
+    PAMO_HIP(hipMemcpyAsync(hb, c.box_part + 6 * nparts, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    const float ex = hb[3] - hb[0], ey = hb[4] - hb[1], ez = hb[5] - hb[2];
+    scale = ex;
+    if (ey > scale) scale = ey;
+    if (ez > scale) scale = ez;
+    return 0;
+}
+
 // The round loop shared by pamo_simplify (PaMO.run's stage 2) and
 // pamo_blender_simplify: stop at the face target, at 10 faces when
 // small_stop is set (PaMO.run only), or after `tolerance` rounds without
@@ -869,23 +923,8 @@ static int simplify_loop(Ctx& c, const float* d_verts, int n_verts, const int* d
     PAMO_TRY(pamo::grow_pts(c, (size_t)n_verts));
     PAMO_TRY(pamo::grow_tris(c, (size_t)n_faces));
     PAMO_TRY(pamo::load_mesh(c, d_verts, n_verts, d_faces, n_faces));
-    // scale = largest bounding-box extent of the input (pamo/__init__.py:153)
     float scale;
-    {
-        const int nparts = (int)pamo::grid_for(n_verts, pamo::kBlock);
-        hipLaunchKernelGGL(pamo::vert_box_k, dim3(nparts), dim3(pamo::kBlock), 0, c.stream, (const float4*)c.p,
-                           n_verts, c.box_part);
-        PAMO_HIP(hipGetLastError());
-        hipLaunchKernelGGL(pamo::whole_box_k, dim3(1), dim3(pamo::kBlock), 0, c.stream, c.box_part, nparts);
-        PAMO_HIP(hipGetLastError());
-        float hb[6];
-        PAMO_HIP(hipMemcpyAsync(hb, c.box_part + 6 * nparts, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
-        PAMO_HIP(hipStreamSynchronize(c.stream));
-        float ex = hb[3] - hb[0], ey = hb[4] - hb[1], ez = hb[5] - hb[2];
-        scale = ex;
-        if (ey > scale) scale = ey;
-        if (ez > scale) scale = ez;
-    }
+    PAMO_TRY(input_scale(c, n_verts, scale));
     int nV = n_verts, nF = n_faces;
     int n_undo = 0, count = 0, rounds = 0;
     bool is_stuck = false;
@@ -946,6 +985,75 @@ int pamo_simplify(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d
     if (min_verts > target) target = min_verts;
     return pamo::simplify_loop(c, d_verts, n_verts, d_faces, n_faces, target, true, tolerance, threshold, iters,
                                d_out_verts, d_out_faces, n_verts_out, n_faces_out, face_hist, hist_cap, rounds_out);
+}
+
+int pamo_pasp_round(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tris, int n_tris, float scale,
+                    float threshold, int init, float* d_out_pts, int* d_out_tris, int* d_out_occ, int* d_out_map) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_pasp_round: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    pamo::ProfScope ps(c);
+    PAMO_CHECK(n_pts > 0 && n_tris > 0, "pamo_pasp_round: empty mesh");
+    PAMO_CHECK(d_pts && d_tris && d_out_pts && d_out_tris && d_out_occ && d_out_map, "pamo_pasp_round: null buffer");
+    PAMO_HIP(hipSetDevice(c.device));
+    PAMO_TRY(pamo::grow_pts(c, (size_t)n_pts));
+    PAMO_TRY(pamo::grow_tris(c, (size_t)n_tris));
+    PAMO_TRY(pamo::load_mesh(c, d_pts, n_pts, d_tris, n_tris));
+    if (init) PAMO_TRY(pamo::shuffle_faces(c, n_tris));
+    PAMO_TRY(pamo::pasp_round_device(c, n_pts, n_tris, scale, threshold));
+    pamo::Mesh m = pamo::cur_mesh(c);
+    hipLaunchKernelGGL(pamo::store_points_k, dim3(pamo::grid_for(n_pts, pamo::kBlock)), dim3(pamo::kBlock), 0,
+                       c.stream, (const float4*)c.p, n_pts, d_out_pts);
+    PAMO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pamo::store_tris_k, dim3(pamo::grid_for(n_tris, pamo::kBlock)), dim3(pamo::kBlock), 0,
+                       c.stream, m, n_tris, d_out_tris);
+    PAMO_HIP(hipGetLastError());
+    PAMO_HIP(hipMemcpyAsync(d_out_occ, c.occ, sizeof(int) * n_pts, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_HIP(hipMemcpyAsync(d_out_map, c.pts_map, sizeof(int) * n_pts, hipMemcpyDeviceToDevice, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int pamo_pasp_run(pamo_ctx* ctx, const float* d_verts, int n_verts, const int* d_faces, int n_faces, float threshold,
+                  long long iters, float* d_out_verts, int* d_out_faces, int* n_verts_out, int* n_faces_out,
+                  int* rounds_out) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_pasp_run: null context");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    pamo::ProfScope ps(c);
+    PAMO_CHECK(n_verts > 0 && n_faces > 0, "pamo_pasp_run: empty mesh");
+    PAMO_CHECK(d_verts && d_faces && d_out_verts && d_out_faces, "pamo_pasp_run: null buffer");
+    PAMO_HIP(hipSetDevice(c.device));
+    PAMO_TRY(pamo::grow_pts(c, (size_t)n_verts));
+    PAMO_TRY(pamo::grow_tris(c, (size_t)n_faces));
+    PAMO_TRY(pamo::load_mesh(c, d_verts, n_verts, d_faces, n_faces));
+    float scale;
+    PAMO_TRY(pamo::input_scale(c, n_verts, scale));
+    // PaSP.run (pamo/__init__.py:224-253): shuffle on the first round, compact
+    // after each, stop when a round removes no face
+    int nV = n_verts, nF = n_faces, rounds = 0;
+    for (long long it = 0; it < iters; ++it) {
+        const int prev = nF;
+        if (rounds == 0) PAMO_TRY(pamo::shuffle_faces(c, nF));
+        PAMO_TRY(pamo::pasp_round_device(c, nV, nF, scale, threshold));
+        ++rounds;
+        PAMO_TRY(pamo::compact_round(c, nV, nF));
+        nV = c.hcnt[pamo::C_NV];
+        nF = c.hcnt[pamo::C_NF];
+        if (nF == prev) break;
+    }
+    pamo::Mesh m = pamo::cur_mesh(c);
+    hipLaunchKernelGGL(pamo::store_points_k, dim3(pamo::grid_for(nV, pamo::kBlock)), dim3(pamo::kBlock), 0,
+                       c.stream, (const float4*)c.p, nV, d_out_verts);
+    PAMO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pamo::store_tris_k, dim3(pamo::grid_for(nF, pamo::kBlock)), dim3(pamo::kBlock), 0,
+                       c.stream, m, nF, d_out_faces);
+    PAMO_HIP(hipGetLastError());
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    if (n_verts_out) *n_verts_out = nV;
+    if (n_faces_out) *n_faces_out = nF;
+    if (rounds_out) *rounds_out = rounds;
+    return 0;
 }
 
 int pamo_blender_simplify(pamo_ctx* ctx, const float* in_verts, const int* in_tris, float* out_verts, int* out_tris,

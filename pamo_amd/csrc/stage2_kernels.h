@@ -260,6 +260,43 @@ __device__ __forceinline__ float heron(F3 a, F3 b, F3 d) {
     return sqrtf(s * (s - x) * (s - y) * (s - z));
 }
 
+constexpr int kLinkRing = 24;
+
+// Link condition of edge (u, v) (:405-444): dup = #{(a, b) : succ_u(a) ==
+// succ_v(b)} over the faces a of ring(u) and b of ring(v), where succ_x(t) is
+// the vertex after x in face t; more than 2 rejects the collapse, fewer keeps
+// the stale cost.  dup is symmetric in u and v and only its total matters, so
+// the smaller ring sits in registers, kLinkRing successors per pass, and the
+// larger one streams past it four faces per load batch.  Returns 3 as soon
+// as the count exceeds 2.
+__device__ __forceinline__ int link_dup(const int4* __restrict__ rv, int u, int ub, int ue, int v, int vb, int ve) {
+    const bool swap = (ve - vb) > (ue - ub);
+    const int ib = swap ? ub : vb, ie = swap ? ue : ve, iw = swap ? u : v;
+    const int ob = swap ? vb : ub, oe = swap ? ve : ue, ow = swap ? v : u;
+    int dup = 0;
+    for (int c0 = ib; c0 < ie; c0 += kLinkRing) {
+        int sv[kLinkRing];
+#pragma unroll
+        for (int b = 0; b < kLinkRing; ++b) sv[b] = c0 + b < ie ? succ_in(rv[c0 + b], iw) : -2;
+        int a = ob;
+        for (; a + 4 <= oe; a += 4) {
+            const int4 t0 = rv[a], t1 = rv[a + 1], t2 = rv[a + 2], t3 = rv[a + 3];
+            const int s0 = succ_in(t0, ow), s1 = succ_in(t1, ow), s2 = succ_in(t2, ow), s3 = succ_in(t3, ow);
+#pragma unroll
+            for (int b = 0; b < kLinkRing; ++b)
+                dup += (s0 == sv[b] ? 1 : 0) + (s1 == sv[b] ? 1 : 0) + (s2 == sv[b] ? 1 : 0) + (s3 == sv[b] ? 1 : 0);
+            if (dup > 2) return 3;
+        }
+        for (; a < oe; ++a) {
+            const int su = succ_in(rv[a], ow);
+#pragma unroll
+            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
+        }
+        if (dup > 2) return 3;
+    }
+    return dup;
+}
+
 struct CostParams {
     float edge_s, tres;
     int is_stuck;
@@ -284,7 +321,6 @@ __global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __
 
 // compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge; lane
 // t takes edge order[t] and writes cost[order[t]].
-constexpr int kLinkRing = 24;
 
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
                                                       const int4* __restrict__ rv, const int* __restrict__ eu,
@@ -302,42 +338,10 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     }
     if (cp.is_stuck) return;
     const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
-    // link condition: successors of u and v in their rings (:405-444); the
-    // successors of v's ring sit in registers (static indices) when it has
-    // at most kLinkRing faces
-    // dup = #{(a, b) : succ_u(a) == succ_v(b)} is symmetric in u and v and
-    // only its total matters (> 2 rejects, != 2 keeps the stale cost), so the
-    // smaller ring sits in registers, kLinkRing successors per pass, and the
-    // larger one streams past it four faces per load batch
-    const bool swap = (ve - vb) > (ue - ub);
-    const int ib = swap ? ub : vb, ie = swap ? ue : ve, iw = swap ? u : v;
-    const int ob = swap ? vb : ub, oe = swap ? ve : ue, ow = swap ? v : u;
-    int dup = 0;
-    for (int c0 = ib; c0 < ie; c0 += kLinkRing) {
-        int sv[kLinkRing];
-#pragma unroll
-        for (int b = 0; b < kLinkRing; ++b) sv[b] = c0 + b < ie ? succ_in(rv[c0 + b], iw) : -2;
-        int a = ob;
-        for (; a + 4 <= oe; a += 4) {
-            const int4 t0 = rv[a], t1 = rv[a + 1], t2 = rv[a + 2], t3 = rv[a + 3];
-            const int s0 = succ_in(t0, ow), s1 = succ_in(t1, ow), s2 = succ_in(t2, ow), s3 = succ_in(t3, ow);
-#pragma unroll
-            for (int b = 0; b < kLinkRing; ++b)
-                dup += (s0 == sv[b] ? 1 : 0) + (s1 == sv[b] ? 1 : 0) + (s2 == sv[b] ? 1 : 0) + (s3 == sv[b] ? 1 : 0);
-            if (dup > 2) {
-                cost[e] = 0xFFFFFFFFu;
-                return;
-            }
-        }
-        for (; a < oe; ++a) {
-            const int su = succ_in(rv[a], ow);
-#pragma unroll
-            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
-        }
-        if (dup > 2) {
-            cost[e] = 0xFFFFFFFFu;
-            return;
-        }
+    const int dup = link_dup(rv, u, ub, ue, v, vb, ve);
+    if (dup > 2) {
+        cost[e] = 0xFFFFFFFFu;
+        return;
     }
     if (dup != 2) return;
     const F3 p0 = ld(m, u), p1 = ld(m, v);
@@ -515,6 +519,114 @@ __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int*
         }
     }
     sel[e] = s;
+}
+
+// ---------------------------------------------------------------- legacy PaSP round (cusimp.cu)
+// compute_edge_cost_kernel of the legacy simplifier (cusimp.cu:283-482):
+// the link test, edge-length, skinny and flip terms and midpoint quadric cost
+// of edge_cost_k, with the legacy arithmetic: the skinny ratio is formed in
+// FP64 (`4.0f * sqrt(3)` takes the double root, so the expression is double
+// until its float cast), it is not scaled by the skinny penalty, and the
+// averaged edge length is counted twice (`edge_len += edge_len + |v0 - v1|
+// / s * t`, cusimp.cu:378).  No invalid-vertex or stuck rules.
+__global__ void __launch_bounds__(kBlock) pasp_cost_k(Mesh m, const int* __restrict__ off, const int4* __restrict__ rv,
+                                                      const int* __restrict__ eu, const int* __restrict__ ev,
+                                                      const int* __restrict__ order, int nE,
+                                                      const float4* __restrict__ Q, float edge_s, float tres,
+                                                      uint32_t* __restrict__ cost) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nE) return;
+    const int e = order[t];
+    const int u = eu[e], v = ev[e];
+    const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
+    const int dup = link_dup(rv, u, ub, ue, v, vb, ve);
+    if (dup > 2) {
+        cost[e] = 0xFFFFFFFFu;
+        return;
+    }
+    if (dup != 2) return;
+    const F3 p0 = ld(m, u), p1 = ld(m, v);
+    const F3 vm = f3divs(f3add(p0, p1), 2.0f);
+    const double k4s3 = 4.0 * sqrt(3.0);
+    float elen = 0.0f, qa = 0.0f;
+    int nedge = 0, ntri = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int w = side ? v : u, other = side ? u : v;
+        const int b0 = side ? vb : ub, b1 = side ? ve : ue;
+        for (int a = b0; a < b1; ++a) {
+            const int4 tt = rv[a];
+            const F3 pi = ld(m, tt.x), pj = ld(m, tt.y), pk = ld(m, tt.z);
+            if (tt.x > tt.y) { elen += f3norm(f3sub(pi, pj)); ++nedge; }
+            if (tt.y > tt.z) { elen += f3norm(f3sub(pj, pk)); ++nedge; }
+            if (tt.z > tt.x) { elen += f3norm(f3sub(pk, pi)); ++nedge; }
+            if (tt.x == other || tt.y == other || tt.z == other) continue;
+            const F3 n0 = tt.x == w ? vm : pi, n1 = tt.y == w ? vm : pj, n2 = tt.z == w ? vm : pk;
+            F3 on = f3cross(f3sub(pj, pi), f3sub(pk, pi));
+            on = f3divs(on, f3norm(on));
+            F3 nn = f3cross(f3sub(n1, n0), f3sub(n2, n0));
+            nn = f3divs(nn, f3norm(nn));
+            if (f3dot(on, nn) < 0) {
+                cost[e] = 0xFFFFFFFFu;
+                return;
+            }
+            const float l01 = f3norm(f3sub(n0, n1)), l12 = f3norm(f3sub(n1, n2)), l20 = f3norm(f3sub(n2, n0));
+            const double qd = k4s3 * (double)heron(n0, n1, n2) / (double)(l01 * l01) + (double)(l12 * l12) +
+                              (double)(l20 * l20) + (double)0.0000001f;
+            qa += 1.0f - tclamp((float)qd, 0.0f, 1.0f);
+            ++ntri;
+        }
+    }
+    elen = elen / (float)nedge / edge_s * tres;
+    elen = elen + (elen + f3norm(f3sub(p0, p1)) / edge_s * tres);
+    const float4 ua = Q[3 * (long long)u], ub4 = Q[3 * (long long)u + 1], uc = Q[3 * (long long)u + 2];
+    const float4 va = Q[3 * (long long)v], vb4 = Q[3 * (long long)v + 1], vc = Q[3 * (long long)v + 2];
+    const float M0 = ua.x + va.x, M1 = ua.y + va.y, M2 = ua.z + va.z, M3 = ua.w + va.w;
+    const float M4 = ub4.x + vb4.x, M5 = ub4.y + vb4.y, M6 = ub4.z + vb4.z, M7 = ub4.w + vb4.w;
+    const float M8 = uc.x + vc.x, M9 = uc.y + vc.y;
+    const float x0 = vm.x, x1 = vm.y, x2 = vm.z, x3 = 1.0f;
+    const float r0 = M0 * x0 + M1 * x1 + M2 * x2 + M3 * x3;
+    const float r1 = M1 * x0 + M4 * x1 + M5 * x2 + M6 * x3;
+    const float r2 = M2 * x0 + M5 * x1 + M7 * x2 + M8 * x3;
+    const float r3 = M3 * x0 + M6 * x1 + M8 * x2 + M9 * x3;
+    const float qc = (r0 * x0 + r1 * x1 + r2 * x2 + r3 * x3) / (edge_s * edge_s);
+    const float tot = qc + elen + qa / (float)ntri * tres;
+    cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+}
+
+// collapse_edge_kernel (cusimp.cu:511-585): an edge collapses when its cost
+// is at most collapse_t and its key is the minimum of every face of
+// ring(u) u ring(v) (propagate_k's per-vertex minima, as in
+// select_collapse_k); u moves to the midpoint, v is removed, faces holding
+// both are deleted and v's other faces take u.  Selected edges have disjoint
+// rings, so the pass is race free and order independent.
+__global__ void __launch_bounds__(kBlock) pasp_collapse_k(Mesh m, int* occ, const int* __restrict__ off,
+                                                          const int* __restrict__ ring, const int4* __restrict__ rv,
+                                                          const int* __restrict__ eu, const int* __restrict__ ev,
+                                                          int nE, const uint32_t* __restrict__ cost,
+                                                          const unsigned long long* __restrict__ vmin,
+                                                          uint32_t collapse_t) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const uint32_t c = cost[e];
+    if (c > collapse_t) return;
+    const unsigned long long key = edge_key(c, e);
+    const int u = eu[e], v = ev[e];
+    for (int side = 0; side < 2; ++side) {
+        const int w = side ? v : u;
+        for (int a = off[w]; a < off[w + 1]; ++a) {
+            const int4 t = rv[a];
+            unsigned long long mn = vmin[t.x];
+            const unsigned long long y = vmin[t.y], z = vmin[t.z];
+            mn = y < mn ? y : mn;
+            mn = z < mn ? z : mn;
+            if (mn != key) return;
+        }
+    }
+    const F3 vm = f3divs(f3add(ld(m, u), ld(m, v)), 2.0f);
+    st(m, u, vm);
+    st(m, v, F3{0.0f, 0.0f, 0.0f});
+    occ[v] = 0;
+    rewrite_rings(m, off, ring, u, v);
 }
 
 __global__ void compact_k(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out,

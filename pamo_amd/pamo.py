@@ -25,6 +25,15 @@ def _check_input(t: torch.Tensor, name: str):
         raise RuntimeError(f"{name} must be contiguous")
 
 
+def _check_faces(triangles: torch.Tensor, n_pts: int, name: str = "triangles"):
+    """Every face row is removed (all -1) or indexes the points; the kernels
+    read the points of every other row."""
+    t = triangles.reshape(-1, 3)
+    ok = (t == -1).all(1) | ((t >= 0) & (t < n_pts)).all(1)
+    if not bool(ok.all()):
+        raise RuntimeError(f"{name}: face indices outside [0, {n_pts}) (rows must be all -1 or valid)")
+
+
 class CUDSP_Free:
     """One collapse round per ``forward`` call (pybind.cpp:52-103).
 
@@ -97,6 +106,7 @@ def simplify_device(ctx: _hip.Context, verts: torch.Tensor, faces: torch.Tensor,
     _check_input(verts, "points")
     _check_input(faces, "triangles")
     nv, nf = int(verts.shape[0]), int(faces.shape[0])
+    _check_faces(faces, nv)
     out_v = torch.empty_like(verts)
     out_f = torch.empty_like(faces)
     onv, onf, rounds = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
@@ -182,3 +192,75 @@ class PaMO(torch.nn.Module):
         verts_np = v2.cpu().numpy() + tris_mean
         faces_np = f2.cpu().numpy()
         return verts_np, faces_np
+
+
+class CUDSP:
+    """The legacy per-round operator behind ``PaSP`` (cusimp::CUDSP,
+    pybind.cpp:117-183): ``forward(points, triangles, scale, threshold,
+    init)`` returns (points, triangles with removed rows -1, occupancy (n,1),
+    map (n,1)).  One object keeps its buffers across calls like the
+    reference's."""
+
+    def __init__(self, device: int | None = None):
+        if device is None:
+            device = torch.cuda.current_device()
+        self.device = device
+        self.ctx = _hip.Context(device)
+
+    def forward(self, points, triangles, scale, threshold, init):
+        _check_input(points, "points")
+        _check_input(triangles, "triangles")
+        if points.dtype != torch.float32:
+            raise RuntimeError("points type must match the sp class")
+        if triangles.dtype != torch.int32:
+            raise RuntimeError("triangles type must match the sp class")
+        n_pts, n_tris = int(points.shape[0]), int(triangles.shape[0])
+        _check_faces(triangles, n_pts)
+        dev = points.device
+        out_pts = torch.empty((n_pts, 3), dtype=torch.float32, device=dev)
+        out_tris = torch.empty((n_tris, 3), dtype=torch.int32, device=dev)
+        out_occ = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
+        out_map = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
+        torch.cuda.current_stream(dev).synchronize()
+        _hip.check(_hip.lib().pamo_pasp_round(
+            self.ctx.ptr, points.data_ptr(), n_pts, triangles.data_ptr(), n_tris, float(scale), float(threshold),
+            int(bool(init)), out_pts.data_ptr(), out_tris.data_ptr(), out_occ.data_ptr(), out_map.data_ptr()),
+            "pamo_pasp_round")
+        return out_pts, out_tris, out_occ, out_map
+
+    __call__ = forward
+
+
+class PaSP(torch.nn.Module):
+    """The legacy simplifier (pamo/__init__.py:208-253): rounds of midpoint
+    collapses of edges whose legacy cost is at most ``threshold``, chosen as
+    independent sets, until a round removes no face; no intersection guard.
+    ``run`` keeps the loop on the device (pamo_pasp_run) and returns CUDA
+    tensors (vertices, faces)."""
+
+    def __init__(self, device: int | None = None):
+        super().__init__()
+        if device is None:
+            device = torch.cuda.current_device()
+        self.ctx = _hip.Context(device)
+        self.rounds = 0
+
+    def run(self, points, triangles, threshold=0.001, iter=1000):
+        _check_input(points, "points")
+        _check_input(triangles, "triangles")
+        if points.dtype != torch.float32 or triangles.dtype != torch.int32:
+            raise RuntimeError("points must be float32 and triangles int32")
+        nv, nf = int(points.shape[0]), int(triangles.shape[0])
+        _check_faces(triangles, nv)
+        out_v = torch.empty_like(points)
+        out_f = torch.empty_like(triangles)
+        onv, onf, rounds = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        torch.cuda.current_stream(points.device).synchronize()
+        _hip.check(_hip.lib().pamo_pasp_run(
+            self.ctx.ptr, points.data_ptr(), nv, triangles.data_ptr(), nf, float(threshold), int(iter),
+            out_v.data_ptr(), out_f.data_ptr(), ctypes.byref(onv), ctypes.byref(onf), ctypes.byref(rounds)),
+            "pamo_pasp_run")
+        self.rounds = rounds.value
+        if self.rounds < int(iter):
+            print("Converged at iteration {}".format(self.rounds - 1))
+        return out_v[: onv.value], out_f[: onf.value]
