@@ -112,7 +112,8 @@ def bench_batch(args):
 
 def bench_serve(args):
     """A Poisson request stream served by pamo_serve_* with --slots slots,
-    against FIFO one-at-a-time service (one slot) of the same stream."""
+    against FIFO one-at-a-time service of the same stream, both in the plain
+    form (one PaMO.run per request) and through the engine with one slot."""
     import numpy as np
     import torch
     from pamo_amd import synth, _hip
@@ -154,10 +155,32 @@ def bench_serve(args):
         srv.close()
         return np.array(lat), np.array(qd), t_end, failed
 
+    def run_plain_fifo():
+        # the plain form: one PaMO object and one run per request, in arrival
+        # order (a request that arrives while another runs waits for it)
+        from pamo_amd.pamo import PaMO
+        lat, failed = [], 0
+        t0 = time.perf_counter()
+        for k, (ta, m, r, s3) in enumerate(jobs):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            try:
+                PaMO(pool[m], use_stage1=False, use_stage3=s3).run(dev[m][0], dev[m][1], r, 4, 1e-3, 100000, 0)
+            except Exception:
+                failed += 1
+            lat.append(time.perf_counter() - (t0 + ta))
+            if (k + 1) % 25 == 0:
+                print(f"[bench] plain fifo: {k + 1}/{len(jobs)} jobs done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        return np.array(lat), time.perf_counter() - t0, failed
+
     lat, qd, t_all, failed = run_stream(args.slots)
     print(f"[bench] served in {t_all:.1f}s", file=sys.stderr, flush=True)
     lat1, qd1, t1, failed1 = run_stream(1)
-    print(f"[bench] fifo in {t1:.1f}s", file=sys.stderr, flush=True)
+    print(f"[bench] one-slot fifo in {t1:.1f}s", file=sys.stderr, flush=True)
+    latp, tp, failedp = run_plain_fifo()
+    print(f"[bench] plain fifo in {tp:.1f}s", file=sys.stderr, flush=True)
     n = len(jobs)
     res = {"metric": "served meshes/s and p50/p99 latency (stages 2+3 on half the jobs)",
            "value": round(n / t_all, 3), "unit": "meshes/s", "n_gpus": 1, "higher_is_better": True,
@@ -167,11 +190,15 @@ def bench_serve(args):
                       "slots": args.slots},
            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+           "plain_fifo_meshes_per_s": round(n / tp, 3),
+           "plain_fifo_p50_latency_s": round(float(np.percentile(latp, 50)), 3),
+           "plain_fifo_p99_latency_s": round(float(np.percentile(latp, 99)), 3),
+           "speedup_vs_plain_fifo": round(tp / t_all, 3),
            "fifo_meshes_per_s": round(n / t1, 3),
            "fifo_p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
            "fifo_p99_latency_s": round(float(np.percentile(lat1, 99)), 3),
            "speedup_vs_fifo": round(t1 / t_all, 3),
-           "failed_jobs": failed, "fifo_failed_jobs": len(failed1)}
+           "failed_jobs": failed, "fifo_failed_jobs": len(failed1), "plain_fifo_failed_jobs": failedp}
     print(json.dumps(res))
 
 

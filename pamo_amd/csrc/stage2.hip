@@ -539,8 +539,9 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     LAUNCH(remove_invalid_k, nT, m, nT);
     LAUNCHP(K_LINE_FLAG, 8.0 * nE + 16.0 * nP + 4.0 * nE, line_flag_k, nE + 1, m, c.eu, c.ev, nE, c.esel);
     PAMO_TRY(scan_exclusive(c.esel, c.pflag, nE + 1, c.ws, c.stream));
+    LAUNCH(line_list_k, nE, c.esel, c.pflag, nE, c.recs, c.dcnt);
     hipLaunchKernelGGL(line_apply_k, dim3(1), dim3(1), 0, c.stream, m, c.occ, c.ring_off, c.ring, c.eu, c.ev,
-                       c.esel, c.pflag, nE, c.recs, c.dcnt);
+                       c.pflag, nE, c.recs, c.dcnt);
     PAMO_HIP(hipGetLastError());
     PAMO_HIP(hipMemsetAsync(c.changed, 0, nT, c.stream));
     LAUNCHP(K_MARK_CHANGED, 1.0 * nT, mark_changed_k, nE, c.recs, c.dcnt, c.ring_off, c.ring, c.eu, c.ev,

@@ -657,21 +657,28 @@ __global__ void line_flag_k(Mesh m, const int* __restrict__ eu, const int* __res
     flag[e] = (p.x == q.x && p.y == q.y && p.z == q.z) ? 1 : 0;
 }
 
+// The flagged line edges in edge order, appended to the records after the
+// regular collapses (recs[cnt[C_NREG] + pos[e]] = e).
+__global__ void line_list_k(const int* __restrict__ flag, const int* __restrict__ pos, int nE,
+                            int* __restrict__ recs, const int* __restrict__ cnt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE || !flag[e]) return;
+    recs[cnt[C_NREG] + pos[e]] = e;
+}
+
 // Line collapses are rare; applying them in edge order in one lane keeps
-// the result deterministic where the reference races on shared faces.
+// the result deterministic where the reference races on shared faces.  The
+// lane walks the list line_list_k built, not the whole edge array.
 __global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
-                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ flag,
-                             const int* __restrict__ pos, int nE, int* __restrict__ recs, int* cnt) {
+                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ pos,
+                             int nE, const int* __restrict__ recs, int* cnt) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int base = cnt[C_NREG];
     const int nl = pos[nE];
     cnt[C_LINE] = nl;
-    if (nl == 0) return;
-    int w = 0;
-    for (int e = 0; e < nE && w < nl; ++e) {
-        if (!flag[e]) continue;
-        int u = eu[e], v = ev[e];
-        recs[base + w++] = e;
+    for (int w = 0; w < nl; ++w) {
+        const int e = recs[base + w];
+        const int u = eu[e], v = ev[e];
         st(m, v, F3{0.0f, 0.0f, 0.0f});
         occ[v] = 0;
         rewrite_rings(m, off, ring, u, v);
