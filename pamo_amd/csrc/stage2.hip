@@ -399,7 +399,11 @@ static int run_guard(Ctx& c, int nT) {
     LAUNCH(query_flag_k, nT + 1, c.bval, c.dcnt, c.changed, nT, c.face_scan);
     PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
-    const long long guard_threads = (((long long)nT + kGuardQ - 1) / kGuardQ) * kWave;
+    // meshes up to 256k faces split every query group over 8 subtrees: late
+    // rounds hold few queries, and one query with a large box would otherwise
+    // hold a whole launch in one wave
+    const int part_log = nT <= (1 << 18) ? 3 : 0;
+    const long long guard_threads = ((((long long)nT + kGuardQ - 1) / kGuardQ) << part_log) * kWave;
     const double guard_bytes = 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT;
     Undecided u{nullptr, c.dcnt + C_UNDECIDED, 0, c.exact_guard == 2 ? 1 : 0};
     if (c.exact_guard) {
@@ -414,16 +418,16 @@ static int run_guard(Ctx& c, int nT) {
         u.cap = (int)std::min<size_t>(c.alloc_ulist, 0x7fffffff);
         PAMO_HIP(hipMemsetAsync(c.dcnt + C_UNDECIDED, 0, sizeof(int), c.stream));
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, true>), guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
         hipLaunchKernelGGL(exact_pairs_k, dim3(1024), dim3(kBlock), 0, c.stream, m, (const int2*)c.ulist,
                            (const int*)(c.dcnt + C_UNDECIDED), u.cap, c.flagged);
         PAMO_HIP(hipGetLastError());
     } else if (c.guard_small_stack) {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStackSmall, false>), guard_threads, m, c.qlist,
-                c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+                c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
     } else {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, false>), guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u);
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
     }
     return 0;
 }

@@ -1039,6 +1039,21 @@ __device__ __forceinline__ int subtree_start(const float4* __restrict__ node, in
     return cur;
 }
 
+// subtree_start with a run-time depth: the root of subtree k of 2^log (k's
+// bits choose left / right from the root down), -1 when that subtree does not
+// exist because a leaf is reached above depth log (the leaf then belongs to
+// the subtree whose remaining bits are zero).  The subtrees partition the
+// leaves.
+__device__ __forceinline__ int subtree_root(const float4* __restrict__ node, int n_leaves, int k, int log) {
+    int cur = 0;
+    for (int d = log - 1; d >= 0; --d) {
+        if (cur >= n_leaves - 1) return (k & ((2 << d) - 1)) != 0 ? -1 : cur;
+        const int left = __float_as_int(node[2 * (long long)cur].w);
+        cur = ((k >> d) & 1) ? __float_as_int(node[2 * (long long)left + 1].w) : left;
+    }
+    return cur;
+}
+
 __device__ __forceinline__ bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
     if (ahi.x < blo.x || bhi.x < alo.x) return false;
     if (ahi.y < blo.y || bhi.y < alo.y) return false;
@@ -1224,24 +1239,30 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
                                                         const float4* __restrict__ fbox,
                                                         const float4* __restrict__ node,
                                                         const int* __restrict__ right, float epsilon,
-                                                        uint8_t* __restrict__ flagged, Undecided u) {
+                                                        uint8_t* __restrict__ flagged, Undecided u, int part_log) {
     constexpr int kW = kBlock / kWave;
     __shared__ int2 stk[kW][kStackCap];
     __shared__ int2 pq[kW][kQueue];
     __shared__ float4 qbox[kW][kGuardQ][2];
     __shared__ int4 qid[kW][kGuardQ];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const long long q0 = ((long long)blockIdx.x * kW + w) * kGuardQ;
+    // wave -> (query group, subtree): the 2^part_log waves of a group each
+    // walk one subtree, so one heavy query spreads over several waves
+    const long long wid = (long long)blockIdx.x * kW + w;
+    const long long q0 = (wid >> part_log) * kGuardQ;
+    const int part = (int)(wid & ((1 << part_log) - 1));
     const int nq = cnt[C_NQUERY];
     const int n = cnt[C_NLIVE];
     if (q0 >= nq || n <= 1) return;  // uniform per wave
+    const int root = subtree_root(node, n, part, part_log);
+    if (root < 0) return;
     const int nl = nq - q0 < kGuardQ ? (int)(nq - q0) : kGuardQ;
     if (lane < nl) {
         const int a = list[q0 + lane];
         qid[w][lane] = make_int4(a, m.i[a], m.j[a], m.k[a]);
         qbox[w][lane][0] = fbox[2 * (long long)a];
         qbox[w][lane][1] = fbox[2 * (long long)a + 1];
-        stk[w][lane] = make_int2(lane, 0);  // root
+        stk[w][lane] = make_int2(lane, root);
     }
     wave_sync();
     int size = nl, qn = 0;  // wave-uniform
