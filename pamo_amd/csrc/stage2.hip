@@ -162,7 +162,7 @@ static int dalloc(T** q, size_t n, hipStream_t st) {
 }
 
 static int ensure_parts(Ctx& c, size_t n_items) {
-    size_t need = 6 * (n_items / 256 + 4);
+    size_t need = 7 * (n_items / 256 + 4);  // 6 floats per block, the whole box, a live count per block
     if (need > c.alloc_parts) {
         c.alloc_parts = need;
         PAMO_TRY(dalloc(&c.box_part, need, c.stream));
@@ -367,11 +367,12 @@ static int sync_counters(Ctx& c) {
 static int run_guard(Ctx& c, int nT) {
     Mesh m = cur_mesh(c);
     const double nP = c.n_pts;
-    PAMO_HIP(hipMemsetAsync(c.flagged, 0, nT, c.stream));
-    PAMO_HIP(hipMemsetAsync(c.dcnt + C_NLIVE, 0, sizeof(int), c.stream));
     const int nparts = (int)grid_for(nT, kBlock);
-    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT, face_box_k, nT, m, nT, c.fbox, c.box_part, c.dcnt);
-    hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, c.stream, c.box_part, nparts);
+    int* live_part = (int*)(c.box_part + 6 * nparts + 6);
+    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT, face_box_k, nT, m, nT, c.fbox, c.box_part, live_part,
+            c.flagged);
+    hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, c.stream, c.box_part, nparts, (const int*)live_part,
+                       c.dcnt);
     PAMO_HIP(hipGetLastError());
     LAUNCHP(K_MORTON, 4.0 * nT + 32.0 * nT + 8.0 * nT, morton_k, nT, m, nT, (const float4*)c.fbox,
             (const float*)(c.box_part + 6 * nparts), c.bkey, c.bval);
@@ -449,9 +450,8 @@ static int undo_candidates(Ctx& c, int& npushed) {
 // Vertex -> face rings in ascending face order (:276-304), the edge list
 // (:306-335), the quadrics (:337-366) and the edge-cost launch order; sets
 // c.n_edges.  Shared by the round and the legacy round.
+// ring_off and ring_fill must hold zeros (round_init_k).
 static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
-    PAMO_HIP(hipMemsetAsync(c.ring_off, 0, sizeof(int) * (nP + 1), c.stream));
-    PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
     LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP, ring_count_k, nT, m, nT, c.ring_off);
     PAMO_TRY(scan_exclusive(c.ring_off, c.ring_off, nP + 1, c.ws, c.stream));
     LAUNCHP(K_RING_FILL, 12.0 * nT + 12.0 * nT + 32.0 * nT, ring_fill_k, nT, m, nT, c.ring_off, c.ring_fill,
@@ -494,13 +494,12 @@ static int pasp_round_device(Ctx& c, int nP, int nT, float scale, float thr) {
     Mesh m = cur_mesh(c);
     c.n_pts = nP;
     c.n_tris = nT;
-    PAMO_TRY(fill(c.occ, 1, nP, c.stream));
-    PAMO_TRY(fill(c.occ + nP, 0, 1, c.stream));
+    LAUNCH(round_init_k, std::max(nP + 1, nT), m, Mesh{nullptr, nullptr, nullptr, nullptr}, nP, nT, c.occ, c.ring_off,
+           c.ring_fill, (unsigned long long*)c.vmin, (uint8_t*)nullptr);
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 4.0 * nE, pasp_cost_k, nE, m,
             c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.Q, scale, thr, c.edge_cost);
-    PAMO_TRY(fill<unsigned long long>((unsigned long long*)c.vmin, ~0ull, nP, c.stream));
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
             (unsigned long long*)c.vmin);
     LAUNCHP(K_SELECT_COLLAPSE, 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP, pasp_collapse_k, nE, m, c.occ,
@@ -516,20 +515,15 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     Mesh m = cur_mesh(c);
     c.n_pts = nP;
     c.n_tris = nT;
-    // snapshot for undo
-    PAMO_HIP(hipMemcpyAsync(c.op, c.p, sizeof(float4) * nP, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.oi, c.ti, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.oj, c.tj, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.ok, c.tk, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
-    PAMO_TRY(fill(c.occ, 1, nP, c.stream));
-    PAMO_TRY(fill(c.occ + nP, 0, 1, c.stream));
+    // snapshot for undo, occupancy, ring counters, minimum keys, changed flags
+    LAUNCH(round_init_k, std::max(nP + 1, nT), m, Mesh{c.op, c.oi, c.oj, c.ok}, nP, nT, c.occ, c.ring_off, c.ring_fill,
+           (unsigned long long*)c.vmin, c.changed);
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
     LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE,
             edge_cost_k, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid, c.Q, cp, c.edge_cost);
     // independent set + collapse
-    PAMO_TRY(fill<unsigned long long>((unsigned long long*)c.vmin, ~0ull, nP, c.stream));
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
             (unsigned long long*)c.vmin);
     LAUNCHP(K_SELECT_COLLAPSE, 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP, select_collapse_k, nE + 1, m,
@@ -547,7 +541,6 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     hipLaunchKernelGGL(line_apply_k, dim3(1), dim3(1), 0, c.stream, m, c.occ, c.ring_off, c.ring, c.eu, c.ev,
                        c.pflag, nE, c.recs, c.dcnt);
     PAMO_HIP(hipGetLastError());
-    PAMO_HIP(hipMemsetAsync(c.changed, 0, nT, c.stream));
     LAUNCHP(K_MARK_CHANGED, 1.0 * nT, mark_changed_k, nE, c.recs, c.dcnt, c.ring_off, c.ring, c.eu, c.ev,
             c.changed);
     PAMO_TRY(sync_counters(c));

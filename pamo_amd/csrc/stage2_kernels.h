@@ -735,14 +735,21 @@ __device__ __forceinline__ void block_minmax6(float lo[3], float hi[3], float (*
 }
 
 // Face boxes + per-block partial whole box + live count.
+// Face boxes, per-block partial bounding boxes and live-face counts, and
+// the guard's flags cleared (flagged[f] = 0).
 __global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __restrict__ fbox,
-                                                     float* __restrict__ part, int* cnt) {
+                                                     float* __restrict__ part, int* __restrict__ live_part,
+                                                     uint8_t* __restrict__ flagged) {
     __shared__ float red[6][kBlock];
+    __shared__ int nlive;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     const float inf = __builtin_huge_valf();
     float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
     bool live = false;
+    if (threadIdx.x == 0) nlive = 0;
+    __syncthreads();
     if (f < nT) {
+        flagged[f] = 0;
         int a = m.i[f];
         if (a != -1) {
             live = true;
@@ -758,9 +765,11 @@ __global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __r
         }
     }
     uint64_t lm = __ballot(live);
-    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&cnt[C_NLIVE], (int)__popcll(lm));
+    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&nlive, (int)__popcll(lm));
     block_minmax6(lo, hi, red);
+    __syncthreads();
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 0) live_part[blockIdx.x] = nlive;
 }
 
 // Per-block partial bounding box of the vertices.
@@ -779,17 +788,57 @@ __global__ void __launch_bounds__(kBlock) vert_box_k(const float4* __restrict__ 
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(kBlock) whole_box_k(float* __restrict__ part, int nparts) {
+// Whole box from the partials (written after them); with live_part, also
+// cnt[C_NLIVE] = the sum of the per-block live counts.
+__global__ void __launch_bounds__(kBlock) whole_box_k(float* __restrict__ part, int nparts,
+                                                      const int* __restrict__ live_part = nullptr,
+                                                      int* __restrict__ cnt = nullptr) {
     __shared__ float red[6][kBlock];
+    __shared__ int nlive;
     const float inf = __builtin_huge_valf();
     float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
-    for (int q = threadIdx.x; q < nparts; q += kBlock)
+    if (threadIdx.x == 0) nlive = 0;
+    __syncthreads();
+    int my = 0;
+    for (int q = threadIdx.x; q < nparts; q += kBlock) {
         for (int d = 0; d < 3; ++d) {
             lo[d] = fminf(lo[d], part[6 * q + d]);
             hi[d] = fmaxf(hi[d], part[6 * q + 3 + d]);
         }
+        if (live_part) my += live_part[q];
+    }
+    if (live_part && my) atomicAdd(&nlive, my);
     block_minmax6(lo, hi, red);
+    __syncthreads();
     if (threadIdx.x < 6) part[6 * nparts + threadIdx.x] = red[threadIdx.x][0];  // whole box after the partials
+    if (live_part && threadIdx.x == 0) cnt[C_NLIVE] = nlive;
+}
+
+// Start-of-round state in one pass: the undo snapshot of points and faces
+// (when snap.p is set), occupancy 1 (0 at nP), zeroed ring counters and
+// changed flags (when given), per-vertex minimum keys at the maximum.
+__global__ void round_init_k(Mesh m, Mesh snap, int nP, int nT, int* __restrict__ occ, int* __restrict__ ring_off,
+                             int* __restrict__ ring_fill, unsigned long long* __restrict__ vmin,
+                             uint8_t* __restrict__ changed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nP) {
+        if (snap.p) snap.p[i] = m.p[i];
+        occ[i] = 1;
+        vmin[i] = ~0ull;
+    }
+    if (i == nP) occ[nP] = 0;
+    if (i <= nP) {
+        ring_off[i] = 0;
+        ring_fill[i] = 0;
+    }
+    if (i < nT) {
+        if (snap.p) {
+            snap.i[i] = m.i[i];
+            snap.j[i] = m.j[i];
+            snap.k[i] = m.k[i];
+        }
+        if (changed) changed[i] = 0;
+    }
 }
 
 __global__ void morton_k(Mesh m, int nT, const float4* __restrict__ fbox, const float* __restrict__ whole,
