@@ -109,6 +109,73 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles_prefix_k(const int* in,
     }
 }
 
+// One workgroup of 1024 threads scans up to kBlockScanMax items in one
+// launch: each thread sums a contiguous run of `per` items (four at a time
+// with 16-byte loads when both arrays are 16-byte aligned), the 16 waves scan
+// the run sums, and each thread writes its run from its offset.  Replaces the
+// two launches of the tiled scan for mid-size arrays, whose cost is mostly
+// the second launch and the wait between the two.  Measured on the 2M-face
+// workload: up to 16k items it is slightly faster than two launches; at 64k
+// the serial runs make it slower.
+constexpr int kBlockScan = 1024;
+constexpr long long kBlockScanMax = 1 << 14;
+
+__global__ void __launch_bounds__(kBlockScan) scan_block_k(const int* in, int* out, int n, int per, int vec) {
+    __shared__ int lds[kBlockScan / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = threadIdx.x * per;
+    const int e = b + per < n ? b + per : n;
+    int s = 0;
+    if (vec) {
+        for (int i = b; i < e; i += 4) {
+            if (i + 4 <= e) {
+                const int4 q = *reinterpret_cast<const int4*>(in + i);
+                s += q.x + q.y + q.z + q.w;
+            } else {
+                for (int k = i; k < e; ++k) s += in[k];
+            }
+        }
+    } else {
+        for (int i = b; i < e; ++i) s += in[i];
+    }
+    int x = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    int ex = x - s;
+    for (int i = 0; i < w; ++i) ex += lds[i];
+    if (vec) {
+        for (int i = b; i < e; i += 4) {
+            if (i + 4 <= e) {
+                const int4 q = *reinterpret_cast<const int4*>(in + i);
+                int4 o;
+                o.x = ex;
+                o.y = o.x + q.x;
+                o.z = o.y + q.y;
+                o.w = o.z + q.z;
+                ex = o.w + q.w;
+                *reinterpret_cast<int4*>(out + i) = o;
+            } else {
+                for (int k = i; k < e; ++k) {
+                    const int v = in[k];
+                    out[k] = ex;
+                    ex += v;
+                }
+            }
+        }
+    } else {
+        for (int i = b; i < e; ++i) {
+            const int v = in[i];
+            out[i] = ex;
+            ex += v;
+        }
+    }
+}
+
 // Workspace (ints) needed by scan_exclusive for n elements.
 inline long long scan_workspace(long long n) { return (n + kScanTile - 1) / kScanTile + 16; }
 
@@ -120,6 +187,16 @@ inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStre
     if (nb == 1) {
         prof_begin(K_SCAN_TILES, 8.0 * n, st);
         hipLaunchKernelGGL(scan_tiles_k, dim3(1), dim3(kScanBlock), 0, st, in, out, (int*)nullptr, n);
+        prof_end(K_SCAN_TILES, st);
+        PAMO_HIP(hipGetLastError());
+        return 0;
+    }
+    if (n <= kBlockScanMax) {
+        // runs of a multiple of 4 items, so that aligned arrays stay aligned per thread
+        const int per = (int)(((n + kBlockScan - 1) / kBlockScan + 3) & ~3LL);
+        const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0 ? 1 : 0;
+        prof_begin(K_SCAN_TILES, 8.0 * n, st);
+        hipLaunchKernelGGL(scan_block_k, dim3(1), dim3(kBlockScan), 0, st, in, out, (int)n, per, vec);
         prof_end(K_SCAN_TILES, st);
         PAMO_HIP(hipGetLastError());
         return 0;
