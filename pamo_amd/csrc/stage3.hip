@@ -798,14 +798,10 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         V3 g[4];
         if (tp.ctype1[i] == BLOCK_PT) pt_pair_grad(x0, x1, x2, x3, ctype2[i], g);
         else ee_pair_grad(x0, x1, x2, x3, ctype2[i], g);
-        float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
-        float c2 = barrier_d2(d, P.d_hat) * P.kappa;
-        const int ids[4] = {k.x, k.y, k.z, k.w};
-        for (int a = 0; a < 4; ++a) {
-            cdd[4 * j + a] = f4(g[a]);
-            atomic_add3(grad, ids[a], g[a] * c1);
-            atomic_add3(diag, ids[a], c2 * mk(g[a].x * g[a].x, g[a].y * g[a].y, g[a].z * g[a].z));
-        }
+        (void)d;
+        // the gradient and diagonal terms are gathered per vertex by
+        // contact_diff_gather_k once the contact incidence is built
+        for (int a = 0; a < 4; ++a) cdd[4 * j + a] = f4(g[a]);
     }
 }
 
@@ -956,6 +952,41 @@ struct Incid {
 // butterfly), so a vertex with many incident contacts does not hold up a
 // whole wave.
 constexpr int kGather = 16;
+
+// Contact terms of the gradient (barrier' kappa coeff g) and of the Jacobi
+// diagonal (barrier'' kappa g*g) per vertex, summed over its active-contact
+// corners in the incidence order by kGather lanes and a fixed shuffle tree,
+// instead of float atomics per corner.  Runs after diff_k (same stream).
+__global__ void __launch_bounds__(kB) contact_diff_gather_k(int nV, Params P, float coeff, Incid ic,
+                                                            const float* __restrict__ cd, const int* __restrict__ act,
+                                                            float4* __restrict__ grad, float4* __restrict__ diag) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = (int)(t / kGather), g = (int)(t % kGather);
+    V3 ga = mk(0.0f, 0.0f, 0.0f), da = ga;
+    if (v < nV) {
+        for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
+            const float d = cd[act[ic.cj[e]]];
+            const float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
+            const float c2 = barrier_d2(d, P.d_hat) * P.kappa;
+            const V3 gg = v3(ic.cdv[e]);
+            ga = ga + gg * c1;
+            da = da + c2 * mk(gg.x * gg.x, gg.y * gg.y, gg.z * gg.z);
+        }
+    }
+#pragma unroll
+    for (int m = kGather / 2; m > 0; m >>= 1) {
+        ga.x += __shfl_xor(ga.x, m, kGather);
+        ga.y += __shfl_xor(ga.y, m, kGather);
+        ga.z += __shfl_xor(ga.z, m, kGather);
+        da.x += __shfl_xor(da.x, m, kGather);
+        da.y += __shfl_xor(da.y, m, kGather);
+        da.z += __shfl_xor(da.z, m, kGather);
+    }
+    if (v < nV && g == 0 && ic.vc_off[v + 1] > ic.vc_off[v]) {
+        grad[v] = f4(v3(grad[v]) + ga);
+        diag[v] = f4(v3(diag[v]) + da);
+    }
+}
 
 __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2gt, int use_elem, int use_hinge,
                                                    int use_contact, int use_dense, const float* __restrict__ vor,
@@ -1310,6 +1341,9 @@ static int compute_diff(Ctx& c, int nC, int mask) {
         L3(contact_keys_k, n, (const int4*)c.cidx, (const int*)c.act, (const int*)c.n_act, c.skey, c.sval);
         PAMO_TRY(finish_incidence(c, n, c.vc_off, c.vc_ent));
         L3(contact_permute_k, n, (const int*)c.vc_ent, (int)n, (const float4*)c.cdd, c.cdv, c.cj);
+        Incid ic{c.vf_off, c.vh_off, c.vc_off, (const float4*)c.cdv, (const int*)c.cj};
+        L3(contact_diff_gather_k, (long long)c.nV * kGather, c.nV, c.P, -1.0f, ic, (const float*)c.cd,
+           (const int*)c.act, c.grad, c.diag);
     } else {
         PAMO_TRY(fill(c.vc_off, 0, c.nV + 1, c.stream));
     }
