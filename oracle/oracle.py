@@ -57,6 +57,8 @@ def lib():
         L.orc_guard_pairs.argtypes = [c_vp]
         L.orc_lbvh.restype = c_int
         L.orc_lbvh.argtypes = [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]
+        L.orc_karras.restype = None
+        L.orc_karras.argtypes = [c_vp, c_vp, c_int, c_vp, c_vp]
         L.orc_candidates.restype = c_ll
         L.orc_candidates.argtypes = [c_vp, c_int, c_vp, c_int, c_vp, c_ll]
         L.orc_ref_guard.restype = c_int
@@ -178,6 +180,17 @@ def lbvh(points, triangles):
     m = lib().orc_lbvh(P.ctypes.data, len(P), T.ctypes.data, n, codes.ctypes.data, faces.ctypes.data,
                        left.ctypes.data, right.ctypes.data)
     return codes[:m], faces[:m], left[:max(m - 1, 0)], right[:max(m - 1, 0)]
+
+
+def karras(codes, faces):
+    """Karras children (left, right) of leaves with sorted keys (code << 32 | face)."""
+    codes = np.ascontiguousarray(codes, np.uint32)
+    faces = np.ascontiguousarray(faces, np.int32)
+    n = len(codes)
+    left = np.empty(max(n - 1, 1), np.int32)
+    right = np.empty(max(n - 1, 1), np.int32)
+    lib().orc_karras(codes.ctypes.data, faces.ctypes.data, n, left.ctypes.data, right.ctypes.data)
+    return left[:max(n - 1, 0)], right[:max(n - 1, 0)]
 
 
 def candidates(points, triangles, cap=None):

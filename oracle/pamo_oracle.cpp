@@ -1226,6 +1226,14 @@ int orc_lbvh(const float* pts, int nP, const int* tris, int nT, unsigned int* co
     return b.n;
 }
 
+// Karras children of the n leaves with sorted keys (codes[s] << 32 | faces[s])
+// (the hierarchy step of build_lbvh, on keys given by the caller).
+void orc_karras(const unsigned int* codes, const int* faces, int n, int* left, int* right) {
+    std::vector<uint64_t> k(n);
+    for (int s = 0; s < n; ++s) k[s] = ((uint64_t)codes[s] << 32) | (uint64_t)(uint32_t)faces[s];
+    for (int i = 0; i + 1 < n; ++i) orc::karras_node(k, n, i, left[i], right[i]);
+}
+
 // Candidate pairs a<b (live, closed-AABB overlap) by LBVH traversal.  Returns
 // the count; writes up to cap pairs.
 long long orc_candidates(const float* pts, int nP, const int* tris, int nT, int* pairs, long long cap) {

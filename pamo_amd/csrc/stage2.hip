@@ -361,24 +361,32 @@ static int sync_counters(Ctx& c) {
     return 0;
 }
 
-// Build the LBVH over live faces and flag the faces of intersecting pairs
-// that involve a changed face (self_intersect.cuh:151-392, restricted to
-// the faces that can change an undo decision).
-static int run_guard(Ctx& c, int nT) {
+// Flag the faces of intersecting pairs that involve a changed face
+// (self_intersect.cuh:151-392, restricted to the faces that can change an
+// undo decision), over an LBVH of the faces live at the start of the round.
+// The first guard of a round builds the tree (Morton codes, sort, Karras
+// hierarchy); a guard after an undo only recomputes the face and node boxes
+// over the same hierarchy (refit): the undo moves vertices and restores
+// faces, and every face it can restore is already a leaf, with an empty box
+// while it is removed.  Which pairs overlap does not depend on the tree's
+// shape, so the flags are those of a tree built afresh.
+static int run_guard(Ctx& c, int nT, bool refit) {
     Mesh m = cur_mesh(c);
     const double nP = c.n_pts;
     const int nparts = (int)grid_for(nT, kBlock);
-    int* live_part = (int*)(c.box_part + 6 * nparts + 6);
-    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT, face_box_k, nT, m, nT, c.fbox, c.box_part, live_part,
-            c.flagged);
-    hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, c.stream, c.box_part, nparts, (const int*)live_part,
-                       c.dcnt);
-    PAMO_HIP(hipGetLastError());
-    LAUNCHP(K_MORTON, 4.0 * nT + 32.0 * nT + 8.0 * nT, morton_k, nT, m, nT, (const float4*)c.fbox,
-            (const float*)(c.box_part + 6 * nparts), c.bkey, c.bval);
-    PAMO_TRY(radix_sort_pairs(c.bkey, c.bval, c.bkey2, c.bval2, nT, 32, c.ws, c.stream));
-    LAUNCHP(K_KARRAS, 8.0 * nT + 16.0 * nT + 8.0 * nT, karras_k, nT, c.bkey, c.bval, c.dcnt, c.bleft, c.bright,
-            c.bparent, c.brange);
+    int* leaf_part = (int*)(c.box_part + 6 * nparts + 6);
+    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT + 4.0 * nT, face_box_k, nT, m, nT, c.fbox, c.box_part,
+            leaf_part, c.flagged, (const int*)c.oi);
+    if (!refit) {
+        hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, c.stream, c.box_part, nparts,
+                           (const int*)leaf_part, c.dcnt);
+        PAMO_HIP(hipGetLastError());
+        LAUNCHP(K_MORTON, 4.0 * nT + 32.0 * nT + 8.0 * nT, morton_k, nT, m, orig_mesh(c), nT, (const float4*)c.fbox,
+                (const float*)(c.box_part + 6 * nparts), c.bkey, c.bval);
+        PAMO_TRY(radix_sort_pairs(c.bkey, c.bval, c.bkey2, c.bval2, nT, 32, c.ws, c.stream));
+        LAUNCHP(K_KARRAS, 8.0 * nT + 16.0 * nT + 8.0 * nT, karras_k, nT, c.bkey, c.bval, c.dcnt, c.bleft, c.bright,
+                c.bparent, c.brange);
+    }
     // node boxes from leaf ranges: no communication between workgroups
     const unsigned nb0 = grid_for(nT, kBlock), nb1 = grid_for(nb0, kBlock);
     prof_begin(K_BVH_BOXES, 4.0 * nT + 32.0 * nT + 32.0 * nT + 64.0 * nT, c.stream);
@@ -397,7 +405,7 @@ static int run_guard(Ctx& c, int nT) {
     prof_end(K_BVH_BOXES, c.stream);
     PAMO_HIP(hipGetLastError());
     // queries: changed live faces in Morton order
-    LAUNCH(query_flag_k, nT + 1, c.bval, c.dcnt, c.changed, nT, c.face_scan);
+    LAUNCH(query_flag_k, nT + 1, c.bval, c.dcnt, c.changed, (const int*)c.ti, nT, c.face_scan);
     PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
     LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
     // meshes up to 256k faces split every query group over 8 subtrees: late
@@ -551,7 +559,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     int npushed = 0;
     c.undo_iters = 0;
     if (c.n_rec > 0) {
-        PAMO_TRY(run_guard(c, nT));
+        PAMO_TRY(run_guard(c, nT, false));
         PAMO_TRY(undo_candidates(c, npushed));
     }
     c.n_vertices_undo += 2 * npushed;
@@ -564,7 +572,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
         LAUNCH(rearrange_k, c.n_vertices_undo - first, c.tmp_undo, c.n_vertices_undo - first, c.pts_map,
                c.undo_list, first);
         first += 2 * npushed;
-        PAMO_TRY(run_guard(c, nT));
+        PAMO_TRY(run_guard(c, nT, true));
         PAMO_TRY(undo_candidates(c, npushed));
         c.n_vertices_undo += 2 * npushed;
         if (it == 5) break;

@@ -60,7 +60,7 @@ struct Feistel {
 enum Counter {
     C_LINE = 0,     // line-collapse count
     C_NREG = 1,     // regular collapse records
-    C_NLIVE = 2,    // live faces (BVH leaves)
+    C_NLIVE = 2,    // guard tree leaves (faces live at the start of the round)
     C_PUSHED = 3,   // undo candidates
     C_NV = 4,       // next-round vertices
     C_NF = 5,       // next-round faces
@@ -734,25 +734,26 @@ __device__ __forceinline__ void block_minmax6(float lo[3], float hi[3], float (*
     }
 }
 
-// Face boxes + per-block partial whole box + live count.
-// Face boxes, per-block partial bounding boxes and live-face counts, and
-// the guard's flags cleared (flagged[f] = 0).
+// Face boxes (an empty box, lo = +inf and hi = -inf, for a removed face),
+// per-block partial bounding boxes of the live faces, per-block counts of
+// the guard tree's leaves, and the guard's flags cleared (flagged[f] = 0).
+// The leaves are the faces live at the start of the round (snap_i[f] != -1):
+// every face live after the collapse, and every face an undo can restore.
 __global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __restrict__ fbox,
-                                                     float* __restrict__ part, int* __restrict__ live_part,
-                                                     uint8_t* __restrict__ flagged) {
+                                                     float* __restrict__ part, int* __restrict__ leaf_part,
+                                                     uint8_t* __restrict__ flagged, const int* __restrict__ snap_i) {
     __shared__ float red[6][kBlock];
-    __shared__ int nlive;
+    __shared__ int nleaf;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     const float inf = __builtin_huge_valf();
     float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
-    bool live = false;
-    if (threadIdx.x == 0) nlive = 0;
+    bool leaf = false;
+    if (threadIdx.x == 0) nleaf = 0;
     __syncthreads();
     if (f < nT) {
         flagged[f] = 0;
         int a = m.i[f];
         if (a != -1) {
-            live = true;
             F3 p = ld(m, a), q = ld(m, m.j[f]), r = ld(m, m.k[f]);
             lo[0] = fminf(p.x, fminf(q.x, r.x));
             lo[1] = fminf(p.y, fminf(q.y, r.y));
@@ -760,16 +761,17 @@ __global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __r
             hi[0] = fmaxf(p.x, fmaxf(q.x, r.x));
             hi[1] = fmaxf(p.y, fmaxf(q.y, r.y));
             hi[2] = fmaxf(p.z, fmaxf(q.z, r.z));
-            fbox[2 * (long long)f] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-            fbox[2 * (long long)f + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
         }
+        leaf = a != -1 || snap_i[f] != -1;
+        fbox[2 * (long long)f] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        fbox[2 * (long long)f + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
     }
-    uint64_t lm = __ballot(live);
-    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&nlive, (int)__popcll(lm));
+    uint64_t lm = __ballot(leaf);
+    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&nleaf, (int)__popcll(lm));
     block_minmax6(lo, hi, red);
     __syncthreads();
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
-    if (threadIdx.x == 0) live_part[blockIdx.x] = nlive;
+    if (threadIdx.x == 0) leaf_part[blockIdx.x] = nleaf;
 }
 
 // Per-block partial bounding box of the vertices.
@@ -789,7 +791,7 @@ __global__ void __launch_bounds__(kBlock) vert_box_k(const float4* __restrict__ 
 }
 
 // Whole box from the partials (written after them); with live_part, also
-// cnt[C_NLIVE] = the sum of the per-block live counts.
+// cnt[C_NLIVE] = the sum of the per-block counts (the guard tree's leaves).
 __global__ void __launch_bounds__(kBlock) whole_box_k(float* __restrict__ part, int nparts,
                                                       const int* __restrict__ live_part = nullptr,
                                                       int* __restrict__ cnt = nullptr) {
@@ -841,17 +843,33 @@ __global__ void round_init_k(Mesh m, Mesh snap, int nP, int nT, int* __restrict_
     }
 }
 
-__global__ void morton_k(Mesh m, int nT, const float4* __restrict__ fbox, const float* __restrict__ whole,
+// Morton code of each face's box centre in the whole box of the live faces.
+// A face removed by this round's collapse but live in the snapshot is still
+// a leaf (an undo may restore it): its code is that of its snapshot box,
+// with the centre clamped into the whole box, so a restored face sits among
+// its neighbours. Faces removed before the round sort behind every leaf.
+__global__ void morton_k(Mesh m, Orig snap, int nT, const float4* __restrict__ fbox, const float* __restrict__ whole,
                          uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nT) return;
     val[f] = (uint32_t)f;
-    if (m.i[f] == -1) {
-        key[f] = 0xFFFFFFFFu;  // removed faces sort behind every live code (< 2^30)
+    float c[3];
+    if (m.i[f] != -1) {
+        float4 lo = fbox[2 * (long long)f], hi = fbox[2 * (long long)f + 1];
+        c[0] = (hi.x + lo.x) * 0.5f;
+        c[1] = (hi.y + lo.y) * 0.5f;
+        c[2] = (hi.z + lo.z) * 0.5f;
+    } else if (snap.i[f] != -1) {
+        F3 p = f3(snap.p[snap.i[f]]), q = f3(snap.p[snap.j[f]]), r = f3(snap.p[snap.k[f]]);
+        c[0] = (fmaxf(p.x, fmaxf(q.x, r.x)) + fminf(p.x, fminf(q.x, r.x))) * 0.5f;
+        c[1] = (fmaxf(p.y, fmaxf(q.y, r.y)) + fminf(p.y, fminf(q.y, r.y))) * 0.5f;
+        c[2] = (fmaxf(p.z, fmaxf(q.z, r.z)) + fminf(p.z, fminf(q.z, r.z))) * 0.5f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[d] = fminf(fmaxf(c[d], whole[d]), whole[3 + d]);
+    } else {
+        key[f] = 0xFFFFFFFFu;  // above every code of a leaf (at most 0xE0000000 | 2^30 - 1)
         return;
     }
-    float4 lo = fbox[2 * (long long)f], hi = fbox[2 * (long long)f + 1];
-    float c[3] = {(hi.x + lo.x) * 0.5f, (hi.y + lo.y) * 0.5f, (hi.z + lo.z) * 0.5f};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         c[d] -= whole[d];
@@ -1123,12 +1141,14 @@ __device__ __forceinline__ void load_tri9(Mesh m, int f, float T[3][3], int idx[
 
 // Changed live faces in Morton order (neighbouring lanes query neighbouring
 // regions of the tree).
+// Queries: the changed live faces among the tree's leaves, in leaf order.
 __global__ void query_flag_k(const uint32_t* __restrict__ val, const int* __restrict__ cnt,
-                             const uint8_t* __restrict__ changed, int nT, int* __restrict__ flag) {
+                             const uint8_t* __restrict__ changed, const int* __restrict__ ti, int nT,
+                             int* __restrict__ flag) {
     const int n = cnt[C_NLIVE];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nT) return;
-    flag[s] = (s < n && changed[val[s]]) ? 1 : 0;
+    flag[s] = (s < n && changed[val[s]] && ti[val[s]] != -1) ? 1 : 0;
 }
 
 __global__ void query_list_k(const uint32_t* __restrict__ val, const int* __restrict__ flag,

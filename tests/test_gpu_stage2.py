@@ -188,38 +188,55 @@ def test_fused_loop_reuses_context(torch_cuda, oracle_lib):
 
 
 def test_lbvh_bit_exact(torch_cuda, oracle_lib):
-    """Morton codes, stable sort order and Karras children of the LBVH built
-    by the last guard of a round equal the oracle's; every internal box is
-    exactly the union of its children's boxes."""
+    """The guard's LBVH after one round.  Its leaves are the faces live at the
+    start of the round (a face removed by the collapse stays a leaf with an
+    empty box, since an undo may restore it); the (code, face) keys strictly
+    increase; the children equal the oracle's Karras construction over the
+    same keys; leaf boxes are the exact boxes of the final faces (the last
+    guard refits the hierarchy the first one built); every internal box is
+    exactly the union of its children's; rope links.  A round without undo
+    built its tree on the final state: then the live leaves, in order, carry
+    exactly the oracle's LBVH codes of the output."""
     from pamo_amd import synth, _hip
     from pamo_amd.pamo import CUDSP_Free
     torch = torch_cuda
-    V, F = synth.ellipsoid_box_union(1, 48)
-    Vc = centred(V, F)
-    print(f"case: LBVH after one round, {len(F)} faces", flush=True)
-    op = CUDSP_Free(0)
-    out = op.forward(torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(),
-                     torch.zeros(0, dtype=torch.int32, device="cuda"), 0, float((Vc.max(0) - Vc.min(0)).max()),
-                     1e-3, False, True)
-    pts, tris = out[0].cpu().numpy(), out[1].cpu().numpy()
-    n = op.debug(_hip.PAMO_DBG_BVH_N)
-    codes_o, faces_o, left_o, right_o = oracle_lib.lbvh(pts, tris)
-    assert n == len(codes_o)
-    codes = op.debug(_hip.PAMO_DBG_BVH_CODES).cpu().numpy().view(np.uint32)
-    faces = op.debug(_hip.PAMO_DBG_BVH_FACES).cpu().numpy()
-    assert np.array_equal(codes, codes_o)
-    assert np.array_equal(faces, faces_o)
-    assert np.array_equal(op.debug(_hip.PAMO_DBG_BVH_LEFT).cpu().numpy(), left_o)
-    assert np.array_equal(op.debug(_hip.PAMO_DBG_BVH_RIGHT).cpu().numpy(), right_o)
-    boxes = op.debug(_hip.PAMO_DBG_BVH_BOXES, dtype=torch.float32).cpu().numpy().reshape(2 * n - 1, 8)
-    lo, hi = boxes[:, 0:3], boxes[:, 4:7]
-    # leaves: exact face boxes
-    T = pts[tris[faces]]
-    assert np.array_equal(lo[n - 1:], T.min(1)) and np.array_equal(hi[n - 1:], T.max(1))
-    # internal: exact union of children
-    assert np.array_equal(lo[: n - 1], np.minimum(lo[left_o], lo[right_o]))
-    assert np.array_equal(hi[: n - 1], np.maximum(hi[left_o], hi[right_o]))
-    # rope links: left child (internal) / face (leaf) in lo.w, escape in hi.w
-    w = boxes.view(np.int32)
-    assert np.array_equal(w[: n - 1, 3], left_o) and np.array_equal(w[n - 1:, 3], faces_o)
-    assert np.array_equal(w[:, 7], escape_links(left_o, right_o, n))
+    for seed, res in [(1, 48), (2, 40)]:
+        V, F = synth.ellipsoid_box_union(seed, res)
+        Vc = centred(V, F)
+        op = CUDSP_Free(0)
+        out = op.forward(torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(),
+                         torch.zeros(0, dtype=torch.int32, device="cuda"), 0, float((Vc.max(0) - Vc.min(0)).max()),
+                         1e-3, False, True)
+        pts, tris = out[0].cpu().numpy(), out[1].cpu().numpy()
+        undo_iters = op.debug(_hip.PAMO_DBG_UNDO_ITERS)
+        n = op.debug(_hip.PAMO_DBG_BVH_N)
+        live = tris[:, 0] >= 0
+        print(f"case: LBVH after one round, seed {seed}, {len(F)} faces, {int(live.sum())} live, "
+              f"undo passes {undo_iters}", flush=True)
+        assert n == len(F)  # every input face is live at the start of the round
+        codes = op.debug(_hip.PAMO_DBG_BVH_CODES).cpu().numpy().view(np.uint32)
+        faces = op.debug(_hip.PAMO_DBG_BVH_FACES).cpu().numpy()
+        left = op.debug(_hip.PAMO_DBG_BVH_LEFT).cpu().numpy()
+        right = op.debug(_hip.PAMO_DBG_BVH_RIGHT).cpu().numpy()
+        assert sorted(faces.tolist()) == list(range(len(F)))
+        keys = (codes.astype(np.uint64) << np.uint64(32)) | faces.astype(np.uint64)
+        assert np.all(keys[1:] > keys[:-1])
+        left_o, right_o = oracle_lib.karras(codes, faces)
+        assert np.array_equal(left, left_o) and np.array_equal(right, right_o)
+        boxes = op.debug(_hip.PAMO_DBG_BVH_BOXES, dtype=torch.float32).cpu().numpy().reshape(2 * n - 1, 8)
+        lo, hi = boxes[:, 0:3], boxes[:, 4:7]
+        # leaves: exact boxes of the final faces; empty boxes for removed faces
+        lv = live[faces]
+        T = pts[tris[faces[lv]]]
+        assert np.array_equal(lo[n - 1:][lv], T.min(1)) and np.array_equal(hi[n - 1:][lv], T.max(1))
+        assert np.all(lo[n - 1:][~lv] == np.inf) and np.all(hi[n - 1:][~lv] == -np.inf)
+        # internal: exact union of children
+        assert np.array_equal(lo[: n - 1], np.minimum(lo[left], lo[right]))
+        assert np.array_equal(hi[: n - 1], np.maximum(hi[left], hi[right]))
+        # rope links: left child (internal) / face (leaf) in lo.w, escape in hi.w
+        w = boxes.view(np.int32)
+        assert np.array_equal(w[: n - 1, 3], left) and np.array_equal(w[n - 1:, 3], faces)
+        assert np.array_equal(w[:, 7], escape_links(left, right, n))
+        if undo_iters == 0:
+            codes_o, faces_o, _, _ = oracle_lib.lbvh(pts, tris)
+            assert np.array_equal(codes[lv], codes_o) and np.array_equal(faces[lv], faces_o)

@@ -102,6 +102,17 @@ def test_lbvh_sort_order_and_codes(oracle_lib):
     assert np.array_equal(faces, live[order])
 
 
+def test_karras_on_given_keys_matches_lbvh(oracle_lib):
+    """orc_karras (the hierarchy step on caller keys) gives the LBVH's children."""
+    V, F = _small_mesh()
+    for drop in ([], [3, 17, 40]):
+        Fd = F.copy()
+        Fd[drop] = -1
+        codes, faces, left, right = oracle_lib.lbvh(V, Fd)
+        l2, r2 = oracle_lib.karras(codes, faces)
+        assert np.array_equal(l2, left) and np.array_equal(r2, right)
+
+
 def test_karras_tree_is_valid(oracle_lib):
     V, F = _small_mesh()
     codes, faces, left, right = oracle_lib.lbvh(V, F)
