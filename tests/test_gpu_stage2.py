@@ -187,6 +187,39 @@ def test_fused_loop_reuses_context(torch_cuda, oracle_lib):
         assert np.array_equal(fg.cpu().numpy(), fo)
 
 
+def _check_guard_tree(op, pts, tris, n_start, oracle_lib, torch):
+    """Structure of the guard's LBVH after a round whose input had n_start
+    faces, all live (see test_lbvh_bit_exact)."""
+    from pamo_amd import _hip
+    n = op.debug(_hip.PAMO_DBG_BVH_N)
+    live = tris[:, 0] >= 0
+    assert n == n_start  # every face live at the start of the round is a leaf
+    codes = op.debug(_hip.PAMO_DBG_BVH_CODES).cpu().numpy().view(np.uint32)
+    faces = op.debug(_hip.PAMO_DBG_BVH_FACES).cpu().numpy()
+    left = op.debug(_hip.PAMO_DBG_BVH_LEFT).cpu().numpy()
+    right = op.debug(_hip.PAMO_DBG_BVH_RIGHT).cpu().numpy()
+    assert sorted(faces.tolist()) == list(range(n_start))
+    keys = (codes.astype(np.uint64) << np.uint64(32)) | faces.astype(np.uint64)
+    assert np.all(keys[1:] > keys[:-1])
+    left_o, right_o = oracle_lib.karras(codes, faces)
+    assert np.array_equal(left, left_o) and np.array_equal(right, right_o)
+    boxes = op.debug(_hip.PAMO_DBG_BVH_BOXES, dtype=torch.float32).cpu().numpy().reshape(2 * n - 1, 8)
+    lo, hi = boxes[:, 0:3], boxes[:, 4:7]
+    # leaves: exact boxes of the final faces; empty boxes for removed faces
+    lv = live[faces]
+    T = pts[tris[faces[lv]]]
+    assert np.array_equal(lo[n - 1:][lv], T.min(1)) and np.array_equal(hi[n - 1:][lv], T.max(1))
+    assert np.all(lo[n - 1:][~lv] == np.inf) and np.all(hi[n - 1:][~lv] == -np.inf)
+    # internal: exact union of children
+    assert np.array_equal(lo[: n - 1], np.minimum(lo[left], lo[right]))
+    assert np.array_equal(hi[: n - 1], np.maximum(hi[left], hi[right]))
+    # rope links: left child (internal) / face (leaf) in lo.w, escape in hi.w
+    w = boxes.view(np.int32)
+    assert np.array_equal(w[: n - 1, 3], left) and np.array_equal(w[n - 1:, 3], faces)
+    assert np.array_equal(w[:, 7], escape_links(left, right, n))
+    return codes, faces, lv
+
+
 def test_lbvh_bit_exact(torch_cuda, oracle_lib):
     """The guard's LBVH after one round.  Its leaves are the faces live at the
     start of the round (a face removed by the collapse stays a leaf with an
@@ -209,34 +242,40 @@ def test_lbvh_bit_exact(torch_cuda, oracle_lib):
                          1e-3, False, True)
         pts, tris = out[0].cpu().numpy(), out[1].cpu().numpy()
         undo_iters = op.debug(_hip.PAMO_DBG_UNDO_ITERS)
-        n = op.debug(_hip.PAMO_DBG_BVH_N)
-        live = tris[:, 0] >= 0
-        print(f"case: LBVH after one round, seed {seed}, {len(F)} faces, {int(live.sum())} live, "
+        print(f"case: LBVH after one round, seed {seed}, {len(F)} faces, {int((tris[:, 0] >= 0).sum())} live, "
               f"undo passes {undo_iters}", flush=True)
-        assert n == len(F)  # every input face is live at the start of the round
-        codes = op.debug(_hip.PAMO_DBG_BVH_CODES).cpu().numpy().view(np.uint32)
-        faces = op.debug(_hip.PAMO_DBG_BVH_FACES).cpu().numpy()
-        left = op.debug(_hip.PAMO_DBG_BVH_LEFT).cpu().numpy()
-        right = op.debug(_hip.PAMO_DBG_BVH_RIGHT).cpu().numpy()
-        assert sorted(faces.tolist()) == list(range(len(F)))
-        keys = (codes.astype(np.uint64) << np.uint64(32)) | faces.astype(np.uint64)
-        assert np.all(keys[1:] > keys[:-1])
-        left_o, right_o = oracle_lib.karras(codes, faces)
-        assert np.array_equal(left, left_o) and np.array_equal(right, right_o)
-        boxes = op.debug(_hip.PAMO_DBG_BVH_BOXES, dtype=torch.float32).cpu().numpy().reshape(2 * n - 1, 8)
-        lo, hi = boxes[:, 0:3], boxes[:, 4:7]
-        # leaves: exact boxes of the final faces; empty boxes for removed faces
-        lv = live[faces]
-        T = pts[tris[faces[lv]]]
-        assert np.array_equal(lo[n - 1:][lv], T.min(1)) and np.array_equal(hi[n - 1:][lv], T.max(1))
-        assert np.all(lo[n - 1:][~lv] == np.inf) and np.all(hi[n - 1:][~lv] == -np.inf)
-        # internal: exact union of children
-        assert np.array_equal(lo[: n - 1], np.minimum(lo[left], lo[right]))
-        assert np.array_equal(hi[: n - 1], np.maximum(hi[left], hi[right]))
-        # rope links: left child (internal) / face (leaf) in lo.w, escape in hi.w
-        w = boxes.view(np.int32)
-        assert np.array_equal(w[: n - 1, 3], left) and np.array_equal(w[n - 1:, 3], faces)
-        assert np.array_equal(w[:, 7], escape_links(left, right, n))
+        codes, faces, lv = _check_guard_tree(op, pts, tris, len(F), oracle_lib, torch)
         if undo_iters == 0:
             codes_o, faces_o, _, _ = oracle_lib.lbvh(pts, tris)
             assert np.array_equal(codes[lv], codes_o) and np.array_equal(faces[lv], faces_o)
+
+
+def test_lbvh_refit_after_undo(torch_cuda, oracle_lib):
+    """Offset sheets, round by round until a round runs the undo loop: the
+    tree its last guard used was refit after the undo (restored faces have
+    their exact boxes again, faces still removed have empty ones), and its
+    structure passes the same checks."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import CUDSP_Free
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(4, 3)
+    V2, F2 = synth.offset_sheets(V, F, rel_offset=5e-3)
+    print(f"case: LBVH refit, offset sheets L4 seed 3, {len(F2)} faces", flush=True)
+    op = CUDSP_Free(0)
+    fwd = _gpu_forward(op, torch)
+    seen = {"n_in": 0, "checked": 0}
+
+    def counting_fwd(Vr, Fr, *a):
+        seen["n_in"] = len(Fr)
+        return fwd(Vr, Fr, *a)
+
+    def on_round(it, out):
+        if seen["checked"] or op.debug(_hip.PAMO_DBG_UNDO_ITERS) == 0:
+            return
+        pts, tris = out[0], out[1]
+        print(f"round {it}: {op.debug(_hip.PAMO_DBG_UNDO_ITERS)} undo passes, {seen['n_in']} faces in", flush=True)
+        _check_guard_tree(op, pts, tris, seen["n_in"], oracle_lib, torch)
+        seen["checked"] = 1
+
+    run_rounds(counting_fwd, centred(V2, F2), F2, 0.05, on_round=on_round)
+    assert seen["checked"], "no round ran the undo loop"
