@@ -125,6 +125,28 @@ static int tree_build(Tree& t, int n, int* ws, hipStream_t st) {
     return 0;
 }
 
+// Recomputes every box of a tree built by tree_build over the same n items,
+// after the items' boxes changed: leaf boxes in the built leaf order, node
+// boxes over the built hierarchy, and the rope links (which share the box
+// records).  The tree stays exact for any item boxes; its quality degrades
+// only as far as the items moved since the build.
+static int tree_refit(Tree& t, int n, hipStream_t st) {
+    PAMO_CHECK(n == t.n, "tree_refit: item count differs from the built tree");
+    if (n <= 0) return 0;
+    const unsigned nb = grid_for(n, kBlock), nb1 = grid_for(nb, kBlock);
+    hipLaunchKernelGGL(box_scan_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, (const uint32_t*)t.val,
+                       (const int*)t.cnt, 0, t.box, t.pre, t.suf, t.blk);
+    hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, st, (const float4*)t.blk, (const uint32_t*)nullptr,
+                       (const int*)t.cnt, 1, (float4*)nullptr, t.blkpre, t.blksuf, t.sblk);
+    hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
+                       (const float4*)t.box, (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blk,
+                       (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
+    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * n, kBlock)), dim3(kBlock), 0, st, (const int*)t.left,
+                       (const int*)t.right, (const int*)t.parent, (const int*)t.cnt, (const uint32_t*)t.val, t.box);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
 struct TreeView {
     int n;
     const float4* box;

@@ -114,6 +114,7 @@ struct Ctx {
     float4* gt_v = nullptr;
     int4* gt_f = nullptr;
     Tree gt_tree, tri_tree, edge_tree;
+    bool contact_trees_built = false;  // tri_tree / edge_tree hold this mesh's hierarchy
     // contacts
     int capC = 0;
     int4* cidx = nullptr;
@@ -1396,10 +1397,19 @@ static int detect_contacts(Ctx& c, int& nC) {
     if (c.P.mask & PAMO_S3_E_COLLISION) {
         PAMO_TRY(tree_alloc(c.tri_tree, c.nF, c.stream));
         PAMO_TRY(tree_alloc(c.edge_tree, c.nE, c.stream));
+        // the hierarchies are built at the first detection on a mesh and refit
+        // after that: Newton steps move vertices by at most the clamp radius,
+        // and the contact set found does not depend on the tree's shape
         L3(tri_items_k, c.nF, (const float4*)c.q, (const int4*)c.tri, c.nF, 0.0f, c.tri_tree.item);
-        PAMO_TRY(tree_build(c.tri_tree, c.nF, c.ws, c.stream));
         L3(edge_items_k, c.nE, (const float4*)c.q, (const int2*)c.edge, c.nE, c.edge_tree.item);
-        PAMO_TRY(tree_build(c.edge_tree, c.nE, c.ws, c.stream));
+        if (c.contact_trees_built) {
+            PAMO_TRY(tree_refit(c.tri_tree, c.nF, c.stream));
+            PAMO_TRY(tree_refit(c.edge_tree, c.nE, c.stream));
+        } else {
+            PAMO_TRY(tree_build(c.tri_tree, c.nF, c.ws, c.stream));
+            PAMO_TRY(tree_build(c.edge_tree, c.nE, c.ws, c.stream));
+            c.contact_trees_built = true;
+        }
         while (true) {
             PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
             const int cap = std::min(c.capC, c.P.max_blocks);
@@ -1788,6 +1798,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
     c.nF = n_f;
     c.nE = n_e;
     c.nH = n_h;
+    c.contact_trees_built = false;
     c.do_nothing = do_nothing != 0;
     c.radius = c.P.radius0;
     c.stat_contacts = c.stat_newton = c.stat_cg = c.stat_ls = 0;
