@@ -71,7 +71,9 @@ int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tri
 #define PAMO_DBG_QUADRICS 6   /* float32 (10, n_pts) component-major */
 #define PAMO_DBG_UNDO_ITERS 7 /* count only                      */
 #define PAMO_DBG_N_REGULAR 8  /* count only                      */
-#define PAMO_DBG_BVH_N 9      /* count only: leaves of the last LBVH */
+#define PAMO_DBG_BVH_N 9      /* count only: leaves of the last round's guard LBVH (the
+                                 faces live at the start of that round; a face removed
+                                 since keeps its leaf, with an empty box) */
 #define PAMO_DBG_BVH_CODES 10 /* uint32 Morton codes, sorted      */
 #define PAMO_DBG_BVH_FACES 11 /* int32 face per sorted leaf       */
 #define PAMO_DBG_BVH_LEFT 12  /* int32 left child of internal node */
