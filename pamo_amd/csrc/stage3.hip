@@ -952,7 +952,7 @@ struct Incid {
 // incidence lists are strided over the group and reduced with a fixed
 // butterfly), so a vertex with many incident contacts does not hold up a
 // whole wave.
-constexpr int kGather = 16;
+constexpr int kGather = 32;
 
 // Contact terms of the gradient (barrier' kappa coeff g) and of the Jacobi
 // diagonal (barrier'' kappa g*g) per vertex, summed over its active-contact
