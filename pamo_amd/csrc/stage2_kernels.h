@@ -1180,7 +1180,7 @@ __device__ __forceinline__ void eval_pair(Mesh m, int a, int b, float epsilon, u
 }
 
 constexpr int kQueue = 128;   // per-wave pair queue (< 64 pending + one append of <= 64)
-constexpr int kGuardQ = 16;   // guard queries per wave
+constexpr int kGuardQ = 8;    // guard queries per wave
 constexpr int kGuardStack = 512;       // per-wave frontier entries
 constexpr int kGuardStackSmall = 32;   // test variant: forces the overflow walk
 
