@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 # each scaled by what it reads for a 512 MiB copy), from rocprofv3 --pmc runs
 # of this workload; re-measure after changing these kernels.
 TRAFFIC_SOURCE = "profiles/counters_traffic_occupancy_lds.md"
-HBM_TRAFFIC_PER_LAUNCH = {"guard_query": 58.3e6, "edge_cost": 104.7e6}
+HBM_TRAFFIC_PER_LAUNCH = {"guard_query": 48.5e6, "edge_cost": 104.7e6}
 
 
 def parse():
