@@ -57,9 +57,12 @@ int pamo_round(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tris, 
                int n_undo, float scale, float threshold, int is_stuck, int init, float* d_out_pts, int* d_out_tris,
                int* d_out_occ, int* d_out_map, int* d_out_undo, int undo_cap, int* n_undo_out);
 
-/* Centring offset of PaMO.run: mean over faces of the face centroids
- * (reference simp_cuda/pamo/__init__.py:60-72,144), FP64 accumulation in a
- * fixed order; out3 is a host float[3]. */
+/* Centring offset of PaMO.run, tris_mean = points[triangles].mean(axis=1)
+ * .mean(axis=0) on float32 numpy arrays (reference
+ * simp_cuda/pamo/__init__.py:60-72,144), bit for bit: per-face
+ * ((a + b) + c) / 3, then a sequential float32 running sum over the faces in
+ * order, divided by n_tris.  Face indices must lie in [0, n_pts).  out3 is a
+ * host float[3]; the call returns when it is written. */
 int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tris, int n_tris, float* out3);
 
 /* Intermediate arrays of the last pamo_round (for parity tests). */

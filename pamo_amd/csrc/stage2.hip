@@ -134,8 +134,7 @@ struct Ctx {
     int n_pts = 0, n_tris = 0, n_edges = 0, n_rec = 0, n_regular = 0, n_vertices_undo = 0;
     int undo_iters = 0;
     int guard_small_stack = 0;  // test option: 32-entry guard frontier
-    double* cent_part = nullptr;  // per-block partial sums of pamo_centroid
-    size_t alloc_cent = 0;
+    float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
     size_t alloc_ulist = 0;
@@ -312,7 +311,7 @@ static void free_all(Ctx& c) {
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
                     c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
-                    c.shuf, c.ws, c.dcnt, c.cent_part, c.ulist};
+                    c.shuf, c.ws, c.dcnt, c.cent_out, c.ulist};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
@@ -761,24 +760,17 @@ int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tri
     pamo::clear_error();
     PAMO_CHECK(ctx != nullptr && out3 != nullptr, "pamo_centroid: null argument");
     PAMO_CHECK(n_pts > 0 && n_tris > 0, "pamo_centroid: empty mesh");
+    PAMO_CHECK(d_pts != nullptr && d_tris != nullptr, "pamo_centroid: null buffer");
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     PAMO_HIP(hipSetDevice(c.device));
-    const int nb = (int)pamo::grid_for(n_tris, pamo::kBlock);
-    if ((size_t)(3 * nb) > c.alloc_cent) {
-        PAMO_TRY(pamo::dalloc(&c.cent_part, 3 * (size_t)nb, c.stream));
-        c.alloc_cent = 3 * (size_t)nb;
-    }
-    double* part = c.cent_part;
-    hipLaunchKernelGGL(pamo::centroid_part_k, dim3(nb), dim3(pamo::kBlock), 0, c.stream, d_pts, d_tris, n_tris, part);
+    if (!c.cent_out) PAMO_TRY(pamo::dalloc(&c.cent_out, 4, c.stream));
+    hipLaunchKernelGGL(pamo::centroid_seq_k, dim3(1), dim3(pamo::kCentThreads), 0, c.stream, d_pts, d_tris, n_tris,
+                       c.cent_out);
     PAMO_HIP(hipGetLastError());
-    std::vector<double> h(3 * (size_t)nb);
-    PAMO_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    float h[3];
+    PAMO_HIP(hipMemcpyAsync(h, c.cent_out, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
-    for (int d = 0; d < 3; ++d) {
-        double acc = 0.0;
-        for (int b = 0; b < nb; ++b) acc += h[3 * (size_t)b + d];
-        out3[d] = (float)(acc / (double)n_tris);
-    }
+    for (int d = 0; d < 3; ++d) out3[d] = h[d];
     return 0;
 }
 

@@ -1483,28 +1483,62 @@ __global__ void compact_mesh_k(Mesh m, int nP, int nT, const int* __restrict__ o
     }
 }
 
-// Mean over faces of the float32 face centroid ((a + b) + c) / 3, summed in
-// FP64 per block in a fixed tree (pamo/__init__.py:60-72 centring).
-__global__ void __launch_bounds__(kBlock) centroid_part_k(const float* __restrict__ pts, const int* __restrict__ tris,
-                                                          int nF, double* __restrict__ part) {
-    __shared__ double red[3][kBlock];
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    double a[3] = {0.0, 0.0, 0.0};
-    if (f < nF) {
-        const int i0 = tris[3 * (long long)f], i1 = tris[3 * (long long)f + 1], i2 = tris[3 * (long long)f + 2];
+// Centring offset of PaMO.run (pamo/__init__.py:60-72,144): numpy's
+// tris.mean(axis=1).mean(axis=0) on the float32 face soup.  The per-face mean
+// is ((a + b) + c) divided by 3 in double and rounded to float (numpy divides
+// the float32 sum by an integer count through a float64 loop); the mean over
+// faces is a sequential float32 running sum from the first face's value, in
+// face order, divided the same way.  A float32 running sum cannot be split,
+// so one workgroup runs it: waves 1..15 write the face means of the next
+// chunk to LDS while lanes 0..2 of wave 0 add up the current chunk, one
+// coordinate per lane.
+constexpr int kCentThreads = 1024;
+constexpr int kCentChunk = 2048;
+constexpr int kCentPitch = kCentChunk + 4;  // rows start in different LDS banks
+
+__device__ inline void centroid_fill(const float* __restrict__ pts, const int* __restrict__ tris, long long base,
+                                     int cnt, float* __restrict__ dst, int t, int nt) {
+    for (int j = t; j < cnt; j += nt) {
+        const long long f = base + j;
+        const int i0 = tris[3 * f], i1 = tris[3 * f + 1], i2 = tris[3 * f + 2];
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-            float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
-            a[d] = (double)(s / 3.0f);
+            const float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
+            dst[d * kCentPitch + j] = (float)((double)s / 3.0);
         }
     }
-    for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = a[d];
+}
+
+__global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __restrict__ pts,
+                                                               const int* __restrict__ tris, int nF,
+                                                               float* __restrict__ out3) {
+    __shared__ float buf[2][3 * kCentPitch];
+    const int t = threadIdx.x;
+    const int nchunks = (nF + kCentChunk - 1) / kCentChunk;
+    centroid_fill(pts, tris, 0, nF < kCentChunk ? nF : kCentChunk, buf[0], t, kCentThreads);
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
+    float acc = 0.0f;
+    for (int k = 0; k < nchunks; ++k) {
+        const long long base = (long long)k * kCentChunk;
+        const int cnt = (int)((nF - base) < kCentChunk ? (nF - base) : kCentChunk);
+        if (t >= kWave) {
+            const long long nb = base + kCentChunk;
+            if (nb < nF) {
+                const int ncnt = (int)((nF - nb) < kCentChunk ? (nF - nb) : kCentChunk);
+                centroid_fill(pts, tris, nb, ncnt, buf[(k + 1) & 1], t - kWave, kCentThreads - kWave);
+            }
+        } else if (t < 3) {
+            const float* row = buf[k & 1] + t * kCentPitch;
+            int i = 0;
+            if (k == 0) {
+                acc = row[0];  // numpy's reduction starts from the first element
+                i = 1;
+            }
+            for (; i < cnt; ++i) acc = acc + row[i];
+        }
         __syncthreads();
     }
-    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (t < 3) out3[t] = (float)((double)acc / (double)nF);
 }
 
 }  // namespace

@@ -25,6 +25,23 @@ def _check_input(t: torch.Tensor, name: str):
         raise RuntimeError(f"{name} must be contiguous")
 
 
+def _check_mesh_types(points: torch.Tensor, triangles: torch.Tensor):
+    """The dtype and shape checks of the per-round operator (pybind.cpp:54-63):
+    float32 (V, 3) points and int32 (F, 3) triangles, else RuntimeError.
+    The dtypes are checked first, so that a float64 or int64 array is named
+    as such wherever it lives."""
+    if not isinstance(points, torch.Tensor) or points.dtype != torch.float32:
+        raise RuntimeError("points type must match the pamo class")
+    if not isinstance(triangles, torch.Tensor) or triangles.dtype != torch.int32:
+        raise RuntimeError("triangles type must match the pamo class")
+    _check_input(points, "points")
+    _check_input(triangles, "triangles")
+    if points.dim() != 2 or points.shape[1] != 3:
+        raise RuntimeError("points must have shape (n, 3)")
+    if triangles.dim() != 2 or triangles.shape[1] != 3:
+        raise RuntimeError("triangles must have shape (n, 3)")
+
+
 def _check_faces(triangles: torch.Tensor, n_pts: int, name: str = "triangles"):
     """Every face row is removed (all -1) or indexes the points; the kernels
     read the points of every other row."""
@@ -49,17 +66,19 @@ class CUDSP_Free:
         self.ctx = _hip.Context(device)
 
     def forward(self, points, triangles, verts_undo, n_verts_undo, scale, threshold, is_stuck, init):
-        _check_input(points, "points")
-        _check_input(triangles, "triangles")
+        _check_mesh_types(points, triangles)
         _check_input(verts_undo, "verts_undo")
-        if points.dtype != torch.float32:
-            raise RuntimeError("points type must match the pamo class")
-        if triangles.dtype != torch.int32:
-            raise RuntimeError("triangles type must match the pamo class")
         if verts_undo.dtype != torch.int32:
             raise RuntimeError("verts_undo must be int32")
         n_pts, n_tris = int(points.shape[0]), int(triangles.shape[0])
         n_undo = int(n_verts_undo)
+        _check_faces(triangles, n_pts)
+        if n_undo < 0 or n_undo > verts_undo.numel():
+            raise RuntimeError(f"n_verts_undo={n_undo} outside [0, {verts_undo.numel()}]")
+        if n_undo > 0:
+            u = verts_undo[:n_undo]
+            if not bool(((u >= 0) & (u < n_pts)).all()):
+                raise RuntimeError(f"verts_undo: vertex indices outside [0, {n_pts})")
         dev = points.device
         out_pts = torch.empty((n_pts, 3), dtype=torch.float32, device=dev)
         out_tris = torch.empty((n_tris, 3), dtype=torch.int32, device=dev)
@@ -93,6 +112,10 @@ class CUDSP_Free:
 
 
 def face_centroid_mean(ctx: _hip.Context, points: torch.Tensor, triangles: torch.Tensor) -> np.ndarray:
+    """tris_mean of PaMO.preprocess_mesh (pamo/__init__.py:60-72): numpy's
+    points[triangles].mean(axis=1).mean(axis=0) in float32, bit for bit."""
+    _check_mesh_types(points, triangles)
+    _check_faces(triangles, int(points.shape[0]))
     out = (ctypes.c_float * 3)()
     torch.cuda.current_stream(points.device).synchronize()
     _hip.check(_hip.lib().pamo_centroid(ctx.ptr, points.data_ptr(), int(points.shape[0]), triangles.data_ptr(),
@@ -102,15 +125,17 @@ def face_centroid_mean(ctx: _hip.Context, points: torch.Tensor, triangles: torch
 
 def simplify_device(ctx: _hip.Context, verts: torch.Tensor, faces: torch.Tensor, ratio: float, tolerance: int = 4,
                     threshold: float = 1e-3, iters: int = 1000000, min_verts: int = 10000000000):
-    """The stage-2 round loop of PaMO.run on device; verts must be centred."""
-    _check_input(verts, "points")
-    _check_input(faces, "triangles")
+    """The stage-2 round loop of PaMO.run on device; verts must be centred.
+    Returns (vertices, faces, face count after each round)."""
+    _check_mesh_types(verts, faces)
     nv, nf = int(verts.shape[0]), int(faces.shape[0])
     _check_faces(faces, nv)
     out_v = torch.empty_like(verts)
     out_f = torch.empty_like(faces)
     onv, onf, rounds = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-    hist = np.zeros(1 << 16, dtype=np.int32)
+    # one entry per round: at most `iters` rounds, and a round that changes
+    # the face count removes at least one face
+    hist = np.zeros(max(1, min(int(iters), nf + int(tolerance) * (nf + 1) + 1, 1 << 26)), dtype=np.int32)
     torch.cuda.current_stream(verts.device).synchronize()
     _hip.check(_hip.lib().pamo_simplify(
         ctx.ptr, verts.data_ptr(), nv, faces.data_ptr(), nf, float(ratio), int(tolerance), float(threshold),
@@ -169,6 +194,7 @@ class PaMO(torch.nn.Module):
 
     def run(self, points, triangles, ratio, tolerance=4, threshold=1e-3, iter=1000000, min_verts=10000000000):
         """pamo/__init__.py:122-205; returns numpy (vertices, faces)."""
+        _check_mesh_types(points, triangles)
         self.target_faces = max(int(ratio * len(triangles)), min_verts)
         t0 = time.perf_counter()
         tris_mean = face_centroid_mean(self.ctx, points, triangles)

@@ -4,9 +4,17 @@ from __future__ import annotations
 import numpy as np
 
 
+def tris_mean(V, F):
+    """The centring offset of PaMO.preprocess_mesh (pamo/__init__.py:60-72),
+    the reference's own numpy expression on float32 points."""
+    V = np.asarray(V, dtype=np.float32)
+    return V[F].mean(axis=1).mean(axis=0)
+
+
 def centred(V, F):
-    mean = V[F].mean(axis=1).mean(axis=0)
-    return (V - mean).astype(np.float32)
+    """points - tris_mean in float32 (pamo/__init__.py:144)."""
+    V = np.asarray(V, dtype=np.float32)
+    return (V - tris_mean(V, F)).astype(np.float32)
 
 
 def scale_of(V):

@@ -32,3 +32,25 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     import pytest
     with pytest.raises(_hip.PamoError):
         _hip.lib()
+
+
+def test_mesh_types_rejected_before_device_work():
+    """float64 points and int64 faces are rejected with the messages of the
+    reference's per-round operator (pybind.cpp:54-63), before any device
+    pointer is taken."""
+    import numpy as np
+    import pytest
+    import torch
+    from pamo_amd.pamo import _check_mesh_types, face_centroid_mean, simplify_device
+    V = torch.zeros((4, 3), dtype=torch.float32)
+    F = torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="points type"):
+        _check_mesh_types(V.double(), F)
+    with pytest.raises(RuntimeError, match="triangles type"):
+        _check_mesh_types(V, F.long())
+    with pytest.raises(RuntimeError, match="points type"):
+        face_centroid_mean(None, torch.from_numpy(np.zeros((4, 3))), F)
+    with pytest.raises(RuntimeError, match="triangles type"):
+        simplify_device(None, V, F.long(), 0.5)
+    with pytest.raises(RuntimeError, match="CUDA tensor"):
+        _check_mesh_types(V, F)
