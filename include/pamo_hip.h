@@ -40,6 +40,13 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * the filter cannot decide; 2 as 1 with every pair sent through the exact
  * pass (test setting).  A context keeps the setting across calls. */
 #define PAMO_OPT_EXACT_GUARD 2
+/* Test option (no counterpart in the reference): every tree walk of the
+ * guard stops after its bound -- each node once for an escape-link walk,
+ * (queries x nodes) frontier steps -- sets an error flag and makes the call
+ * fail with "a tree walk exceeded its bound" instead of looping on a
+ * corrupted tree.  A value > 0 lowers both bounds to it, so that tests reach
+ * that error path on a valid tree; 0 (default) keeps the bounds. */
+#define PAMO_OPT_WALK_LIMIT 3
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).

@@ -790,9 +790,21 @@ static void guard(Ctx& c, const std::vector<uint8_t>& changed, float epsilon) {
 }
 
 // get_undo_candidate_kernel (:776-827): returns the pushed edges in record order
+// get_undo_candidate_kernel (cusimp_free.cu:776-827) over the first n_cand
+// records: its grid is sized from the regular-collapse count read before the
+// line collapses (:1004,1016: h_n_collapsed threads rounded up to blocks of
+// BLOCK_SIZE = 512), so a line-collapse record past that grid is never a
+// candidate, and none is when the round has no regular record.
+static int undo_candidate_count(const Ctx& c) {
+    const int grid = (c.n_regular + 511) / 512 * 512;
+    return std::min((int)c.collapsed.size(), grid);
+}
+
 static void undo_candidates(Ctx& c, std::vector<int>& pushed) {
     pushed.clear();
-    for (int e : c.collapsed) {
+    const int n_cand = undo_candidate_count(c);
+    for (int r = 0; r < n_cand; ++r) {
+        const int e = c.collapsed[r];
         int u = c.eu[e], v = c.ev[e];
         bool hit = false;
         for (int w : {u, v}) {
@@ -929,7 +941,7 @@ static void forward(Ctx& c, const float* pts, int nP, const int* tris, int nT, c
 
     c.undo_iters_edges.clear();
     std::vector<int> pushed;
-    if (!c.collapsed.empty()) {
+    if (undo_candidate_count(c) > 0) {
         guard(c, changed, epsilon);
         undo_candidates(c, pushed);
     }

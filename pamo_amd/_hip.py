@@ -15,6 +15,7 @@ c_int, c_float, c_double, c_ll, c_vp = ctypes.c_int, ctypes.c_float, ctypes.c_do
 
 PAMO_OPT_GUARD_SMALL_STACK = 1
 PAMO_OPT_EXACT_GUARD = 2
+PAMO_OPT_WALK_LIMIT = 3
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3

@@ -53,6 +53,13 @@ enum KernelId {
 void prof_begin(int id, double bytes, hipStream_t st);
 void prof_end(int id, hipStream_t st);
 
+// Tree walks are bounded: over a valid tree of n leaves (2n - 1 nodes) an
+// escape-link walk visits each node at most once, a parent-link walk at most
+// three times, and a shared frontier pops at most (queries x nodes) entries.
+// A walk that exceeds its bound (a corrupted tree) stops and sets *err, which
+// the host reports as an error status instead of the GPU spinning.
+__device__ __forceinline__ void walk_overrun(int* err) { atomicOr(err, 1); }
+
 inline unsigned grid_for(long long n, int block) {
     long long g = (n + block - 1) / block;
     return (unsigned)(g < 1 ? 1 : g);

@@ -108,7 +108,7 @@ __device__ __forceinline__ double box_d2(D3 p, float4 lo, float4 hi) {
 __global__ void __launch_bounds__(kB) point_mesh_k(const double* __restrict__ P, long long nP, TreeView T,
                                                   const double* __restrict__ V, const int* __restrict__ F,
                                                   double* __restrict__ sqd, int* __restrict__ I,
-                                                  double* __restrict__ C) {
+                                                  double* __restrict__ C, int* err) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nP) return;
     const D3 p = load3(P, i);
@@ -132,7 +132,12 @@ __global__ void __launch_bounds__(kB) point_mesh_k(const double* __restrict__ P,
         leaf((int)T.val[0]);
     } else if (n > 1) {
         int cur = 0, prev = -1;
-        while (cur >= 0) {
+        const long long max_steps = 3LL * (2LL * n - 1) + 1;
+        for (long long steps = 0; cur >= 0; ++steps) {
+            if (steps >= max_steps || cur > 2 * n - 2) {
+                walk_overrun(err);
+                break;
+            }
             const int par = cur == 0 ? -1 : T.parent[cur];
             int next;
             if (prev == par) {
@@ -181,67 +186,94 @@ __global__ void __launch_bounds__(kB) max_bits_k(const double* __restrict__ x, l
     if (threadIdx.x == 0) atomicMax(out, red[0]);
 }
 
+// Device memory of one call: the tree over the mesh's faces, the sort and
+// scan workspace, and two flags (faces indexing outside the vertices; a tree
+// walk that exceeded its bound).  A metrics context owns one that grows with
+// the mesh; pamo_point_mesh_sqdist_noalloc carves one out of a static array.
+struct Work {
+    Tree tree;
+    int* ws = nullptr;
+    int* flags = nullptr;  // [2] device
+    int* h_flags = nullptr;  // [2] host
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    Tree tree;
-    int* ws = nullptr;
+    Work w;
     long long cap_ws = 0;
     double* sqd = nullptr;  // Hausdorff: per-vertex squared distances
     long long cap_sqd = 0;
-    int* bad = nullptr;                 // [1]
     unsigned long long* mx = nullptr;   // [1]
-    int* h_bad = nullptr;               // pinned
     unsigned long long* h_mx = nullptr; // pinned
 };
 
 static void ctx_free(Ctx& c) {
-    s3::tree_free(c.tree);
-    void* ptrs[] = {c.ws, c.sqd, c.bad, c.mx};
+    s3::tree_free(c.w.tree);
+    void* ptrs[] = {c.w.ws, c.sqd, c.w.flags, c.mx};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    if (c.h_bad) (void)hipHostFree(c.h_bad);
+    if (c.w.h_flags) (void)hipHostFree(c.w.h_flags);
     if (c.h_mx) (void)hipHostFree(c.h_mx);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
-// Builds the tree over the faces of (V, F) after checking every index.
-static int build_tree(Ctx& c, const double* V, int nV, const int* F, int nF, const char* who) {
+// Checks every face index, then builds the tree over the faces of (V, F);
+// w's tree and workspace must hold nF faces.
+static int build_tree(Work& w, hipStream_t st, const double* V, int nV, const int* F, int nF, const char* who) {
     PAMO_CHECK(V && F && nV > 0 && nF > 0, std::string(who) + ": empty or null mesh");
-    PAMO_HIP(hipMemsetAsync(c.bad, 0, sizeof(int), c.stream));
-    hipLaunchKernelGGL(check_faces_k, dim3(grid_for(nF, kB)), dim3(kB), 0, c.stream, F, nF, nV, c.bad);
+    PAMO_TRY(fill(w.flags, 0, 1, st));
+    hipLaunchKernelGGL(check_faces_k, dim3(grid_for(nF, kB)), dim3(kB), 0, st, F, nF, nV, w.flags);
     PAMO_HIP(hipGetLastError());
-    PAMO_HIP(hipMemcpyAsync(c.h_bad, c.bad, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    PAMO_HIP(hipStreamSynchronize(c.stream));
-    PAMO_CHECK(*c.h_bad == 0, std::string(who) + ": " + std::to_string(*c.h_bad) + " faces index outside the vertices");
-    const long long need = std::max(radix_workspace(nF + 64LL), scan_workspace(nF + 65LL)) + 256;
-    if (need > c.cap_ws) {
-        c.cap_ws = need + need / 4;
-        PAMO_TRY(s3::dalloc(&c.ws, (size_t)c.cap_ws, c.stream));
-    }
-    PAMO_TRY(s3::tree_alloc(c.tree, nF, c.stream));
-    hipLaunchKernelGGL(face_box_d_k, dim3(grid_for(nF, kB)), dim3(kB), 0, c.stream, V, F, nF, c.tree.item);
+    PAMO_HIP(hipMemcpyAsync(w.h_flags, w.flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    PAMO_HIP(hipStreamSynchronize(st));
+    PAMO_CHECK(w.h_flags[0] == 0,
+               std::string(who) + ": " + std::to_string(w.h_flags[0]) + " faces index outside the vertices");
+    hipLaunchKernelGGL(face_box_d_k, dim3(grid_for(nF, kB)), dim3(kB), 0, st, V, F, nF, w.tree.item);
     PAMO_HIP(hipGetLastError());
-    return s3::tree_build(c.tree, nF, c.ws, c.stream);
+    return s3::tree_build(w.tree, nF, w.ws, st);
 }
 
-static int query(Ctx& c, const double* P, long long nP, const double* V, const int* F, double* sqd, int* I,
-                 double* C) {
+static int query(Work& w, hipStream_t st, const double* P, long long nP, const double* V, const int* F, double* sqd,
+                 int* I, double* C) {
     if (nP <= 0) return 0;
-    hipLaunchKernelGGL(point_mesh_k, dim3(grid_for(nP, kB)), dim3(kB), 0, c.stream, P, nP, s3::view(c.tree), V, F,
-                       sqd, I, C);
+    hipLaunchKernelGGL(point_mesh_k, dim3(grid_for(nP, kB)), dim3(kB), 0, st, P, nP, s3::view(w.tree), V, F, sqd, I,
+                       C, w.flags + 1);
     PAMO_HIP(hipGetLastError());
     return 0;
 }
 
+// Clears the walk flag at the start of a call.
+static int begin(Work& w, hipStream_t st) { return fill(w.flags + 1, 0, 1, st); }
+
+// Reads the walk flag after the stream's work and reports an overrun.
+static int finish(Work& w, hipStream_t st, const char* who) {
+    PAMO_HIP(hipMemcpyAsync(w.h_flags + 1, w.flags + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    PAMO_HIP(hipStreamSynchronize(st));
+    PAMO_CHECK(w.h_flags[1] == 0, std::string(who) + ": a tree walk exceeded its bound (corrupted LBVH)");
+    return 0;
+}
+
+// Sizes the context's tree and workspace for nF faces.
+static int reserve(Ctx& c, int nF) {
+    const long long need = std::max(radix_workspace(nF + 64LL), scan_workspace(nF + 65LL)) + 256;
+    if (need > c.cap_ws) {
+        c.cap_ws = need + need / 4;
+        PAMO_TRY(s3::dalloc(&c.w.ws, (size_t)c.cap_ws, c.stream));
+    }
+    return s3::tree_alloc(c.w.tree, nF, c.stream);
+}
+
 // max over sqd of the one-sided distances of (VA -> B), written to c.mx
 static int one_side(Ctx& c, const double* VA, int nVA, const double* VB, int nVB, const int* FB, int nFB) {
-    PAMO_TRY(build_tree(c, VB, nVB, FB, nFB, "pamo_hausdorff"));
+    PAMO_CHECK(nFB > 0, "pamo_hausdorff: empty or null mesh");
+    PAMO_TRY(reserve(c, nFB));
+    PAMO_TRY(build_tree(c.w, c.stream, VB, nVB, FB, nFB, "pamo_hausdorff"));
     if (nVA > c.cap_sqd) {
         c.cap_sqd = nVA + nVA / 4;
         PAMO_TRY(s3::dalloc(&c.sqd, (size_t)c.cap_sqd, c.stream));
     }
-    PAMO_TRY(query(c, VA, nVA, VB, FB, c.sqd, nullptr, nullptr));
+    PAMO_TRY(query(c.w, c.stream, VA, nVA, VB, FB, c.sqd, nullptr, nullptr));
     const unsigned nb = std::min<unsigned>(grid_for(nVA, kB), 1024u);
     hipLaunchKernelGGL(max_bits_k, dim3(nb), dim3(kB), 0, c.stream, (const double*)c.sqd, (long long)nVA, c.mx);
     PAMO_HIP(hipGetLastError());
@@ -262,9 +294,9 @@ int pamo_metrics_create(pamo_metrics** out, int device) {
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&c->bad, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->w.flags, 2 * sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void**)&c->mx, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_bad, sizeof(int), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->w.h_flags, 2 * sizeof(int), 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_mx, sizeof(unsigned long long), 0);
     if (e != hipSuccess) {
         pamo::set_error(std::string("pamo_metrics_create: ") + hipGetErrorString(e));
@@ -294,10 +326,12 @@ int pamo_point_mesh_sqdist(pamo_metrics* m, const double* P, long long n_points,
     PAMO_CHECK(n_points >= 0 && (n_points == 0 || (P && sqr_d)), "pamo_point_mesh_sqdist: null argument");
     MCtx& c = *reinterpret_cast<MCtx*>(m);
     PAMO_HIP(hipSetDevice(c.device));
-    PAMO_TRY(pamo::metrics::build_tree(c, V, n_v, F, n_f, "pamo_point_mesh_sqdist"));
-    PAMO_TRY(pamo::metrics::query(c, P, n_points, V, F, sqr_d, face_index, closest));
-    PAMO_HIP(hipStreamSynchronize(c.stream));
-    return 0;
+    PAMO_CHECK(V && F && n_v > 0 && n_f > 0, "pamo_point_mesh_sqdist: empty or null mesh");
+    PAMO_TRY(pamo::metrics::reserve(c, n_f));
+    PAMO_TRY(pamo::metrics::begin(c.w, c.stream));
+    PAMO_TRY(pamo::metrics::build_tree(c.w, c.stream, V, n_v, F, n_f, "pamo_point_mesh_sqdist"));
+    PAMO_TRY(pamo::metrics::query(c.w, c.stream, P, n_points, V, F, sqr_d, face_index, closest));
+    return pamo::metrics::finish(c.w, c.stream, "pamo_point_mesh_sqdist");
 }
 
 int pamo_hausdorff(pamo_metrics* m, const double* VA, int n_va, const int* FA, int n_fa, const double* VB, int n_vb,
@@ -307,11 +341,12 @@ int pamo_hausdorff(pamo_metrics* m, const double* VA, int n_va, const int* FA, i
     MCtx& c = *reinterpret_cast<MCtx*>(m);
     PAMO_HIP(hipSetDevice(c.device));
     PAMO_HIP(hipMemsetAsync(c.mx, 0, sizeof(unsigned long long), c.stream));
+    PAMO_TRY(pamo::metrics::begin(c.w, c.stream));
     // vertices of B against the surface of A, then vertices of A against B
     PAMO_TRY(pamo::metrics::one_side(c, VB, n_vb, VA, n_va, FA, n_fa));
     PAMO_TRY(pamo::metrics::one_side(c, VA, n_va, VB, n_vb, FB, n_fb));
     PAMO_HIP(hipMemcpyAsync(c.h_mx, c.mx, sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
-    PAMO_HIP(hipStreamSynchronize(c.stream));
+    PAMO_TRY(pamo::metrics::finish(c.w, c.stream, "pamo_hausdorff"));
     double d2;
     std::memcpy(&d2, c.h_mx, sizeof(double));
     *d_out = std::sqrt(d2);

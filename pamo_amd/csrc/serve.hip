@@ -105,18 +105,37 @@ struct Engine {
     bool stop = false;
 };
 
+// Grows a slot's staging buffers.  The old buffers are released first (the
+// slot's work on them has finished); a failed allocation leaves the slot
+// with no buffer and zero capacity, so the next job allocates again instead
+// of writing through a stale pointer.
 static int ensure_slot_buffers(Slot& s, int nv, int nf) {
     if ((size_t)nv > s.cap_v) {
         if (s.vbuf) (void)hipFree(s.vbuf);
         if (s.obuf) (void)hipFree(s.obuf);
-        s.cap_v = (size_t)nv + nv / 4 + 64;
-        PAMO_HIP(hipMalloc((void**)&s.vbuf, 12 * s.cap_v));
-        PAMO_HIP(hipMalloc((void**)&s.obuf, 12 * s.cap_v));
+        s.vbuf = s.obuf = nullptr;
+        s.cap_v = 0;
+        const size_t cap = (size_t)nv + nv / 4 + 64;
+        float *vb = nullptr, *ob = nullptr;
+        hipError_t e = hipMalloc((void**)&vb, 12 * cap);
+        if (e == hipSuccess) e = hipMalloc((void**)&ob, 12 * cap);
+        if (e != hipSuccess) {
+            if (vb) (void)hipFree(vb);
+            PAMO_HIP(e);
+        }
+        s.vbuf = vb;
+        s.obuf = ob;
+        s.cap_v = cap;
     }
     if ((size_t)nf > s.cap_f) {
         if (s.fbuf) (void)hipFree(s.fbuf);
-        s.cap_f = (size_t)nf + nf / 4 + 64;
-        PAMO_HIP(hipMalloc((void**)&s.fbuf, 12 * s.cap_f));
+        s.fbuf = nullptr;
+        s.cap_f = 0;
+        const size_t cap = (size_t)nf + nf / 4 + 64;
+        int* fb = nullptr;
+        PAMO_HIP(hipMalloc((void**)&fb, 12 * cap));
+        s.fbuf = fb;
+        s.cap_f = cap;
     }
     return 0;
 }
@@ -402,9 +421,14 @@ int pamo_serve_submit(pamo_server* srv, const float* d_verts, int n_verts, const
     j->iters = iters;
     j->min_verts = min_verts;
     j->owns_out = true;
-    PAMO_HIP(hipSetDevice(e->device));
-    PAMO_HIP(hipMalloc((void**)&j->out_v, 12 * (size_t)n_verts));
-    PAMO_HIP(hipMalloc((void**)&j->out_f, 12 * (size_t)n_faces));
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = hipMalloc((void**)&j->out_v, 12 * (size_t)n_verts);
+    if (err == hipSuccess) err = hipMalloc((void**)&j->out_f, 12 * (size_t)n_faces);
+    if (err != hipSuccess) {
+        if (j->out_v) (void)hipFree(j->out_v);
+        delete j;
+        PAMO_HIP(err);
+    }
     *job_id = pamo::serve::submit(e, j);
     return 0;
 }

@@ -132,8 +132,10 @@ struct Ctx {
     int* hcnt = nullptr;
     // last round, host view
     int n_pts = 0, n_tris = 0, n_edges = 0, n_rec = 0, n_regular = 0, n_vertices_undo = 0;
+    int n_cand = 0;  // records scanned for undo candidates
     int undo_iters = 0;
     int guard_small_stack = 0;  // test option: 32-entry guard frontier
+    long long walk_cap = 0;     // test option: lowers the guard's walk bounds
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -354,9 +356,18 @@ static Orig orig_mesh(Ctx& c) { return Orig{c.op, c.oi, c.oj, c.ok}; }
         PAMO_HIP(hipGetLastError());                                                                      \
     } while (0)
 
+// Reads the device counters; reports (and clears) the error flags that
+// kernels set instead of reading out of bounds or walking a corrupted tree.
 static int sync_counters(Ctx& c) {
     PAMO_HIP(hipMemcpyAsync(c.hcnt, c.dcnt, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
+    if (c.hcnt[C_WALK_ERR] || c.hcnt[C_BAD_UNDO]) {
+        const bool walk = c.hcnt[C_WALK_ERR] != 0;
+        c.hcnt[C_WALK_ERR] = c.hcnt[C_BAD_UNDO] = 0;
+        PAMO_HIP(hipMemsetAsync(c.dcnt + C_WALK_ERR, 0, 2 * sizeof(int), c.stream));
+        PAMO_CHECK(!walk, "guard: a tree walk exceeded its bound (corrupted LBVH)");
+        PAMO_CHECK(false, "undo list: entries outside the points of the previous call");
+    }
     return 0;
 }
 
@@ -426,25 +437,30 @@ static int run_guard(Ctx& c, int nT, bool refit) {
         u.cap = (int)std::min<size_t>(c.alloc_ulist, 0x7fffffff);
         PAMO_HIP(hipMemsetAsync(c.dcnt + C_UNDECIDED, 0, sizeof(int), c.stream));
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, true>), guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
         hipLaunchKernelGGL(exact_pairs_k, dim3(1024), dim3(kBlock), 0, c.stream, m, (const int2*)c.ulist,
                            (const int*)(c.dcnt + C_UNDECIDED), u.cap, c.flagged);
         PAMO_HIP(hipGetLastError());
     } else if (c.guard_small_stack) {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStackSmall, false>), guard_threads, m, c.qlist,
-                c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
+                c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
     } else {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, false>), guard_threads, m, c.qlist, c.dcnt,
-                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log);
+                c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
     }
     return 0;
 }
 
+// Undo candidates among the first n_cand records (get_undo_candidate_kernel,
+// cusimp_free.cu:776-827, whose grid covers the regular-collapse count
+// rounded up to blocks of 512: line-collapse records past it are never
+// candidates).
 static int undo_candidates(Ctx& c, int& npushed) {
-    LAUNCHP(K_UNDO_CAND, 16.0 * c.n_rec + 1.0 * c.n_tris, undo_cand_k, c.n_rec + 1, c.recs, c.n_rec, c.ring_off,
-            c.ring, c.eu, c.ev, c.flagged, c.pflag);
-    PAMO_TRY(scan_exclusive(c.pflag, c.esel, c.n_rec + 1, c.ws, c.stream));
-    LAUNCH(undo_compact_k, std::max(c.n_rec, 1), c.recs, c.n_rec, c.pflag, c.esel, c.pushed, c.dcnt);
+    const int n = c.n_cand;
+    LAUNCHP(K_UNDO_CAND, 16.0 * n + 1.0 * c.n_tris, undo_cand_k, n + 1, c.recs, n, c.ring_off, c.ring, c.eu, c.ev,
+            c.flagged, c.pflag);
+    PAMO_TRY(scan_exclusive(c.pflag, c.esel, n + 1, c.ws, c.stream));
+    LAUNCH(undo_compact_k, std::max(n, 1), c.recs, n, c.pflag, c.esel, c.pushed, c.dcnt);
     PAMO_TRY(sync_counters(c));
     npushed = c.hcnt[C_PUSHED];
     PAMO_CHECK(!c.exact_guard || c.hcnt[C_UNDECIDED] <= (long long)c.alloc_ulist,
@@ -553,11 +569,12 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_TRY(sync_counters(c));
     c.n_regular = c.hcnt[C_NREG];
     c.n_rec = c.n_regular + c.hcnt[C_LINE];
+    c.n_cand = std::min(c.n_rec, (c.n_regular + 511) / 512 * 512);
     // guard + undo loop (:1014-1040)
     int first = c.n_vertices_undo;  // entries already placed by the prologue
     int npushed = 0;
     c.undo_iters = 0;
-    if (c.n_rec > 0) {
+    if (c.n_cand > 0) {
         PAMO_TRY(run_guard(c, nT, false));
         PAMO_TRY(undo_candidates(c, npushed));
     }
@@ -585,7 +602,9 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
 static int prologue(Ctx& c, const int* undo_in, int n_undo_in, bool is_stuck) {
     c.n_vertices_undo = 0;
     if (is_stuck) {
-        if (n_undo_in > 0) LAUNCH(stuck_prologue_k, n_undo_in, undo_in, n_undo_in, c.tmp_undo, c.undo_list, c.pts_map);
+        if (n_undo_in > 0)
+            LAUNCH(stuck_prologue_k, n_undo_in, undo_in, n_undo_in, c.tmp_undo, c.undo_list, c.pts_map,
+                   (int)c.alloc_pts, c.dcnt + C_BAD_UNDO);
         c.n_vertices_undo += n_undo_in;
     } else if (n_undo_in != 0) {
         PAMO_HIP(hipMemsetAsync(c.invalid, 0, c.alloc_pts, c.stream));
@@ -692,6 +711,11 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     if (option == PAMO_OPT_GUARD_SMALL_STACK) {
         PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: value must be 0 or 1");
         c.guard_small_stack = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_WALK_LIMIT) {
+        PAMO_CHECK(value >= 0, "pamo_ctx_set_option: walk limit must be >= 0");
+        c.walk_cap = value;
         return 0;
     }
     PAMO_CHECK(option == PAMO_OPT_EXACT_GUARD, "pamo_ctx_set_option: unknown option");

@@ -67,6 +67,8 @@ enum Counter {
     C_NEDGE = 6,
     C_NQUERY = 7,   // guard queries
     C_UNDECIDED = 8,  // exact guard: pairs the filtered predicates left undecided
+    C_WALK_ERR = 9,   // a guard tree walk exceeded its bound (walk_overrun)
+    C_BAD_UNDO = 10,  // an incoming undo entry outside the context's points
     C_COUNT = 16
 };
 
@@ -133,12 +135,19 @@ __global__ void shuffle_scatter_k(uint32_t domain, uint32_t m, const int* __rest
 }
 
 // ---------------------------------------------------------------- undo-list prologue
+// pts_map holds cap + 1 entries; an entry outside them cannot come from an
+// earlier call of the context: it is reported and maps to 0.
 __global__ void stuck_prologue_k(const int* __restrict__ undo_in, int n, int* tmp_undo, int* undo_list,
-                                 const int* __restrict__ pts_map) {
+                                 const int* __restrict__ pts_map, int cap, int* err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int v = undo_in[i];
     tmp_undo[i] = v;
+    if (v < 0 || v > cap) {
+        atomicOr(err, 1);
+        undo_list[i] = 0;
+        return;
+    }
     undo_list[i] = pts_map[v];
 }
 
@@ -1271,8 +1280,13 @@ template <bool kExact>
 __device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restrict__ changed,
                                                 const float4* __restrict__ node, int n, int x, int stop,
                                                 float4 qlo, float4 qhi, int a, int ia0, int ia1, int ia2,
-                                                float epsilon, uint8_t* __restrict__ flagged, Undecided u) {
-    while (x != stop) {
+                                                float epsilon, uint8_t* __restrict__ flagged, Undecided u,
+                                                int* err, long long max_visits) {
+    for (long long visits = 0; x != stop; ++visits) {
+        if (visits >= max_visits || x < 0) {
+            walk_overrun(err);
+            return;
+        }
         const float4 lo = node[2 * (long long)x], hi = node[2 * (long long)x + 1];
         const int lw = __float_as_int(lo.w), esc = __float_as_int(hi.w);
         if (overlap(qlo, qhi, lo, hi)) {
@@ -1308,7 +1322,8 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
                                                         const float4* __restrict__ fbox,
                                                         const float4* __restrict__ node,
                                                         const int* __restrict__ right, float epsilon,
-                                                        uint8_t* __restrict__ flagged, Undecided u, int part_log) {
+                                                        uint8_t* __restrict__ flagged, Undecided u, int part_log,
+                                                        int* err, long long walk_cap) {
     constexpr int kW = kBlock / kWave;
     __shared__ int2 stk[kW][kStackCap];
     __shared__ int2 pq[kW][kQueue];
@@ -1335,7 +1350,17 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
     }
     wave_sync();
     int size = nl, qn = 0;  // wave-uniform
-    while (size > 0) {
+    // bounds of a valid tree; walk_cap > 0 lowers them (tests of the error path)
+    long long max_steps = (long long)nl * (2LL * n - 1), max_visits = 2LL * n;
+    if (walk_cap > 0) {
+        max_steps = min(max_steps, walk_cap);
+        max_visits = min(max_visits, walk_cap);
+    }
+    for (long long steps = 0; size > 0; ++steps) {
+        if (steps >= max_steps) {  // wave-uniform
+            if (lane == 0) walk_overrun(err);
+            return;
+        }
         const int take = size < kWave ? size : kWave;
         int l = 0, x = -1;
         if (lane < take) {
@@ -1375,7 +1400,7 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
             } else {
                 const int4 id = qid[w][l];
                 guard_walk_subtree<kExact>(m, changed, node, n, c0, esc, qlo, qhi, id.x, id.y, id.z, id.w, epsilon,
-                                           flagged, u);
+                                           flagged, u, err, max_visits);
             }
         }
         size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));

@@ -33,32 +33,6 @@
 namespace pamo {
 namespace s3 {
 
-// Stackless walk of every leaf whose box overlaps [lo, hi]; f(item) per hit.
-template <typename F>
-__device__ __forceinline__ void tree_query(const TreeView& T, V3 lo, V3 hi, F f) {
-    const int n = T.n;
-    if (n <= 0) return;
-    if (n == 1) {
-        if (box_hit(lo, hi, T.box[0], T.box[1])) f((int)T.val[0]);
-        return;
-    }
-    int cur = 0, prev = -1;
-    while (cur >= 0) {
-        const int par = cur == 0 ? -1 : T.parent[cur];
-        int next;
-        if (prev == par) {
-            if (!box_hit(lo, hi, T.box[2 * (long long)cur], T.box[2 * (long long)cur + 1])) next = par;
-            else if (cur >= n - 1) {
-                f((int)T.val[cur - (n - 1)]);
-                next = par;
-            } else next = T.left[cur];
-        } else if (prev == T.left[cur]) next = T.right[cur];
-        else next = par;
-        prev = cur;
-        cur = next;
-    }
-}
-
 __device__ __forceinline__ float box_dist2(V3 p, float4 lo, float4 hi) {
     float dx = fmaxf(fmaxf(lo.x - p.x, 0.0f), p.x - hi.x);
     float dy = fmaxf(fmaxf(lo.y - p.y, 0.0f), p.y - hi.y);
@@ -271,7 +245,7 @@ __global__ void edge_items_k(const float4* __restrict__ x, const int2* __restric
 // subtrees whose box is no closer than the best distance so far skipped.
 __global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__ x, int nV, TreeView T,
                                                      const float4* __restrict__ gv, const int4* __restrict__ gf,
-                                                     float4* __restrict__ target) {
+                                                     float4* __restrict__ target, int* err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nV) return;
     const V3 p = v3(x[i]);
@@ -291,7 +265,12 @@ __global__ void __launch_bounds__(kB) closest_point_k(const float4* __restrict__
         leaf((int)T.val[0]);
     } else if (n > 1) {
         int cur = 0, prev = -1;
-        while (cur >= 0) {
+        const long long max_steps = 3LL * (2LL * n - 1) + 1;
+        for (long long steps = 0; cur >= 0; ++steps) {
+            if (steps >= max_steps || cur > 2 * n - 2) {
+                walk_overrun(err);
+                break;
+            }
             const int par = cur == 0 ? -1 : T.parent[cur];
             int next;
             if (prev == par) {
@@ -358,8 +337,12 @@ __device__ __noinline__ void det_walk_subtree(const float4* __restrict__ x, cons
                                               const int2* __restrict__ edge, const float4* __restrict__ node,
                                               int n_leaves, int cur, int stop, V3 lo, V3 hi, int q, float thr,
                                               float ee_thres, int* counter, int cap, int4* __restrict__ cidx,
-                                              int* __restrict__ ctype1) {
-    while (cur != stop) {
+                                              int* __restrict__ ctype1, int* err) {
+    for (long long visits = 0; cur != stop; ++visits) {
+        if (visits >= 2LL * n_leaves || cur < 0) {
+            walk_overrun(err);
+            return;
+        }
         const float4 blo = node[2 * (long long)cur], bhi = node[2 * (long long)cur + 1];
         const int lw = __float_as_int(blo.w), esc = __float_as_int(bhi.w);
         if (box_hit(lo, hi, blo, bhi)) {
@@ -398,7 +381,7 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
                                                const int2* __restrict__ edge, const float4* __restrict__ node,
                                                const int* __restrict__ right, int n_leaves, float d_hat,
                                                float radius, float ee_thres, int* counter, int cap,
-                                               int4* __restrict__ cidx, int* __restrict__ ctype1) {
+                                               int4* __restrict__ cidx, int* __restrict__ ctype1, int* err) {
     constexpr int kW = kB / kWave;
     __shared__ int2 stk[kW][kDetStack];
     __shared__ int2 pq[kW][kDetQ];
@@ -447,7 +430,12 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
         }
     };
     int size = nl, qn = 0;  // wave-uniform
-    while (size > 0) {
+    const long long max_steps = (long long)nl * (2LL * n_leaves - 1);
+    for (long long steps = 0; size > 0; ++steps) {
+        if (steps >= max_steps) {  // wave-uniform
+            if (lane == 0) walk_overrun(err);
+            return;
+        }
         const int take = size < kWave ? size : kWave;
         int l = 0, cur = -1;
         if (lane < take) {
@@ -488,7 +476,7 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
                 stk[w][size + 2 * below + 1] = make_int2(l, c1);
             } else {
                 det_walk_subtree<kPT>(x, tri, edge, node, n_leaves, c0, esc, lo, hi, (int)q0 + l, thr, ee_thres,
-                                      counter, cap, cidx, ctype1);
+                                      counter, cap, cidx, ctype1, err);
             }
         }
         size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));
@@ -1392,6 +1380,15 @@ static int ensure_contacts(Ctx& c, int n) {
 // _detect_contact (system.py:304-318, energy.py:896-948): refit the edge and
 // triangle trees, gather PT and EE pairs within 2r + d_hat, halve r and
 // retry while the count exceeds max_blocks; q_prev_detection <- q.
+// Reports a tree walk that exceeded its bound (hcount[1] read from ccount[1]
+// at the last synchronisation) and clears the flag.
+static int check_walks(Ctx& c) {
+    if (c.hcount[1] == 0) return 0;
+    c.hcount[1] = 0;
+    PAMO_HIP(hipMemsetAsync(c.ccount + 1, 0, sizeof(int), c.stream));
+    PAMO_CHECK(false, "stage 3: a tree walk exceeded its bound (corrupted LBVH)");
+}
+
 static int detect_contacts(Ctx& c, int& nC) {
     nC = 0;
     if (c.P.mask & PAMO_S3_E_COLLISION) {
@@ -1417,12 +1414,13 @@ static int detect_contacts(Ctx& c, int& nC) {
             const long long ee_threads = ((long long)c.nE + kDetW - 1) / kDetW * kWave;
             L3(detect_k<true>, pt_threads, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.tri_tree.box, (const int*)c.tri_tree.right, c.nF, c.P.d_hat, c.radius, c.P.ee_thres,
-               c.ccount, cap, c.cidx, c.ctype1);
+               c.ccount, cap, c.cidx, c.ctype1, c.ccount + 1);
             L3(detect_k<false>, ee_threads, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.edge_tree.box, (const int*)c.edge_tree.right, c.nE, c.P.d_hat, c.radius,
-               c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1);
-            PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+               c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1, c.ccount + 1);
+            PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
             PAMO_HIP(hipStreamSynchronize(c.stream));
+            PAMO_TRY(check_walks(c));
             const int n = c.hcount[0];
             if (n > c.P.max_blocks) {
                 PAMO_CHECK(c.radius > 1e-30f, "contact detection: more than max_blocks pairs within d_hat");
@@ -1445,7 +1443,7 @@ static int detect_contacts(Ctx& c, int& nC) {
 static int update_target(Ctx& c) {
     if (c.P.mask & PAMO_S3_E_M2GT)
         L3(closest_point_k, c.nV, (const float4*)c.q, c.nV, view(c.gt_tree), (const float4*)c.gt_v,
-           (const int4*)c.gt_f, c.target);
+           (const int4*)c.gt_f, c.target, c.ccount + 1);
     return 0;
 }
 
@@ -1516,7 +1514,9 @@ static int step_once(Ctx& c, pamo_s3_newton_fn on_newton = nullptr, void* user =
         }
     }
     PAMO_HIP(hipMemcpyAsync(c.hsc, c.sc, sizeof(double) * SC_N, hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
+    PAMO_TRY(check_walks(c));
     c.last_energy = c.hsc[SC_E];
     return 0;
 }
@@ -1688,13 +1688,15 @@ int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
     P.k_m2gt = (float)prm->mesh2gt_dist_stiffness;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&c->ccount, sizeof(int));
+    // ccount[0]: contacts found; ccount[1]: a tree walk exceeded its bound
+    if (e == hipSuccess) e = hipMalloc((void**)&c->ccount, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->ccount, 0, 2 * sizeof(int), c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->n_act, sizeof(int));
     if (e == hipSuccess) e = hipMemsetAsync(c->n_act, 0, sizeof(int), c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->sc, sizeof(double) * pamo::s3::SC_N);
     if (e == hipSuccess) e = hipMemsetAsync(c->sc, 0, sizeof(double) * pamo::s3::SC_N, c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, sizeof(float));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcount, sizeof(int), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcount, 2 * sizeof(int), 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hsc, sizeof(double) * pamo::s3::SC_N, 0);
     if (e != hipSuccess) {
         pamo::set_error(std::string("pamo_s3_create: ") + hipGetErrorString(e));

@@ -75,10 +75,9 @@ class CUDSP_Free:
         _check_faces(triangles, n_pts)
         if n_undo < 0 or n_undo > verts_undo.numel():
             raise RuntimeError(f"n_verts_undo={n_undo} outside [0, {verts_undo.numel()}]")
-        if n_undo > 0:
-            u = verts_undo[:n_undo]
-            if not bool(((u >= 0) & (u < n_pts)).all()):
-                raise RuntimeError(f"verts_undo: vertex indices outside [0, {n_pts})")
+        # the entries are vertex ids of the previous call's numbering (the
+        # reference maps them through that call's stale map); the library
+        # rejects ids outside the context's points
         dev = points.device
         out_pts = torch.empty((n_pts, 3), dtype=torch.float32, device=dev)
         out_tris = torch.empty((n_tris, 3), dtype=torch.int32, device=dev)
@@ -186,7 +185,10 @@ class PaMO(torch.nn.Module):
             from . import stage3
             self.config = stage3.Stage3Config()
             self.system = stage3.Stage3System(self.config)
-            # the input surface stays resident on the device for stage 3
+            # the input surface, normalised from its float64 vertices as
+            # process() does (processing.py:15-20,57), stays resident on the
+            # device for stage 3
+            self.s3_target = stage3.normalized_target(self.gt_vertices)
             self.gt_v_dev = torch.as_tensor(self.gt_vertices, dtype=torch.float32).cuda()
             self.gt_f_dev = torch.as_tensor(self.gt_faces, dtype=torch.int32).cuda()
         self.target_faces = None
@@ -205,6 +207,9 @@ class PaMO(torch.nn.Module):
         t1 = time.perf_counter()
         self.rounds = len(hist)
         self.face_hist = hist
+        # the stage-2 result (device, centred) and the centring offset
+        self.stage2_out = (v2, f2)
+        self.tris_mean = tris_mean
         self.timings = {"stage2_s": t1 - t0}
         if self.use_stage3:
             # pamo/__init__.py:191-203: the stage-2 result (+ centring offset,
@@ -212,7 +217,7 @@ class PaMO(torch.nn.Module):
             from . import stage3
             v3 = v2 + torch.from_numpy(tris_mean).to(v2.device)
             verts_np, faces_np = stage3.process(self.gt_v_dev, self.gt_f_dev, v3, f2, 5, system=self.system,
-                                                config=self.config)
+                                                config=self.config, target=self.s3_target)
             self.timings["stage3_s"] = time.perf_counter() - t1
             return verts_np, faces_np.astype(np.int64)
         verts_np = v2.cpu().numpy() + tris_mean

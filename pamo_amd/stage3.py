@@ -344,23 +344,41 @@ def normalization_transform_device(system: "Stage3System", V: torch.Tensor):
     return sc.value, np.array([t[0], t[1], t[2]])
 
 
+def normalized_target(gt_V):
+    """The normalised input surface of process (processing.py:57-58) from
+    the caller's float64 vertices, as the reference computes it: (scale, t)
+    by numpy in float64, gt_V * scale + t in float64, then float32 on the
+    device.  Returns (scale, t, device tensor) for process(target=...)."""
+    V = np.asarray(gt_V, dtype=np.float64)
+    scale, t = normalization_transform(V)
+    gn = torch.from_numpy((V * scale + t).astype(np.float32)).cuda()
+    return scale, t, gn
+
+
 def process(gt_V, gt_F, stage2_V, stage2_F, n_iters, system: Stage3System | None = None,
-            config: Stage3Config | None = None, eval=False, return_curve=False):
+            config: Stage3Config | None = None, eval=False, return_curve=False, target=None):
     """processing.py:23-108: normalise by the input surface, n_iters steps,
     map back.  Inputs may be numpy arrays or CUDA tensors (kept on the
     device).  Returns (float64 vertices, faces) as numpy, and with
     return_curve also the Chamfer / Hausdorff distances after every Newton
     iteration (normalised frame).  eval logs the distances before the first
-    step and after each step (processing.py:66-97)."""
+    step and after each step (processing.py:66-97).  target, from
+    normalized_target(gt_V), supplies the normalisation and the normalised
+    input surface already on the device (a caller projecting several meshes
+    onto one input prepares it once); gt_V is then only used for eval."""
     if system is None:
         system = Stage3System(config if config is not None else Stage3Config())
     else:
         system.clear()
-    if isinstance(gt_V, torch.Tensor) and gt_V.is_cuda:
-        scale, t = normalization_transform_device(system, gt_V)
+    if target is not None:
+        scale, t, gn = target
+        system.set_target(gn, gt_F, 1.0, np.zeros(3))
     else:
-        scale, t = normalization_transform(np.asarray(gt_V))
-    system.set_target(gt_V, gt_F, scale, t)
+        if isinstance(gt_V, torch.Tensor) and gt_V.is_cuda:
+            scale, t = normalization_transform_device(system, gt_V)
+        else:
+            scale, t = normalization_transform(np.asarray(gt_V))
+        system.set_target(gt_V, gt_F, scale, t)
     system.set_mesh(stage2_V, stage2_F, scale, t)
     F = stage2_F.cpu().numpy() if isinstance(stage2_F, torch.Tensor) else np.asarray(stage2_F)
     if not (eval or return_curve):

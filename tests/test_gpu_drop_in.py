@@ -135,3 +135,26 @@ def test_type_errors_name_the_array(torch_cuda):
     with pytest.raises(RuntimeError, match="face indices"):
         op.forward(torch.from_numpy(V).cuda(), torch.from_numpy(bad).cuda(), torch.zeros(0, dtype=torch.int32).cuda(),
                    0, 1.0, 1e-3, False, True)
+
+
+def test_guard_walk_bound_reports_an_error(torch_cuda, oracle_lib):
+    """A guard tree walk that exceeds its bound stops and fails the call with
+    an error status (a corrupted tree would otherwise hold the GPU).  The
+    bound is lowered to one step on a valid tree to reach that path; the
+    same context then runs normally again."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    from stage2_helpers import centred
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(4, 2)
+    Vc = centred(V, F)
+    ctx = _hip.Context(0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_WALK_LIMIT, 1), "set_option")
+    with pytest.raises(_hip.PamoError, match="exceeded its bound"):
+        simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.1, min_verts=0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_WALK_LIMIT, 0), "set_option")
+    _hip.check(_hip.lib().pamo_ctx_reset(ctx.ptr), "reset")
+    vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.1, min_verts=0)
+    vo, fo, ho = oracle_lib.run(Vc, F, 0.1, min_verts=0)
+    assert np.array_equal(hg, ho)
+    assert np.array_equal(fg.cpu().numpy(), fo)

@@ -279,3 +279,49 @@ def test_lbvh_refit_after_undo(torch_cuda, oracle_lib):
 
     run_rounds(counting_fwd, centred(V2, F2), F2, 0.05, on_round=on_round)
     assert seen["checked"], "no round ran the undo loop"
+
+
+def test_scripted_undo_and_line_collapse_rounds_match_oracle(torch_cuda, oracle_lib):
+    """The scripted cases of the oracle pins, through the C ABI: crossing
+    spheres (undo cap, stale maps, a stuck call) and a line collapse outside
+    the undo-candidate grid next to a piercing tetrahedron."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle_stage2_pins import _interpenetrating_spheres, _bipyramid_coincident
+    from pamo_amd.pamo import CUDSP_Free
+    torch = torch_cuda
+    V, F = _interpenetrating_spheres()
+    print(f"case: crossing spheres, {len(F)} faces, 3 scripted calls", flush=True)
+    op, ref = CUDSP_Free(0), oracle_lib.Round()
+    fwd = _gpu_forward(op, torch)
+    scale = float((V.max(0) - V.min(0)).max())
+    undo = np.zeros(0, np.int32)
+    for it, stuck in enumerate((0, 0, 1)):
+        a = fwd(V, F, undo, len(undo), scale, 1e-3, stuck, it == 0)
+        b = ref.forward(V, F, undo, len(undo), scale, 1e-3, stuck, it == 0)
+        _compare_round(it, a, b)
+        assert op.debug(_hip_mod().PAMO_DBG_UNDO_ITERS) == len(ref.undo_iters())
+        pv, pt, occ, mp, undo = b
+        occ = occ.reshape(-1).astype(bool)
+        V, F = pv[occ], mp.reshape(-1)[pt[pt[:, 0] >= 0]].astype(np.int32)
+    Vb, Fb = _bipyramid_coincident()
+    print("case: line collapse beside a piercing tetrahedron", flush=True)
+    c = (Vb[2] + Vb[0] + Vb[3]) / 3
+    n = np.cross(Vb[0] - Vb[2], Vb[3] - Vb[2])
+    n = n / np.linalg.norm(n)
+    T = np.array([c + 0.3 * n, c - 0.3 * n + [0.05, 0, 0], c - 0.3 * n + [-0.03, 0.04, 0],
+                  c - 0.3 * n + [-0.03, -0.04, 0.02]], np.float32)
+    V2 = np.r_[Vb, T].astype(np.float32)
+    F2 = np.r_[Fb, np.array([[5, 6, 7], [5, 7, 8], [5, 8, 6], [6, 8, 7]], np.int32)]
+    op, ref = CUDSP_Free(0), oracle_lib.Round()
+    fwd = _gpu_forward(op, torch)
+    s2 = float((V2.max(0) - V2.min(0)).max())
+    a = fwd(V2, F2, np.zeros(0, np.int32), 0, s2, 1e-9, 0, False)
+    b = ref.forward(V2, F2, np.zeros(0, np.int32), 0, s2, 1e-9, 0, False)
+    _compare_round(0, a, b)
+    assert a[2].reshape(-1)[1] == 0
+
+
+def _hip_mod():
+    from pamo_amd import _hip
+    return _hip
