@@ -22,11 +22,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# HBM bytes per launch over the whole headline run (FETCH_SIZE + WRITE_SIZE,
-# each scaled by what it reads for a 512 MiB copy), from rocprofv3 --pmc runs
-# of this workload; re-measure after changing these kernels.
-TRAFFIC_SOURCE = "profiles/counters_traffic_occupancy_lds.md"
-HBM_TRAFFIC_PER_LAUNCH = {"guard_query": 48.5e6, "edge_cost": 104.7e6}
+# The copy that calibrates the HBM counters: FETCH_SIZE and WRITE_SIZE of a
+# plain device-to-device copy of this size convert the counters to bytes.
+CALIB_COPY_MB = 512
 
 
 def parse():
@@ -35,7 +33,6 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
     p.add_argument("--stage3", type=int, default=1, help="1/0: stage 3 on/off")
     p.add_argument("--exact-guard", action="store_true",
                    help="exact intersection guard instead of the reference's predicate (results differ from the reference)")
@@ -49,22 +46,29 @@ def parse():
     p.add_argument("--slots", type=int, default=8, help="batch / serve: concurrent slots")
     p.add_argument("--serve-seconds", type=float, default=5.0, help="serve: length of the arrival stream")
     p.add_argument("--serve-rate", type=float, default=40.0, help="serve: Poisson arrival rate (jobs/s)")
+    p.add_argument("--no-legs", action="store_true", help="headline only (no icosphere / batch / serve legs)")
+    p.add_argument("--leg-batch-n", type=int, default=24, help="legs: meshes in the batch leg")
+    p.add_argument("--leg-serve-seconds", type=float, default=1.5, help="legs: length of the served stream")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM traffic")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="(child of the traffic passes) one stage-2 run of the workload, then the calibration copy")
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="after the run, one device-to-device copy of this many MiB (HBM counter calibration)")
     return p.parse_args()
 
 
-def bench_batch(args):
+def bench_batch(args, n_meshes=None):
     """Batched stages 2+3 over many meshes (pamo_batch_run) against the plain
     form, one PaMO.run per mesh in sequence, on the same device-resident
-    meshes."""
+    meshes.  Returns the result record."""
     import numpy as np
     import torch
     from pamo_amd import synth
     from pamo_amd.batch import pack_csr, simplify_batch
     from pamo_amd.pamo import PaMO
 
-    meshes, ratios = synth.batch_workload(args.batch_n)
+    meshes, ratios = synth.batch_workload(args.batch_n if n_meshes is None else n_meshes)
     nF = int(sum(len(F) for _, F in meshes))
     print(f"[bench] batch: {len(meshes)} meshes, {nF} faces", file=sys.stderr, flush=True)
     packed = pack_csr(meshes)
@@ -107,21 +111,23 @@ def bench_batch(args):
                                   "{0.005, 0.01, 0.02, 0.05}", "faces_in": nF, "slots": args.slots},
            "plain_meshes_per_s": round(n / t_plain, 3), "speedup_vs_plain": round(t_plain / t_batch, 3),
            "batch_s": round(t_batch, 3), "plain_s": round(t_plain, 3), "faces_identical_to_plain": same_faces}
-    print(json.dumps(res))
+    return res
 
 
-def bench_serve(args):
+def bench_serve(args, seconds=None, one_slot=True):
     """A Poisson request stream served by pamo_serve_* with --slots slots,
-    against FIFO one-at-a-time service of the same stream, both in the plain
-    form (one PaMO.run per request) and through the engine with one slot."""
+    against FIFO one-at-a-time service of the same stream, in the plain form
+    (one PaMO.run per request) and, with one_slot, through the engine with
+    one slot.  Returns the result record."""
     import numpy as np
     import torch
     from pamo_amd import synth, _hip
     from pamo_amd.batch import Server
 
-    pool, jobs = synth.serve_stream(args.serve_seconds, args.serve_rate)
+    seconds = args.serve_seconds if seconds is None else seconds
+    pool, jobs = synth.serve_stream(seconds, args.serve_rate)
     dev = [(torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()) for V, F in pool]
-    print(f"[bench] serve: {len(jobs)} jobs over {args.serve_seconds}s, pool of {len(pool)} meshes",
+    print(f"[bench] serve: {len(jobs)} jobs over {seconds}s, pool of {len(pool)} meshes",
           file=sys.stderr, flush=True)
 
     def run_stream(n_slots):
@@ -177,15 +183,16 @@ def bench_serve(args):
 
     lat, qd, t_all, failed = run_stream(args.slots)
     print(f"[bench] served in {t_all:.1f}s", file=sys.stderr, flush=True)
-    lat1, qd1, t1, failed1 = run_stream(1)
-    print(f"[bench] one-slot fifo in {t1:.1f}s", file=sys.stderr, flush=True)
+    if one_slot:
+        lat1, qd1, t1, failed1 = run_stream(1)
+        print(f"[bench] one-slot fifo in {t1:.1f}s", file=sys.stderr, flush=True)
     latp, tp, failedp = run_plain_fifo()
     print(f"[bench] plain fifo in {tp:.1f}s", file=sys.stderr, flush=True)
     n = len(jobs)
     res = {"metric": "served meshes/s and p50/p99 latency (stages 2+3 on half the jobs)",
            "value": round(n / t_all, 3), "unit": "meshes/s", "n_gpus": 1, "higher_is_better": True,
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {n} jobs, faces "
+           "config": {"workload": f"Poisson {args.serve_rate}/s for {seconds}s, {n} jobs, faces "
                                   f"log-uniform in [10k, 2M] over a pool of {len(pool)} meshes, ratio U[0.002, 0.05]",
                       "slots": args.slots},
            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
@@ -194,12 +201,13 @@ def bench_serve(args):
            "plain_fifo_p50_latency_s": round(float(np.percentile(latp, 50)), 3),
            "plain_fifo_p99_latency_s": round(float(np.percentile(latp, 99)), 3),
            "speedup_vs_plain_fifo": round(tp / t_all, 3),
-           "fifo_meshes_per_s": round(n / t1, 3),
-           "fifo_p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
-           "fifo_p99_latency_s": round(float(np.percentile(lat1, 99)), 3),
-           "speedup_vs_fifo": round(t1 / t_all, 3),
-           "failed_jobs": failed, "fifo_failed_jobs": len(failed1), "plain_fifo_failed_jobs": failedp}
-    print(json.dumps(res))
+           "failed_jobs": failed, "plain_fifo_failed_jobs": failedp}
+    if one_slot:
+        res.update({"fifo_meshes_per_s": round(n / t1, 3),
+                    "fifo_p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
+                    "fifo_p99_latency_s": round(float(np.percentile(lat1, 99)), 3),
+                    "speedup_vs_fifo": round(t1 / t_all, 3), "fifo_failed_jobs": len(failed1)})
+    return res
 
 
 def bench_stress(args):
@@ -266,22 +274,220 @@ def bench_stress(args):
     print(json.dumps(res))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, pm, V, F, ratio, tol, thr, iters, use_s3):
+    """The CPU oracle on the same job.  Returns (cpu_baseline record,
+    stage-2 parity record)."""
+    import numpy as np
+    from oracle import oracle
+    oracle.build()
+    v2, f2 = pm.stage2_out
+    mean = pm.tris_mean
+    # the reference's centring, in numpy
+    mean_np = V[F].mean(axis=1).mean(axis=0)
+    Vc = (V - mean_np).astype(np.float32)
+    print("[bench] cpu baseline: stage-2 oracle, whole run", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    vo, fo, ho = oracle.run(Vc, F, ratio, tol, thr, iters=iters, min_verts=0)
+    t_s2 = time.perf_counter() - t0
+    parity = {"centring_bit_exact": bool(np.array_equal(np.asarray(mean, np.float32).view(np.uint32),
+                                                          mean_np.astype(np.float32).view(np.uint32))),
+              "face_history_equal": bool(np.array_equal(pm.face_hist, ho)),
+              "faces_bit_exact": bool(np.array_equal(f2.cpu().numpy(), fo)),
+              "vertices_bit_exact": bool(np.array_equal(v2.cpu().numpy().view(np.uint32), vo.view(np.uint32))),
+              "oracle_rounds": int(len(ho))}
+    print(f"[bench] stage-2 oracle {t_s2:.1f}s, parity {parity}", file=sys.stderr, flush=True)
+    cores = oracle.lib().orc_num_threads()
+    rec = {"value": round(t_s2, 3), "unit": "s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+           "stage2_s": round(t_s2, 3),
+           "sample": f"stage 2: the whole oracle run ({len(ho)} rounds, same input and parameters)"}
+    if use_s3:
+        from pamo_amd import stage3
+        from oracle import stage3 as s3o
+        sV = (vo + mean_np).astype(np.float32)
+        scale, t = stage3.normalization_transform(V.astype(np.float64))
+        gn = (V.astype(np.float64) * scale + t).astype(np.float32)
+        vn = (sV.astype(np.float64) * scale + t).astype(np.float32)
+        vor = stage3.voronoi_areas(sV.astype(np.float64) * scale + t, fo).astype(np.float32)
+        E, H = stage3.edges_unique(fo), stage3.hinges(fo)
+        # the closest points on the input surface come from the device: the
+        # oracle finds them by brute force over the 2M input faces, which is
+        # not a CPU implementation worth timing
+        sysg = stage3.Stage3System()
+        sysg.set_target(torch_from(gn), torch_from(F), 1.0, np.zeros(3))
+        sysg.set_mesh(torch_from(vn), torch_from(fo), 1.0, np.zeros(3))
+        from pamo_amd import _hip
+        tg = sysg.debug(_hip.PAMO_S3_DBG_TARGET).reshape(-1, 3)
+        times = {}
+        for n_newton in (1, 2):
+            o = s3o.System(fp64=True, n_newton=n_newton)
+            o.set_target(gn, F)
+            o.set_mesh(vn, fo, E, H, vor)
+            o.set_targets(tg)
+            t0 = time.perf_counter()
+            o.step(1, keep_target=True)
+            times[n_newton] = time.perf_counter() - t0
+            print(f"[bench] stage-3 oracle: one step of {n_newton} Newton iterations {times[n_newton]:.1f}s",
+                  file=sys.stderr, flush=True)
+        t_newton = max(times[2] - times[1], 0.0)
+        t_s3 = 50 * t_newton
+        rec["stage3_s"] = round(t_s3, 3)
+        rec["value"] = round(t_s2 + t_s3, 3)
+        rec["sample"] += (f"; stage 3: one step of 1 and of 2 Newton iterations (FP64) on the oracle's stage-2 "
+                          f"output ({times[1]:.1f}s, {times[2]:.1f}s) give {t_newton:.2f}s per Newton iteration, "
+                          f"scaled to the job's 5 steps x 10 iterations; the 5 closest-point updates against the "
+                          f"2M-face input are not included (targets taken from the device)")
+    return rec, parity
+
+
+def torch_from(a):
+    import torch
+    return torch.from_numpy(a).cuda()
+
+
+def leg_icosphere(args):
+    """The icosphere workload (level 5, spherical-harmonic displacement,
+    20,480 faces, ratio 0.05, stage 2 only) through PaMO.run, against the
+    oracle on the reference-centred input."""
+    import numpy as np
+    import torch
+    from pamo_amd import synth
+    from pamo_amd.pamo import PaMO
+    from oracle import oracle
+    V, F = synth.icosphere_sh(5, 0)
+    Vd, Fd = torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()
+    pm = PaMO((V, F), use_stage1=False, use_stage3=False)
+    for _ in range(2):
+        pm.run(Vd, Fd, 0.05, 4, 1e-3, 100000, 0)
+    ts = []
+    for _ in range(7):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        vo, fo = pm.run(Vd, Fd, 0.05, 4, 1e-3, 100000, 0)
+        ts.append(time.perf_counter() - t0)
+    mean = V[F].mean(axis=1).mean(axis=0)
+    t0 = time.perf_counter()
+    vr, fr, hr = oracle.run((V - mean).astype(np.float32), F, 0.05, min_verts=0)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "icosphere level 5 + spherical-harmonic displacement (seed 0), 20,480 faces, ratio 0.05, "
+                        "stage 2 only", "value": round(float(np.median(ts)), 5), "unit": "s",
+            "faces_out": int(len(fo)), "rounds": int(pm.rounds), "runs": len(ts),
+            "cpu_oracle_s": round(t_cpu, 4), "cpu_cores": oracle.lib().orc_num_threads(),
+            "bit_exact_vs_oracle": bool(np.array_equal(fo, fr) and
+                                        np.array_equal(vo.view(np.uint32), (vr + mean).view(np.uint32)))}
+
+
+def legs(args):
+    """The other workloads, at sizes that keep the default run within a few
+    minutes: the icosphere, a batch of --leg-batch-n meshes against the plain
+    form, and a --leg-serve-seconds Poisson stream against plain FIFO."""
+    out = {}
+    print("[bench] leg: icosphere", file=sys.stderr, flush=True)
+    out["icosphere_stage2"] = leg_icosphere(args)
+    print("[bench] leg: batch", file=sys.stderr, flush=True)
+    out["batch"] = bench_batch(args, n_meshes=args.leg_batch_n)
+    print("[bench] leg: serve", file=sys.stderr, flush=True)
+    out["serve"] = bench_serve(args, seconds=args.leg_serve_seconds, one_slot=False)
+    return out
+
+
+def traffic_probe():
+    """The program the two counter passes profile: one stage-2 run of the
+    headline workload on a fresh object, then a plain copy of CALIB_COPY_MB."""
+    import torch
+    from pamo_amd import synth
+    from pamo_amd.pamo import PaMO
+    V, F = synth.headline_mesh(1)
+    Vd, Fd = torch.from_numpy(V).cuda(), torch.from_numpy(F).cuda()
+    pm = PaMO((V, F), use_stage1=False, use_stage3=False)
+    pm.run(Vd, Fd, 0.01, 4, 1e-3, 100000, 0)
+    torch.cuda.synchronize()
+    src = torch.empty(CALIB_COPY_MB << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[probe] {pm.rounds} rounds, copy of {CALIB_COPY_MB} MiB", file=sys.stderr, flush=True)
+
+
+def measure_traffic(kernel):
+    """HBM bytes per launch of `kernel` (a kernel-name fragment): two
+    rocprofv3 --pmc passes over traffic_probe, FETCH_SIZE then WRITE_SIZE,
+    each scaled by what the counter reads for the calibration copy.  Returns
+    (bytes per launch, detail) or (None, reason)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    per = {}
+    detail = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="pamo_pmc_")
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", f"{kernel}|copyBuffer", "--output-format", "csv",
+               "-d", out, "-o", "probe", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe"]
+        print(f"[bench] traffic pass: {counter}", file=sys.stderr, flush=True)
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=180, text=True)
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None, f"{counter} pass wrote no counters (rc {r.returncode}): {r.stderr[-300:]}"
+        vals, copy = [], []
+        for row in csv.DictReader(open(files[0])):
+            if row.get("Counter_Name") != counter:
+                continue
+            name, v = row["Kernel_Name"], float(row["Counter_Value"])
+            if kernel in name:
+                vals.append(v)
+            elif "copyBuffer" in name:
+                copy.append(v)
+        shutil.rmtree(out, ignore_errors=True)
+        if not vals or not copy:
+            return None, f"{counter}: {len(vals)} kernel and {len(copy)} copy dispatches"
+        # the calibration copy is the largest copy dispatch
+        calib = max(copy)
+        expect = (CALIB_COPY_MB << 20) / 1024.0  # KiB the copy reads and writes
+        factor = expect / calib if calib > 0 else float("nan")
+        per[counter] = sum(vals) / len(vals) * 1024.0 * factor
+        detail[counter] = {"launches": len(vals), "avg_kib": round(sum(vals) / len(vals), 1),
+                           "calib_kib": round(calib, 1), "factor": round(factor, 4)}
+    return per["FETCH_SIZE"] + per["WRITE_SIZE"], detail
+
+
 def main():
     args = parse()
+    if args.traffic_probe:
+        return traffic_probe()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
               "build", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)))
+        return
     if args.workload == "serve":
-        return bench_serve(args)
+        print(json.dumps(bench_serve(args)))
+        return
     if args.workload == "stress":
         return bench_stress(args)
     import numpy as np
     import torch
     from pamo_amd import synth, _hip
-    from pamo_amd.pamo import PaMO, face_centroid_mean
+    from pamo_amd.pamo import PaMO
 
     ratio, tol, thr, iters = 0.01, 4, 1e-3, args.iters
     V, F = synth.headline_mesh(1)
@@ -296,6 +502,11 @@ def main():
     prof = _hip.Profiler(ctx)
 
     def one_run():
+        # each run starts from a fresh object's stage-2 state: the context
+        # keeps its allocations but restarts its persistent buffers at zero
+        # (a second run() on one reference PaMO object would start from the
+        # stale buffers the first left, and take different decisions)
+        _hip.check(_hip.lib().pamo_ctx_reset(ctx.ptr), "pamo_ctx_reset")
         vo, fo = pm.run(Vd, Fd, ratio, tol, thr, iters, 0)
         return vo, fo, pm.face_hist
 
@@ -335,10 +546,19 @@ def main():
     # roofline of the dominant kernel (live HIP-event timing inside the library)
     dom = [s for s in prof.stats() if s["name"] == dom_name][0]
     achieved = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9 if dom["total_ms"] > 0 else 0.0
-    traffic = HBM_TRAFFIC_PER_LAUNCH.get(dom["name"])
+    traffic, tdetail = (None, "not measured (--no-traffic)")
+    if not args.no_traffic:
+        traffic, tdetail = measure_traffic(dom["name"] + "_k")
+        print(f"[bench] traffic of {dom['name']}: {traffic} bytes per launch ({tdetail})", file=sys.stderr,
+              flush=True)
     roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": 8000.0,
-                "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": traffic,
-                "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
+                "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                "traffic": round(traffic) if traffic is not None else None,
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over one stage-2 run of the same workload "
+                                   f"(bench.py --traffic-probe), scaled by a {CALIB_COPY_MB} MiB copy")
+                if traffic is not None else None,
+                "traffic_detail": tdetail,
+                "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
                 "avg_launch_us": round(1e3 * dom["total_ms"] / max(dom["launches"], 1), 2),
                 "launches": dom["launches"]}
 
@@ -351,34 +571,14 @@ def main():
                "metric_s": round(time.perf_counter() - tq, 3)}
     print(f"[bench] quality: Hausdorff {hd:.4e}, Chamfer {cd:.4e}", file=sys.stderr, flush=True)
 
-    # CPU baseline: the oracle on the first rounds of the same workload,
-    # scaled to the whole job by faces processed.
-    cpu = None
+    # CPU baseline, stages 2+3, on this box's host cores: the whole stage-2
+    # oracle run (its output doubles as a bit-exactness check of this run's
+    # stage 2), and the stage-3 oracle timed for one and two Newton
+    # iterations of a step on the stage-2 output, scaled to the job's five
+    # steps of ten iterations.
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
-        from oracle import oracle
-        oracle.build()
-        mean = face_centroid_mean(ctx, Vd, Fd)
-        Vc = (V - mean).astype(np.float32)
-        r = oracle.Round()
-        budget = args.cpu_seconds
-        rounds_done, frs, t_cpu = 0, 0, 0.0
-        # one round at a time on the oracle until the budget is spent
-        hist_gpu = list(hist)
-        cur_rounds = 1
-        while True:
-            tc0 = time.perf_counter()
-            _, _, h = oracle.run(Vc, F, ratio, tol, thr, iters=cur_rounds, min_verts=0, ctx=oracle.Round())
-            dt = time.perf_counter() - tc0
-            rounds_done = len(h)
-            frs = int(nF0 + h[:-1].sum())
-            t_cpu = dt
-            if dt > budget / 3 or rounds_done >= len(hist_gpu):
-                break
-            cur_rounds *= 2
-        est = t_cpu * face_rounds / frs
-        cpu = {"value": round(est, 3), "unit": "s", "cores": oracle.lib().orc_num_threads(), "kind": "port",
-               "sample": f"oracle on the first {rounds_done} rounds ({frs} face-rounds of {face_rounds}) of the "
-                         f"same workload in {t_cpu:.2f}s, scaled by face-rounds"}
+        cpu, parity = cpu_baseline(args, pm, V, F, ratio, tol, thr, iters, use_s3)
 
     out = {
         "metric": "2M->20k faces wall-time (stages 2+3)" if use_s3 else "2M->20k faces wall-time (stage 2)",
@@ -403,9 +603,12 @@ def main():
         "face_rounds": face_rounds,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "stage2_vs_oracle": parity,
         "quality": quality,
         "kernels": stats,
     }
+    if not args.no_legs:
+        out["legs"] = legs(args)
     if args.calib_copy_mb > 0:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
         # convert the counters of the kernels above into bytes

@@ -47,6 +47,23 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * corrupted tree.  A value > 0 lowers both bounds to it, so that tests reach
  * that error path on a valid tree; 0 (default) keeps the bounds. */
 #define PAMO_OPT_WALK_LIMIT 3
+/* Collapse position (no counterpart in the reference, which always collapses
+ * an edge to its midpoint, simp_cuda/src/cusimp_free.cu:491,687): 0 (default)
+ * the midpoint, bit-exact with the reference; 1 optimal placement -- the
+ * minimiser of the summed quadric Q_u + Q_v from its 3x3 system solved in
+ * FP64, falling back to the midpoint when the system is singular or the
+ * minimiser lies farther from the midpoint than the edge is long.  The cost,
+ * flip and skinny tests and the collapse all use the chosen position.  A
+ * context keeps the setting across calls. */
+#define PAMO_OPT_PLACEMENT 4
+/* Guard tree reuse (no counterpart in the reference, which builds an LBVH
+ * at every check, simp_cuda/src/bvh/self_intersect.cuh:151-392): the fused
+ * loop keeps the guard tree of one round for the next ones -- its leaves are
+ * renumbered at each compaction, removed faces keep empty boxes, and each
+ * round refits the boxes -- and builds a new one once fewer than this
+ * percentage of its leaves are live (default 50; 100 builds every round).
+ * The flags, hence every result, do not depend on the tree. */
+#define PAMO_OPT_TREE_KEEP 5
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).
@@ -171,6 +188,8 @@ typedef struct pamo_s3_params {
     double elas_poisson_ratio;       /* 0.0  */
     double hinge_stiffness;          /* 1e-2 */
     double mesh2gt_dist_stiffness;   /* 1e3  */
+    double min_angle_thres;          /* 0.0: a mesh whose smallest angle (degrees) is at most this
+                                        is not moved (system.py:217-222) */
 } pamo_s3_params;
 
 /* Replaces Stage3System.__init__ (system.py:20-70). */

@@ -68,6 +68,10 @@ def lib():
                                        c_vp]
         L.orc_pasp_run.restype = c_int
         L.orc_pasp_run.argtypes = [c_vp, c_vp, c_int, c_vp, c_int, c_float, c_ll, c_vp, c_vp, c_vp, c_vp]
+        L.orc_set_placement.restype = None
+        L.orc_set_placement.argtypes = [c_vp, c_int]
+        L.orc_place_optimal.restype = None
+        L.orc_place_optimal.argtypes = [c_vp, c_vp, c_vp, c_vp]
         L.orc_run.restype = c_int
         L.orc_run.argtypes = [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_float, c_ll, c_ll, c_int, c_vp, c_vp,
                               c_vp, c_vp, c_vp, c_int]
@@ -111,6 +115,10 @@ class Round:
         if getattr(self, "ctx", None):
             lib().orc_ctx_destroy(self.ctx)
             self.ctx = None
+
+    def set_placement(self, mode: int):
+        """0: midpoint collapses (the reference); 1: optimal placement."""
+        lib().orc_set_placement(self.ctx, int(mode))
 
     def forward(self, points, triangles, verts_undo, n_verts_undo, scale, threshold, is_stuck, init, pred_mode=0):
         P = _f32(points).reshape(-1, 3)
@@ -213,13 +221,26 @@ def ref_guard(cur9, ci, qry9, qi, epsilon=1e-3) -> bool:
     return bool(lib().orc_ref_guard(a.ctypes.data, ia.ctypes.data, b.ctypes.data, ib.ctypes.data, float(epsilon)))
 
 
+def place_optimal(M10, p0, p1):
+    """The oracle's optimal collapse position for M = Q_u + Q_v (10 floats:
+    00 01 02 03 11 12 13 22 23 33) and the edge's end points."""
+    M = np.ascontiguousarray(M10, np.float32).reshape(10)
+    a = _f32(p0).reshape(3)
+    b = _f32(p1).reshape(3)
+    out = np.empty(3, np.float32)
+    lib().orc_place_optimal(M.ctypes.data, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
 def run(verts, faces, ratio, tolerance=4, threshold=1e-3, iters=1000000, min_verts=10000000000, pred_mode=0,
-        ctx: Round | None = None):
+        ctx: Round | None = None, placement: int = 0):
     """The PaMO.run stage-2 loop on already-centred vertices."""
     V = _f32(verts).reshape(-1, 3)
     F = _i32(faces).reshape(-1, 3)
     own = ctx is None
     r = Round() if own else ctx
+    if own:
+        r.set_placement(placement)
     ov = np.empty_like(V)
     of = np.empty_like(F)
     nv = c_int()

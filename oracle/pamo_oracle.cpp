@@ -527,6 +527,7 @@ struct Ctx {
     std::vector<int> undo_iters_edges;  // debug: undone edge count per iteration
     int n_vertices_undo = 0;
     int pred_mode = 0;
+    int placement = 0;               // 0 midpoint (the reference), 1 optimal (opt-in)
     long long guard_pairs = 0;       // ordered pairs evaluated (debug)
 };
 
@@ -626,6 +627,53 @@ static inline float heron(V3 a, V3 b, V3 d) {
 }
 
 // compute_edge_cost_kernel, cusimp_free.cu:382-585
+// Opt-in optimal placement (no counterpart in the reference, which always
+// collapses to the midpoint, cusimp_free.cu:491,687): the minimiser of
+// x^T M x over x = (p, 1) for M = Q_u + Q_v (10 floats 00 01 02 03 11 12 13
+// 22 23 33), from the 3x3 system A p = -b solved in FP64 by cofactors.  The
+// midpoint is kept when A is singular (|det| <= 1e-9 max|a_ij|^3, NaN
+// included) or when the minimiser lies farther from the midpoint than the
+// edge is long.
+static V3 place_optimal(const float* M, V3 p0, V3 p1) {
+    const V3 mid = vdivs(vadd(p0, p1), 2.0f);
+    const double a00 = M[0], a01 = M[1], a02 = M[2], a11 = M[4], a12 = M[5], a22 = M[7];
+    const double b0 = -(double)M[3], b1 = -(double)M[6], b2 = -(double)M[8];
+    const double c00 = a11 * a22 - a12 * a12;
+    const double c01 = a02 * a12 - a01 * a22;
+    const double c02 = a01 * a12 - a02 * a11;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    double amax = std::fabs(a00);
+    for (double a : {a01, a02, a11, a12, a22}) amax = std::fmax(amax, std::fabs(a));
+    if (!(std::fabs(det) > 1e-9 * amax * amax * amax)) return mid;
+    const double c11 = a00 * a22 - a02 * a02;
+    const double c12 = a01 * a02 - a00 * a12;
+    const double c22 = a00 * a11 - a01 * a01;
+    const double x = (c00 * b0 + c01 * b1 + c02 * b2) / det;
+    const double y = (c01 * b0 + c11 * b1 + c12 * b2) / det;
+    const double z = (c02 * b0 + c12 * b1 + c22 * b2) / det;
+    const double ex = (double)p1.x - (double)p0.x, ey = (double)p1.y - (double)p0.y, ez = (double)p1.z - (double)p0.z;
+    const double dx = x - (double)mid.x, dy = y - (double)mid.y, dz = z - (double)mid.z;
+    if (!(dx * dx + dy * dy + dz * dz <= ex * ex + ey * ey + ez * ez)) return mid;
+    return V3{(float)x, (float)y, (float)z};
+}
+
+// M = Q_u + Q_v in the 10-float storage order.
+static void edge_quadric(const Ctx& c, int u, int v, float* M) {
+    const float* qu = &c.Q[(size_t)u * 10];
+    const float* qv = &c.Q[(size_t)v * 10];
+    for (int i = 0; i < 10; ++i) M[i] = qu[i] + qv[i];
+}
+
+// The collapse position of edge (u, v).
+static V3 collapse_point(const Ctx& c, int u, int v) {
+    if (c.placement == 1) {
+        float M[10];
+        edge_quadric(c, u, v, M);
+        return place_optimal(M, c.P[u], c.P[v]);
+    }
+    return vdivs(vadd(c.P[u], c.P[v]), 2.0f);
+}
+
 static void edge_costs(Ctx& c, bool is_stuck, float edge_s, float tres) {
     const float k4s3 = 4.0f * std::sqrt(3.0f);
 #pragma omp parallel for schedule(dynamic, 256)
@@ -662,7 +710,7 @@ static void edge_costs(Ctx& c, bool is_stuck, float edge_s, float tres) {
         }
         elen = elen / (float)nedge / edge_s * tres;
         elen = elen + vnorm(vsub(p0, p1)) / edge_s * tres;
-        V3 vm = vdivs(vadd(p0, p1), 2.0f);
+        V3 vm = collapse_point(c, u, v);
         float qa = 0.0f;
         int ntri = 0;
         bool flip = false;
@@ -893,7 +941,7 @@ static void forward(Ctx& c, const float* pts, int nP, const int* tris, int nT, c
         if (!sel[e]) continue;
         int u = c.eu[e], v = c.ev[e];
         c.collapsed.push_back(e);
-        V3 vm = vdivs(vadd(c.P[u], c.P[v]), 2.0f);
+        V3 vm = collapse_point(c, u, v);
         c.P[u] = vm;
         c.P[v] = V3{0, 0, 0};
         c.occ[v] = 0;
@@ -1128,6 +1176,17 @@ static void pasp_forward(Ctx& c, const float* pts, int nP, const int* tris, int 
 extern "C" {
 
 void* orc_ctx_create() { return new orc::Ctx(); }
+
+// 0 midpoint (default), 1 optimal placement; kept across calls.
+void orc_set_placement(void* ctx, int mode) { ((orc::Ctx*)ctx)->placement = mode; }
+
+// place_optimal on a caller's M (10 floats) and edge end points.
+void orc_place_optimal(const float* M10, const float* p0, const float* p1, float* out3) {
+    orc::V3 r = orc::place_optimal(M10, orc::V3{p0[0], p0[1], p0[2]}, orc::V3{p1[0], p1[1], p1[2]});
+    out3[0] = r.x;
+    out3[1] = r.y;
+    out3[2] = r.z;
+}
 void orc_ctx_destroy(void* p) { delete (orc::Ctx*)p; }
 
 int orc_num_threads() {

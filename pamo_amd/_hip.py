@@ -16,6 +16,8 @@ c_int, c_float, c_double, c_ll, c_vp = ctypes.c_int, ctypes.c_float, ctypes.c_do
 PAMO_OPT_GUARD_SMALL_STACK = 1
 PAMO_OPT_EXACT_GUARD = 2
 PAMO_OPT_WALK_LIMIT = 3
+PAMO_OPT_PLACEMENT = 4
+PAMO_OPT_TREE_KEEP = 5
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3
@@ -56,7 +58,7 @@ class S3Params(ctypes.Structure):
         ("max_blocks", c_int), ("ccd_max_iters", c_int), ("d_hat", c_double), ("coll_stiffness", c_double),
         ("contact_detection_radius", c_double), ("ee_classify_thres", c_double), ("ccd_slackness", c_double),
         ("ccd_thickness", c_double), ("elas_young_modulus", c_double), ("elas_poisson_ratio", c_double),
-        ("hinge_stiffness", c_double), ("mesh2gt_dist_stiffness", c_double),
+        ("hinge_stiffness", c_double), ("mesh2gt_dist_stiffness", c_double), ("min_angle_thres", c_double),
     ]
 
 

@@ -136,6 +136,14 @@ struct Ctx {
     int undo_iters = 0;
     int guard_small_stack = 0;  // test option: 32-entry guard frontier
     long long walk_cap = 0;     // test option: lowers the guard's walk bounds
+    int placement = 0;          // 0 midpoint collapses (the reference), 1 optimal placement
+    // guard tree carried across rounds (pamo_simplify's loop): the leaves of
+    // the last build, renumbered at each compaction, dead ones with empty
+    // boxes; rebuilt once fewer than tree_keep_pct % of them are live
+    int tree_keep_pct = 50;     // 100: rebuild every round
+    bool tree_valid = false;    // bval / hierarchy describe the current faces
+    bool tree_fresh = false;    // the next round may refit instead of building
+    int tree_n = 0;             // leaves of the last tree, as last read back
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -383,9 +391,9 @@ static int sync_counters(Ctx& c) {
 static int run_guard(Ctx& c, int nT, bool refit) {
     Mesh m = cur_mesh(c);
     const double nP = c.n_pts;
-    const int nparts = (int)grid_for(nT, kBlock);
+    const int nparts = (int)grid_for(nT + 1, kBlock);  // face_box_k also writes the sentinel box at nT
     int* leaf_part = (int*)(c.box_part + 6 * nparts + 6);
-    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT + 4.0 * nT, face_box_k, nT, m, nT, c.fbox, c.box_part,
+    LAUNCHP(K_FACE_BOX, 12.0 * nT + 16.0 * nP + 32.0 * nT + 4.0 * nT, face_box_k, nT + 1, m, nT, c.fbox, c.box_part,
             leaf_part, c.flagged, (const int*)c.oi);
     if (!refit) {
         hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, c.stream, c.box_part, nparts,
@@ -397,27 +405,30 @@ static int run_guard(Ctx& c, int nT, bool refit) {
         LAUNCHP(K_KARRAS, 8.0 * nT + 16.0 * nT + 8.0 * nT, karras_k, nT, c.bkey, c.bval, c.dcnt, c.bleft, c.bright,
                 c.bparent, c.brange);
     }
+    // leaves of the tree: at most nT for a tree built this round, the last
+    // build's count for a tree carried across rounds
+    const int nL = refit ? std::max(nT, c.tree_n) : nT;
     // node boxes from leaf ranges: no communication between workgroups
-    const unsigned nb0 = grid_for(nT, kBlock), nb1 = grid_for(nb0, kBlock);
+    const unsigned nb0 = grid_for(nL, kBlock), nb1 = grid_for(nb0, kBlock);
     prof_begin(K_BVH_BOXES, 4.0 * nT + 32.0 * nT + 32.0 * nT + 64.0 * nT, c.stream);
     hipLaunchKernelGGL(box_scan_k, dim3(nb0), dim3(kBlock), 0, c.stream, (const float4*)c.fbox,
                        (const uint32_t*)c.bval, (const int*)c.dcnt, 0, c.bbox, c.bpre, c.bsuf, c.blk);
     hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, c.stream, (const float4*)c.blk,
                        (const uint32_t*)nullptr, (const int*)c.dcnt, 1, (float4*)nullptr, c.blkpre, c.blksuf,
                        c.sblk);
-    hipLaunchKernelGGL(node_box_k, dim3(grid_for(nT, kBlock)), dim3(kBlock), 0, c.stream, (const int2*)c.brange,
+    hipLaunchKernelGGL(node_box_k, dim3(grid_for(nL, kBlock)), dim3(kBlock), 0, c.stream, (const int2*)c.brange,
                        (const int*)c.dcnt, (const float4*)c.bbox + 0, (const float4*)c.bpre, (const float4*)c.bsuf,
                        (const float4*)c.blk, (const float4*)c.blkpre, (const float4*)c.blksuf,
                        (const float4*)c.sblk, c.bbox);
-    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * nT, kBlock)), dim3(kBlock), 0, c.stream, (const int*)c.bleft,
+    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * nL, kBlock)), dim3(kBlock), 0, c.stream, (const int*)c.bleft,
                        (const int*)c.bright, (const int*)c.bparent, (const int*)c.dcnt, (const uint32_t*)c.bval,
                        c.bbox);
     prof_end(K_BVH_BOXES, c.stream);
     PAMO_HIP(hipGetLastError());
     // queries: changed live faces in Morton order
-    LAUNCH(query_flag_k, nT + 1, c.bval, c.dcnt, c.changed, (const int*)c.ti, nT, c.face_scan);
-    PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nT + 1, c.ws, c.stream));
-    LAUNCH(query_list_k, nT, c.bval, c.face_scan, c.pflag, nT, c.qlist, c.dcnt);
+    LAUNCH(query_flag_k, nL + 1, c.bval, c.dcnt, c.changed, (const int*)c.ti, nT, nL, c.face_scan);
+    PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nL + 1, c.ws, c.stream));
+    LAUNCH(query_list_k, nL, c.bval, c.face_scan, c.pflag, nL, c.qlist, c.dcnt);
     // meshes up to 256k faces split every query group over 8 subtrees: late
     // rounds hold few queries, and one query with a large box would otherwise
     // hold a whole launch in one wave
@@ -463,6 +474,7 @@ static int undo_candidates(Ctx& c, int& npushed) {
     LAUNCH(undo_compact_k, std::max(n, 1), c.recs, n, c.pflag, c.esel, c.pushed, c.dcnt);
     PAMO_TRY(sync_counters(c));
     npushed = c.hcnt[C_PUSHED];
+    c.tree_n = c.hcnt[C_NLIVE];  // leaves of the tree this guard used
     PAMO_CHECK(!c.exact_guard || c.hcnt[C_UNDECIDED] <= (long long)c.alloc_ulist,
                "exact guard: more pairs left undecided than the list holds");
     return 0;
@@ -544,14 +556,25 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
-    LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE,
-            edge_cost_k, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid, c.Q, cp, c.edge_cost);
+    const double cost_bytes = 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE;
+    if (c.placement)
+        LAUNCHP(K_EDGE_COST, cost_bytes, edge_cost_k<true>, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid,
+                c.Q, cp, c.edge_cost);
+    else
+        LAUNCHP(K_EDGE_COST, cost_bytes, edge_cost_k<false>, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE,
+                c.invalid, c.Q, cp, c.edge_cost);
     // independent set + collapse
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
             (unsigned long long*)c.vmin);
-    LAUNCHP(K_SELECT_COLLAPSE, 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP, select_collapse_k, nE + 1, m,
-            orig_mesh(c), c.occ, c.ring_off, c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost,
-            (const unsigned long long*)c.vmin, collapse_t, c.esel);
+    const double sel_bytes = 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP;
+    if (c.placement)
+        LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, select_collapse_k<true>, nE + 1, m, orig_mesh(c), c.occ, c.ring_off,
+                c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
+                c.esel, (const float4*)c.Q);
+    else
+        LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, select_collapse_k<false>, nE + 1, m, orig_mesh(c), c.occ, c.ring_off,
+                c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
+                c.esel, (const float4*)c.Q);
     PAMO_TRY(scan_exclusive(c.esel, c.pflag, nE + 1, c.ws, c.stream));
     LAUNCH(compact_k, nE, c.esel, c.pflag, nE, c.recs, 0);
     hipLaunchKernelGGL(store_count_k, dim3(1), dim3(1), 0, c.stream, c.pflag, nE, c.dcnt, (int)C_NREG);
@@ -575,7 +598,9 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     int npushed = 0;
     c.undo_iters = 0;
     if (c.n_cand > 0) {
-        PAMO_TRY(run_guard(c, nT, false));
+        // a tree carried from the last round is refit; otherwise built
+        PAMO_TRY(run_guard(c, nT, c.tree_valid && c.tree_fresh));
+        c.tree_valid = true;
         PAMO_TRY(undo_candidates(c, npushed));
     }
     c.n_vertices_undo += 2 * npushed;
@@ -615,14 +640,32 @@ static int prologue(Ctx& c, const int* undo_in, int n_undo_in, bool is_stuck) {
 
 // Drop removed vertices and faces and renumber (pamo/__init__.py:163-167):
 // stable compaction by pts_map and by a scan of live faces.
-static int compact_round(Ctx& c, int nP, int nT) {
+// With carry, the guard tree's leaves are renumbered to the compacted
+// faces (carry_leaves_k) and the next round refits it unless fewer than
+// tree_keep_pct % of its leaves are still live.
+static int compact_round(Ctx& c, int nP, int nT, bool carry = false) {
     Mesh m = cur_mesh(c);
     LAUNCH(live_flag_k, nT + 1, m, nT, c.face_scan);
     PAMO_TRY(scan_exclusive(c.face_scan, c.face_scan, nT + 1, c.ws, c.stream));
     LAUNCHP(K_COMPACT_MESH, 4.0 * nP + 4.0 * nP + 16.0 * nP + 16.0 * nP + 12.0 * nT + 4.0 * nT + 12.0 * nT,
             compact_mesh_k, std::max(nP, nT), m, nP, nT, c.occ, c.pts_map, c.face_scan, c.np, c.ni, c.nj, c.nk,
             c.dcnt);
+    carry = carry && c.tree_valid && c.tree_keep_pct < 100;
+    if (carry) {
+        hipLaunchKernelGGL(fill_k<int>, dim3(1), dim3(64), 0, c.stream, c.dcnt + C_LIVE_LEAVES, 0, 1LL);
+        // over every leaf: a carried tree can have more leaves than faces
+        LAUNCH(carry_leaves_k, std::max(nT, c.tree_n), c.bval, (const int*)c.ti, (const int*)c.face_scan, nT,
+               c.dcnt);
+    }
     PAMO_TRY(sync_counters(c));
+    c.tree_n = c.hcnt[C_NLIVE];
+    if (carry) {
+        const long long leaves = c.hcnt[C_NLIVE], live = c.hcnt[C_LIVE_LEAVES];
+        c.tree_fresh = live > 1 && live * 100 >= leaves * (long long)c.tree_keep_pct;
+        c.tree_valid = c.tree_fresh;
+    } else {
+        c.tree_valid = c.tree_fresh = false;
+    }
     std::swap(c.p, c.np);
     std::swap(c.ti, c.ni);
     std::swap(c.tj, c.nj);
@@ -713,6 +756,16 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
         c.guard_small_stack = (int)value;
         return 0;
     }
+    if (option == PAMO_OPT_TREE_KEEP) {
+        PAMO_CHECK(value >= 1 && value <= 100, "pamo_ctx_set_option: tree keep percentage must be in [1, 100]");
+        c.tree_keep_pct = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_PLACEMENT) {
+        PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: placement must be 0 (midpoint) or 1 (optimal)");
+        c.placement = (int)value;
+        return 0;
+    }
     if (option == PAMO_OPT_WALK_LIMIT) {
         PAMO_CHECK(value >= 0, "pamo_ctx_set_option: walk limit must be >= 0");
         c.walk_cap = value;
@@ -730,6 +783,8 @@ int pamo_ctx_reset(pamo_ctx* ctx) {
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     c.cap_pts = c.cap_tris = c.cap_edges = 0;
     c.n_pts = c.n_tris = c.n_edges = c.n_rec = c.n_regular = c.n_vertices_undo = 0;
+    c.tree_valid = c.tree_fresh = false;
+    c.tree_n = 0;
     c.undo_iters = 0;
     return 0;
 }
@@ -758,6 +813,8 @@ int pamo_round(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tris, 
     PAMO_TRY(pamo::grow_pts(c, (size_t)n_pts));
     PAMO_TRY(pamo::grow_tris(c, (size_t)n_tris));
     PAMO_TRY(pamo::load_mesh(c, d_pts, n_pts, d_tris, n_tris));
+    c.tree_valid = c.tree_fresh = false;  // the caller compacts: no tree carries across calls
+    c.tree_n = 0;
     if (init) PAMO_TRY(pamo::shuffle_faces(c, n_tris));
     PAMO_TRY(pamo::round_device(c, n_pts, n_tris, scale, threshold, is_stuck != 0));
     PAMO_CHECK(c.n_vertices_undo <= undo_cap, "pamo_round: undo_cap too small");
@@ -945,6 +1002,8 @@ static int simplify_loop(Ctx& c, const float* d_verts, int n_verts, const int* d
     PAMO_TRY(pamo::grow_pts(c, (size_t)n_verts));
     PAMO_TRY(pamo::grow_tris(c, (size_t)n_faces));
     PAMO_TRY(pamo::load_mesh(c, d_verts, n_verts, d_faces, n_faces));
+    c.tree_valid = c.tree_fresh = false;
+    c.tree_n = 0;
     float scale;
     PAMO_TRY(input_scale(c, n_verts, scale));
     int nV = n_verts, nF = n_faces;
@@ -964,7 +1023,7 @@ static int simplify_loop(Ctx& c, const float* d_verts, int n_verts, const int* d
         if (n_undo > 0)
             PAMO_HIP(hipMemcpyAsync(c.undo_prev, c.undo_list, sizeof(int) * n_undo, hipMemcpyDeviceToDevice,
                                     c.stream));
-        PAMO_TRY(pamo::compact_round(c, nV, nF));
+        PAMO_TRY(pamo::compact_round(c, nV, nF, true));
         nV = c.hcnt[pamo::C_NV];
         nF = c.hcnt[pamo::C_NF];
         if (face_hist && rounds <= hist_cap) face_hist[rounds - 1] = nF;
@@ -1021,6 +1080,8 @@ int pamo_pasp_round(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_t
     PAMO_TRY(pamo::grow_pts(c, (size_t)n_pts));
     PAMO_TRY(pamo::grow_tris(c, (size_t)n_tris));
     PAMO_TRY(pamo::load_mesh(c, d_pts, n_pts, d_tris, n_tris));
+    c.tree_valid = c.tree_fresh = false;  // the caller compacts: no tree carries across calls
+    c.tree_n = 0;
     if (init) PAMO_TRY(pamo::shuffle_faces(c, n_tris));
     PAMO_TRY(pamo::pasp_round_device(c, n_pts, n_tris, scale, threshold));
     pamo::Mesh m = pamo::cur_mesh(c);

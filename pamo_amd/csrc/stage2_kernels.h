@@ -69,6 +69,7 @@ enum Counter {
     C_UNDECIDED = 8,  // exact guard: pairs the filtered predicates left undecided
     C_WALK_ERR = 9,   // a guard tree walk exceeded its bound (walk_overrun)
     C_BAD_UNDO = 10,  // an incoming undo entry outside the context's points
+    C_LIVE_LEAVES = 11,  // guard tree leaves still live after a compaction
     C_COUNT = 16
 };
 
@@ -311,6 +312,49 @@ struct CostParams {
     int is_stuck;
 };
 
+// Opt-in optimal placement (the reference always collapses to the midpoint,
+// cusimp_free.cu:491,687): the minimiser of x^T M x over x = (p, 1) for
+// M = Q_u + Q_v, from the 3x3 system A p = -b solved in FP64 by cofactors in
+// registers.  The midpoint is kept when A is singular
+// (|det| <= 1e-9 max|a_ij|^3, NaN included) or when the minimiser lies
+// farther from the midpoint than the edge is long.  M holds the 10 float
+// sums 00 01 02 03 11 12 13 22 23 33.
+__device__ __forceinline__ F3 place_optimal(const float M[10], F3 p0, F3 p1) {
+    const F3 mid = f3divs(f3add(p0, p1), 2.0f);
+    const double a00 = M[0], a01 = M[1], a02 = M[2], a11 = M[4], a12 = M[5], a22 = M[7];
+    const double b0 = -(double)M[3], b1 = -(double)M[6], b2 = -(double)M[8];
+    const double c00 = a11 * a22 - a12 * a12;
+    const double c01 = a02 * a12 - a01 * a22;
+    const double c02 = a01 * a12 - a02 * a11;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    double amax = fabs(a00);
+    amax = fmax(amax, fabs(a01));
+    amax = fmax(amax, fabs(a02));
+    amax = fmax(amax, fabs(a11));
+    amax = fmax(amax, fabs(a12));
+    amax = fmax(amax, fabs(a22));
+    if (!(fabs(det) > 1e-9 * amax * amax * amax)) return mid;
+    const double c11 = a00 * a22 - a02 * a02;
+    const double c12 = a01 * a02 - a00 * a12;
+    const double c22 = a00 * a11 - a01 * a01;
+    const double x = (c00 * b0 + c01 * b1 + c02 * b2) / det;
+    const double y = (c01 * b0 + c11 * b1 + c12 * b2) / det;
+    const double z = (c02 * b0 + c12 * b1 + c22 * b2) / det;
+    const double ex = (double)p1.x - (double)p0.x, ey = (double)p1.y - (double)p0.y, ez = (double)p1.z - (double)p0.z;
+    const double dx = x - (double)mid.x, dy = y - (double)mid.y, dz = z - (double)mid.z;
+    if (!(dx * dx + dy * dy + dz * dz <= ex * ex + ey * ey + ez * ez)) return mid;
+    return F3{(float)x, (float)y, (float)z};
+}
+
+// M = Q_u + Q_v from the per-vertex records (3 float4: 00 01 02 03 | 11 12 13 22 | 23 33 - -).
+__device__ __forceinline__ void edge_quadric(const float4* __restrict__ Q, int u, int v, float M[10]) {
+    const float4 ua = Q[3 * (long long)u], ub = Q[3 * (long long)u + 1], uc = Q[3 * (long long)u + 2];
+    const float4 va = Q[3 * (long long)v], vb = Q[3 * (long long)v + 1], vc = Q[3 * (long long)v + 2];
+    M[0] = ua.x + va.x; M[1] = ua.y + va.y; M[2] = ua.z + va.z; M[3] = ua.w + va.w;
+    M[4] = ub.x + vb.x; M[5] = ub.y + vb.y; M[6] = ub.z + vb.z; M[7] = ub.w + vb.w;
+    M[8] = uc.x + vc.x; M[9] = uc.y + vc.y;
+}
+
 // Launch order of the edge-cost lanes: edge ids grouped by their smaller
 // endpoint (counting sort: histogram, exclusive scan, scatter).  Vertex ids of
 // a mesh are spatially coherent while edge ids follow the shuffled face
@@ -331,6 +375,7 @@ __global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __
 // compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge; lane
 // t takes edge order[t] and writes cost[order[t]].
 
+template <bool kOptimal>
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
                                                       const int4* __restrict__ rv, const int* __restrict__ eu,
                                                       const int* __restrict__ ev, const int* __restrict__ order,
@@ -354,7 +399,9 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     }
     if (dup != 2) return;
     const F3 p0 = ld(m, u), p1 = ld(m, v);
-    const F3 vm = f3divs(f3add(p0, p1), 2.0f);
+    float M[10];
+    edge_quadric(Q, u, v, M);
+    const F3 vm = kOptimal ? place_optimal(M, p0, p1) : f3divs(f3add(p0, p1), 2.0f);
     const float k4s3 = 4.0f * sqrtf(3.0f);
     // one pass over both rings: the edge-length sum and the skinny / flip
     // terms read the same three vertices; each sum keeps its own order
@@ -389,11 +436,8 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
     qa = 5.0f * qa;
     // M = Q_u + Q_v; storage 0:00 1:01 2:02 3:03 4:11 5:12 6:13 7:22 8:23 9:33
-    const float4 ua = Q[3 * (long long)u], ub4 = Q[3 * (long long)u + 1], uc = Q[3 * (long long)u + 2];
-    const float4 va = Q[3 * (long long)v], vb4 = Q[3 * (long long)v + 1], vc = Q[3 * (long long)v + 2];
-    const float M0 = ua.x + va.x, M1 = ua.y + va.y, M2 = ua.z + va.z, M3 = ua.w + va.w;
-    const float M4 = ub4.x + vb4.x, M5 = ub4.y + vb4.y, M6 = ub4.z + vb4.z, M7 = ub4.w + vb4.w;
-    const float M8 = uc.x + vc.x, M9 = uc.y + vc.y;
+    const float M0 = M[0], M1 = M[1], M2 = M[2], M3 = M[3], M4 = M[4], M5 = M[5], M6 = M[6], M7 = M[7];
+    const float M8 = M[8], M9 = M[9];
     const float x0 = vm.x, x1 = vm.y, x2 = vm.z, x3 = 1.0f;
     float r0 = M0 * x0 + M1 * x1 + M2 * x2 + M3 * x3;
     float r1 = M1 * x0 + M4 * x1 + M5 * x2 + M6 * x3;
@@ -478,12 +522,14 @@ __device__ __forceinline__ void restore_edge(Mesh m, Orig o, int* occ, const int
 // and its result does not depend on execution order.
 // Selected <=> cost <= collapse_t and every face t of ring(u) u ring(v) has
 // tri_min[t] == key(e), i.e. min(vmin of t's vertices) == key(e).
+template <bool kOptimal>
 __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int* occ, const int* __restrict__ off,
                                                             const int* __restrict__ ring, const int4* __restrict__ rv,
                                                             const int* __restrict__ eu, const int* __restrict__ ev,
                                                             int nE, const uint32_t* __restrict__ cost,
                                                             const unsigned long long* __restrict__ vmin,
-                                                            uint32_t collapse_t, int* __restrict__ sel) {
+                                                            uint32_t collapse_t, int* __restrict__ sel,
+                                                            const float4* __restrict__ Q) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > nE) return;
     if (e == nE) {
@@ -509,7 +555,14 @@ __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int*
         }
         if (ok) {
             s = 1;
-            F3 vm = f3divs(f3add(ld(m, u), ld(m, v)), 2.0f);
+            F3 vm;
+            if (kOptimal) {
+                float M[10];
+                edge_quadric(Q, u, v, M);
+                vm = place_optimal(M, ld(m, u), ld(m, v));
+            } else {
+                vm = f3divs(f3add(ld(m, u), ld(m, v)), 2.0f);
+            }
             st(m, u, vm);
             st(m, v, F3{0.0f, 0.0f, 0.0f});
             occ[v] = 0;
@@ -759,6 +812,11 @@ __global__ void __launch_bounds__(kBlock) face_box_k(Mesh m, int nT, float4* __r
     bool leaf = false;
     if (threadIdx.x == 0) nleaf = 0;
     __syncthreads();
+    if (f == nT) {
+        // the box of a carried tree's dead leaves (face id nT): empty
+        fbox[2 * (long long)f] = make_float4(inf, inf, inf, 0.0f);
+        fbox[2 * (long long)f + 1] = make_float4(-inf, -inf, -inf, 0.0f);
+    }
     if (f < nT) {
         flagged[f] = 0;
         int a = m.i[f];
@@ -1151,13 +1209,39 @@ __device__ __forceinline__ void load_tri9(Mesh m, int f, float T[3][3], int idx[
 // Changed live faces in Morton order (neighbouring lanes query neighbouring
 // regions of the tree).
 // Queries: the changed live faces among the tree's leaves, in leaf order.
+// nL >= the tree's leaf count (a carried tree can have more leaves than
+// there are faces); flag[0..nL] is written.
 __global__ void query_flag_k(const uint32_t* __restrict__ val, const int* __restrict__ cnt,
-                             const uint8_t* __restrict__ changed, const int* __restrict__ ti, int nT,
+                             const uint8_t* __restrict__ changed, const int* __restrict__ ti, int nT, int nL,
                              int* __restrict__ flag) {
     const int n = cnt[C_NLIVE];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > nT) return;
-    flag[s] = (s < n && changed[val[s]] && ti[val[s]] != -1) ? 1 : 0;
+    if (s > nL) return;
+    const int f = s < n ? (int)val[s] : nT;  // a carried tree's dead leaves hold nT
+    flag[s] = (f < nT && changed[f] && ti[f] != -1) ? 1 : 0;
+}
+
+// Carries the guard tree across a compaction: leaf s keeps its place in the
+// hierarchy and takes its face's new index, or nT_new (the empty sentinel
+// box) if the face was removed; counts the leaves still live.  Every face
+// live at the next round's start is live now, hence already a leaf.
+__global__ void carry_leaves_k(uint32_t* __restrict__ val, const int* __restrict__ ti, const int* __restrict__ fpos,
+                               int nT_old, int* __restrict__ cnt) {
+    __shared__ int nlive;
+    const int n = cnt[C_NLIVE], n_new = cnt[C_NF];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) nlive = 0;
+    __syncthreads();
+    bool live = false;
+    if (s < n) {
+        const int f = (int)val[s];  // nT_old: already dead
+        live = f >= 0 && f < nT_old && ti[f] != -1 && fpos[f] < n_new;
+        val[s] = live ? (uint32_t)fpos[f] : (uint32_t)n_new;
+    }
+    const uint64_t lm = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&nlive, (int)__popcll(lm));
+    __syncthreads();
+    if (threadIdx.x == 0 && nlive) atomicAdd(&cnt[C_LIVE_LEAVES], nlive);
 }
 
 __global__ void query_list_k(const uint32_t* __restrict__ val, const int* __restrict__ flag,
@@ -1510,15 +1594,17 @@ __global__ void compact_mesh_k(Mesh m, int nP, int nT, const int* __restrict__ o
 
 // Centring offset of PaMO.run (pamo/__init__.py:60-72,144): numpy's
 // tris.mean(axis=1).mean(axis=0) on the float32 face soup.  The per-face mean
-// is ((a + b) + c) divided by 3 in double and rounded to float (numpy divides
-// the float32 sum by an integer count through a float64 loop); the mean over
-// faces is a sequential float32 running sum from the first face's value, in
-// face order, divided the same way.  A float32 running sum cannot be split,
-// so one workgroup runs it: waves 1..15 write the face means of the next
-// chunk to LDS while lanes 0..2 of wave 0 add up the current chunk, one
-// coordinate per lane.
+// is ((a + b) + c) / 3 (numpy divides the float32 sum by the count through a
+// float64 loop; a float64 quotient rounded to float32 is the correctly
+// rounded float32 quotient, since 53 >= 2 * 24 + 2); the mean over faces is
+// a sequential float32 running sum from the first face's value, in face
+// order, divided the same way.  A float32 running sum cannot be split, so
+// one workgroup runs it: waves 1..15 write the face means of the next chunk
+// to LDS while lanes 0..2 of wave 0 add up the current chunk, one coordinate
+// per lane, reading 32 values ahead of the additions so that the chain of
+// dependent adds is not held up by LDS latency.
 constexpr int kCentThreads = 1024;
-constexpr int kCentChunk = 2048;
+constexpr int kCentChunk = 2048;            // multiple of 32
 constexpr int kCentPitch = kCentChunk + 4;  // rows start in different LDS banks
 
 __device__ inline void centroid_fill(const float* __restrict__ pts, const int* __restrict__ tris, long long base,
@@ -1529,15 +1615,43 @@ __device__ inline void centroid_fill(const float* __restrict__ pts, const int* _
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
-            dst[d * kCentPitch + j] = (float)((double)s / 3.0);
+            dst[d * kCentPitch + j] = s / 3.0f;
         }
     }
+}
+
+// acc + row[0] + ... + row[n - 1], left to right; row is 16-byte aligned
+__device__ __forceinline__ float centroid_chain(float acc, const float* __restrict__ row, int n) {
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+    const int full = n >> 5;  // batches of 32
+    float4 cur[8], nxt[8];
+    if (full > 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cur[j] = r4[j];
+    }
+    for (int b = 0; b < full; ++b) {
+        if (b + 1 < full) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) nxt[j] = r4[8 * (b + 1) + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            acc = acc + cur[j].x;
+            acc = acc + cur[j].y;
+            acc = acc + cur[j].z;
+            acc = acc + cur[j].w;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
+    }
+    for (int i = full << 5; i < n; ++i) acc = acc + row[i];
+    return acc;
 }
 
 __global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __restrict__ pts,
                                                                const int* __restrict__ tris, int nF,
                                                                float* __restrict__ out3) {
-    __shared__ float buf[2][3 * kCentPitch];
+    __shared__ __align__(16) float buf[2][3 * kCentPitch];
     const int t = threadIdx.x;
     const int nchunks = (nF + kCentChunk - 1) / kCentChunk;
     centroid_fill(pts, tris, 0, nF < kCentChunk ? nF : kCentChunk, buf[0], t, kCentThreads);
@@ -1554,12 +1668,17 @@ __global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __re
             }
         } else if (t < 3) {
             const float* row = buf[k & 1] + t * kCentPitch;
-            int i = 0;
             if (k == 0) {
-                acc = row[0];  // numpy's reduction starts from the first element
-                i = 1;
+                // numpy's reduction starts from the first element; the rest
+                // of the first chunk from index 1 (unaligned) one at a time
+                // up to 32, then in batches
+                acc = row[0];
+                const int head = cnt < 32 ? cnt : 32;
+                for (int i = 1; i < head; ++i) acc = acc + row[i];
+                acc = centroid_chain(acc, row + head, cnt - head);
+            } else {
+                acc = centroid_chain(acc, row, cnt);
             }
-            for (; i < cnt; ++i) acc = acc + row[i];
         }
         __syncthreads();
     }

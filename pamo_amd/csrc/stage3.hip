@@ -53,6 +53,7 @@ struct Ctx {
     int nV = 0, nF = 0, nE = 0, nH = 0, nGV = 0, nGF = 0;
     int capV = 0, capF = 0, capE = 0, capH = 0, capG = 0;
     bool do_nothing = false;
+    double min_angle_thres = 0.0;  // degrees (Stage3Config.min_angle_thres)
     float radius = 0.0f;
     // vertices (float4)
     float4 *q = nullptr, *qn = nullptr, *qd = nullptr, *qrest = nullptr, *p = nullptr, *target = nullptr;
@@ -1686,6 +1687,7 @@ int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
     P.la = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
     P.k_hinge = (float)prm->hinge_stiffness;
     P.k_m2gt = (float)prm->mesh2gt_dist_stiffness;
+    c->min_angle_thres = prm->min_angle_thres;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     // ccount[0]: contacts found; ccount[1]: a tree walk exceeded its bound
@@ -2109,7 +2111,9 @@ int pamo_s3_set_mesh_device(pamo_s3* s, const float* d_v, int n_v, const int* d_
     const int use_hinge = (c.P.mask & PAMO_S3_E_HINGE) != 0;
     const int n_e = (int)(hp.edges.size() / 2), n_h = use_hinge ? (int)(hp.hinges.size() / 4) : 0;
     PAMO_CHECK(!use_hinge || n_h == n_e, "Number of hinges != number of edges (the mesh is not closed and manifold)");
-    const int do_nothing = hp.min_angle_deg <= 0.0 ? 1 : 0;  // min_angle_thres 0 (config.py:80)
+    // system.py:217-222: min angle in degrees against min_angle_thres (a NaN
+    // angle compares false, as numpy's does)
+    const int do_nothing = hp.min_angle_deg <= c.min_angle_thres ? 1 : 0;
     return pamo_s3_set_mesh(s, d_v, n_v, d_f, n_f, scale, t3, hp.edges.data(), n_e,
                             n_h ? hp.hinges.data() : nullptr, n_h, hp.vor.data(), do_nothing);
 }

@@ -159,10 +159,15 @@ class PaMO(torch.nn.Module):
     ``exact_guard=True`` (not in the reference's signature) replaces the
     reference's guard predicate by the exact intersection test, so that no
     collapse that makes two faces intersect survives; the default keeps the
-    reference's predicate and results.
+    reference's predicate and results.  ``placement="optimal"`` (not in the
+    reference, which always collapses an edge to its midpoint) moves a
+    collapsed edge to the minimiser of its summed quadric, solved in FP64,
+    with the midpoint kept where that system is singular; the default
+    "midpoint" keeps the reference's results.
     """
 
-    def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True, exact_guard: bool = False):
+    def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True, exact_guard: bool = False,
+                 placement: str = "midpoint"):
         super().__init__()
         if use_stage1:
             raise NotImplementedError(
@@ -181,6 +186,11 @@ class PaMO(torch.nn.Module):
         self.ctx = _hip.Context(torch.cuda.current_device())
         if exact_guard:
             _hip.check(_hip.lib().pamo_ctx_set_option(self.ctx.ptr, _hip.PAMO_OPT_EXACT_GUARD, 1), "set_option")
+        if placement not in ("midpoint", "optimal"):
+            raise ValueError(f"placement must be 'midpoint' or 'optimal', not {placement!r}")
+        self.placement = placement
+        if placement == "optimal":
+            _hip.check(_hip.lib().pamo_ctx_set_option(self.ctx.ptr, _hip.PAMO_OPT_PLACEMENT, 1), "set_option")
         if use_stage3:
             from . import stage3
             self.config = stage3.Stage3Config()

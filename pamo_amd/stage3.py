@@ -80,7 +80,8 @@ class Stage3Config:
             contact_detection_radius=self.contact_detection_radius, ee_classify_thres=self.ee_classify_thres,
             ccd_slackness=self.ccd_slackness, ccd_thickness=self.ccd_thickness,
             elas_young_modulus=self.elas_young_modulus, elas_poisson_ratio=self.elas_poisson_ratio,
-            hinge_stiffness=self.hinge_stiffness, mesh2gt_dist_stiffness=self.mesh2gt_dist_stiffness)
+            hinge_stiffness=self.hinge_stiffness, mesh2gt_dist_stiffness=self.mesh2gt_dist_stiffness,
+            min_angle_thres=float(self.min_angle_thres))
 
 
 # ---------------------------------------------------------------- host preprocessing

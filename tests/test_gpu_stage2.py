@@ -325,3 +325,56 @@ def test_scripted_undo_and_line_collapse_rounds_match_oracle(torch_cuda, oracle_
 def _hip_mod():
     from pamo_amd import _hip
     return _hip
+
+
+@pytest.mark.parametrize("level", [4, 6])
+def test_optimal_placement_matches_oracle(torch_cuda, oracle_lib, level):
+    """Opt-in optimal placement (FP64 3x3 solve per edge in the cost and the
+    collapse kernels) against the oracle's, bit for bit, through the fused
+    loop; the default context on the same input keeps the midpoint."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(level, 5)
+    Vc = centred(V, F)
+    print(f"case: optimal placement, icosphere L{level}, {len(F)} faces", flush=True)
+    ctx = _hip.Context(0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_PLACEMENT, 1), "set_option")
+    vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.05, min_verts=0)
+    vo, fo, ho = oracle_lib.run(Vc, F, 0.05, min_verts=0, placement=1)
+    assert np.array_equal(hg, ho)
+    assert np.array_equal(fg.cpu().numpy(), fo)
+    assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32))
+    assert synth.is_watertight_manifold(fo)
+    vm, fm, hm = oracle_lib.run(Vc, F, 0.05, min_verts=0)
+    assert not (len(fm) == len(fo) and np.array_equal(vm, vo))  # the option changes the result
+
+
+@pytest.mark.parametrize("case", ["icosphere_l6", "sheets_l5"])
+def test_tree_carry_matches_rebuild_every_round(torch_cuda, oracle_lib, case):
+    """The fused loop carries the guard tree across rounds (leaves renumbered
+    at each compaction, removed faces with empty boxes) until fewer than half
+    its leaves are live.  Which pairs overlap does not depend on the tree, so
+    carrying, rebuilding every round and the oracle agree bit for bit."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    if case == "icosphere_l6":
+        V, F = synth.icosphere_sh(6, 2)
+    else:
+        V0, F0 = synth.icosphere_sh(5, 4)
+        V, F = synth.offset_sheets(V0, F0, rel_offset=3e-3)
+    Vc = centred(V, F)
+    print(f"case: tree carry vs rebuild, {case}, {len(F)} faces", flush=True)
+    outs = []
+    for keep in (50, 90, 100):
+        ctx = _hip.Context(0)
+        _hip.check(_hip.lib().pamo_ctx_set_option(ctx.ptr, _hip.PAMO_OPT_TREE_KEEP, keep), "set_option")
+        vg, fg, hg = simplify_device(ctx, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.02,
+                                     min_verts=0)
+        outs.append((vg.cpu().numpy(), fg.cpu().numpy(), hg))
+    vo, fo, ho = oracle_lib.run(Vc, F, 0.02, min_verts=0)
+    for vg, fg, hg in outs:
+        assert np.array_equal(hg, ho)
+        assert np.array_equal(fg, fo)
+        assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32))

@@ -306,3 +306,72 @@ def test_undo_list_uses_the_previous_calls_map_and_stuck_prologue(oracle_lib):
     _, out3 = _round(oracle_lib, V2, F2, 1e-3, ctx=r, undo=undo2, is_stuck=1,
                      scale=float((V.max(0) - V.min(0)).max()))
     assert np.array_equal(out3[4][: len(undo2)], map2[undo2])
+
+
+# ----------------------------------------------------------------- opt-in optimal placement
+_IU = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+
+
+def _quadric(planes, weights=None):
+    """The 10 stored entries of sum w p p^T over planes p = (n, d), n.x + d = 0."""
+    M = np.zeros((4, 4))
+    for k, (n, d) in enumerate(planes):
+        n = np.asarray(n, float) / np.linalg.norm(n)
+        p = np.r_[n, d]
+        M += (1.0 if weights is None else weights[k]) * np.outer(p, p)
+    return np.array([M[i, j] for i, j in _IU], np.float32)
+
+
+def test_optimal_placement_closed_forms(oracle_lib):
+    """Three planes meeting in a point give that point; two planes (a line)
+    or none leave the system singular and keep the midpoint; a minimiser
+    farther from the midpoint than the edge is long keeps the midpoint."""
+    place = oracle_lib.place_optimal
+    p0, p1 = np.array([0.7, 1.6, 2.9], np.float32), np.array([1.4, 2.3, 3.2], np.float32)
+    mid = (p0 + p1) / np.float32(2)
+    # x = 1, y = 2, z = 3
+    M = _quadric([((1, 0, 0), -1.0), ((0, 1, 0), -2.0), ((0, 0, 1), -3.0)])
+    assert np.allclose(place(M, p0, p1), [1.0, 2.0, 3.0], atol=1e-6)
+    # three oblique planes through (1.2, 1.9, 3.0), with weights
+    q = np.array([1.2, 1.9, 3.0])
+    ns = [np.array([1.0, 0.3, -0.2]), np.array([-0.4, 1.0, 0.5]), np.array([0.2, -0.6, 1.0]), np.array([1, 1, 1.0])]
+    planes = [(n, -float(np.dot(n / np.linalg.norm(n), q))) for n in ns]
+    got = place(_quadric(planes, [1.0, 2.0, 0.5, 3.0]), p0, p1)
+    assert np.allclose(got, q, atol=2e-5)
+    # a line of minimisers (two planes): midpoint, bit for bit
+    M2 = _quadric([((1, 0, 0), -1.0), ((0, 1, 0), -2.0)])
+    assert np.array_equal(place(M2, p0, p1).view(np.uint32), mid.view(np.uint32))
+    assert np.array_equal(place(np.zeros(10, np.float32), p0, p1).view(np.uint32), mid.view(np.uint32))
+    # a point 10 edge lengths away: midpoint
+    far = _quadric([((1, 0, 0), -10.0), ((0, 1, 0), -2.0), ((0, 0, 1), -3.0)])
+    assert np.array_equal(place(far, p0, p1).view(np.uint32), mid.view(np.uint32))
+
+
+def test_optimal_placement_round_moves_to_the_minimiser(oracle_lib):
+    """A round with optimal placement on the stretched octahedron: every
+    collapsed vertex sits where place_optimal puts it for its edge's
+    quadric (from the round's own quadrics), and the default round keeps
+    the midpoint."""
+    from pamo_amd import synth
+    V, F = synth.icosphere(2)
+    V = (V * np.array([1.0, 1.7, 0.6])).astype(np.float32)
+    F = F.astype(np.int32)
+    s = float((V.max(0) - V.min(0)).max())
+    for mode in (0, 1):
+        r = oracle_lib.Round()
+        r.set_placement(mode)
+        pv, pt, occ, mp, undo = r.forward(V, F, np.zeros(0, np.int32), 0, s, 0.05, 0, 0)
+        rec, nreg = r.collapsed()
+        E = r.edges()
+        Q = r.quadrics(len(V))
+        moved = 0
+        for e in rec[:nreg].tolist():
+            u, v = E[e]
+            if occ.reshape(-1)[v] != 0:
+                continue  # undone
+            want = oracle_lib.place_optimal(Q[u] + Q[v], V[u], V[v]) if mode else (V[u] + V[v]) / np.float32(2)
+            assert np.array_equal(pv[u].view(np.uint32), np.asarray(want, np.float32).view(np.uint32))
+            moved += int(mode and not np.array_equal(want, (V[u] + V[v]) / np.float32(2)))
+        assert nreg > 0
+        if mode:
+            assert moved > 0
