@@ -349,6 +349,21 @@ int pamo_point_mesh_sqdist(pamo_metrics* m, const double* points, long long n_po
 int pamo_hausdorff(pamo_metrics* m, const double* va, int n_va, const int* fa, int n_fa, const double* vb, int n_vb,
                    const int* fb, int n_fb, double* d_out);
 
+/* No counterpart in the reference.  pamo_point_mesh_sqdist for callers that
+ * cannot allocate device memory (code between captures, serving loops, hosts
+ * that forbid allocation): the same results, with the tree and workspace in
+ * a static device array of 16 MiB per GPU instead of a context's buffers, so
+ * the call makes no hipMalloc or hipFree.  Runs on the current device;
+ * stream is a hipStream_t of that device or NULL.  At most 65536 faces
+ * (n_f above that fails with an error status).  Calls on one device share its
+ * array: they are serialised by a process-wide lock and each returns after
+ * its kernels finish, so calls never overlap on a GPU; work the caller puts
+ * on the same array by other means would race.  The first call on a device
+ * looks up that device's array address. */
+int pamo_point_mesh_sqdist_noalloc(const double* points, long long n_points, const double* verts, int n_v,
+                                   const int* faces, int n_f, double* sqr_d, int* face_index, double* closest,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

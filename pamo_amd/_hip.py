@@ -114,6 +114,7 @@ SIGNATURES = [
     ("pamo_point_mesh_sqdist", c_int, [c_vp, c_vp, c_ll, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     ("pamo_hausdorff", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_int,
                                ctypes.POINTER(ctypes.c_double)]),
+    ("pamo_point_mesh_sqdist_noalloc", c_int, [c_vp, c_ll, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),
 ]
 
 # pamo_batch_run's per-mesh completion callback: (mesh, status, user)

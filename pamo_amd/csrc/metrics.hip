@@ -18,7 +18,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "pamo_hip.h"
 #include "lbvh_tree.h"
@@ -280,6 +282,74 @@ static int one_side(Ctx& c, const double* VA, int nVA, const double* VB, int nVB
     return 0;
 }
 
+// ------------------------------------------------------------ allocation-free variant
+// pamo_point_mesh_sqdist_noalloc keeps everything a call needs beyond its
+// inputs and outputs (the tree over at most kNoAllocMaxFaces faces, the sort
+// and scan workspace and the two flags) in one static device array, so that
+// it calls no allocator.  Host code uses the array only through the address
+// hipGetSymbolAddress returns for the current device: in host code the
+// symbol's name is a host-side stand-in, not a device address.
+constexpr int kNoAllocMaxFaces = 1 << 16;
+constexpr size_t kNoAllocBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocDevice {
+    std::mutex mu;          // serialises calls on one device
+    unsigned char* base = nullptr;  // device address of this device's scratch
+    Work w;                 // views into it
+};
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// Carves the work buffers for up to max_faces faces out of [base, base +
+// bytes) (base may be null to only measure).  Returns the bytes needed.
+static size_t noalloc_layout(unsigned char* base, int max_faces, Work& w) {
+    size_t off = 0;
+    auto take = [&](auto*& ptr, size_t count) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        off = (off + 255) & ~(size_t)255;
+        ptr = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * std::max<size_t>(count, 1);
+    };
+    // the sizes of s3::tree_alloc for a capacity of max_faces + 16 items
+    const size_t a = (size_t)max_faces + 16;
+    Tree& t = w.tree;
+    take(t.item, 2 * a);
+    take(t.key, a); take(t.val, a); take(t.key2, a); take(t.val2, a);
+    take(t.left, a); take(t.right, a); take(t.parent, 2 * a);
+    take(t.range, a);
+    take(t.box, 4 * a);
+    take(t.pre, 2 * a); take(t.suf, 2 * a);
+    const size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
+    take(t.blk, 2 * nb); take(t.blkpre, 2 * nb); take(t.blksuf, 2 * nb);
+    take(t.sblk, 2 * ns);
+    take(t.part, 6 * (a / kBlock + 4));
+    take(t.cnt, (size_t)C_COUNT);
+    t.cap = (int)a;
+    const long long ws = std::max(radix_workspace(max_faces + 64LL), scan_workspace(max_faces + 65LL)) + 256;
+    take(w.ws, (size_t)ws);
+    take(w.flags, 2);
+    return off;
+}
+
+// Resolves this device's scratch address once (with the device current) and
+// lays the work buffers out in it.
+static int noalloc_setup(int device, NoAllocDevice*& out) {
+    PAMO_CHECK(device >= 0 && device < kNoAllocMaxDevices, "pamo_point_mesh_sqdist_noalloc: device id out of range");
+    NoAllocDevice& d = g_noalloc[device];
+    out = &d;
+    if (d.base) return 0;
+    void* ptr = nullptr;
+    PAMO_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PAMO_CHECK(ptr != nullptr, "pamo_point_mesh_sqdist_noalloc: no device address for the scratch array");
+    Work probe;
+    PAMO_CHECK(noalloc_layout(nullptr, kNoAllocMaxFaces, probe) <= kNoAllocBytes,
+               "pamo_point_mesh_sqdist_noalloc: the scratch array is smaller than its layout");
+    noalloc_layout(static_cast<unsigned char*>(ptr), kNoAllocMaxFaces, d.w);
+    d.base = static_cast<unsigned char*>(ptr);
+    return 0;
+}
+
 }  // namespace metrics
 }  // namespace pamo
 
@@ -351,6 +421,38 @@ int pamo_hausdorff(pamo_metrics* m, const double* VA, int n_va, const int* FA, i
     std::memcpy(&d2, c.h_mx, sizeof(double));
     *d_out = std::sqrt(d2);
     return 0;
+}
+
+int pamo_point_mesh_sqdist_noalloc(const double* P, long long n_points, const double* V, int n_v, const int* F,
+                                   int n_f, double* sqr_d, int* face_index, double* closest, void* stream_handle) {
+    using namespace pamo::metrics;
+    hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+    pamo::clear_error();
+    PAMO_CHECK(n_points >= 0 && (n_points == 0 || (P && sqr_d)), "pamo_point_mesh_sqdist_noalloc: null argument");
+    PAMO_CHECK(V && F && n_v > 0 && n_f > 0, "pamo_point_mesh_sqdist_noalloc: empty or null mesh");
+    PAMO_CHECK(n_f <= kNoAllocMaxFaces, "pamo_point_mesh_sqdist_noalloc: " + std::to_string(n_f) +
+                                           " faces, more than the " + std::to_string(kNoAllocMaxFaces) +
+                                           " its static scratch holds (use pamo_point_mesh_sqdist)");
+    int device = 0;
+    PAMO_HIP(hipGetDevice(&device));
+    if (stream) {
+        hipDevice_t sdev = 0;
+        PAMO_HIP(hipStreamGetDevice(stream, &sdev));
+        PAMO_CHECK((int)sdev == device, "pamo_point_mesh_sqdist_noalloc: the stream belongs to another device");
+    }
+    // the setup and the call hold the device's lock: calls on one device
+    // share its scratch and run one at a time, each finishing before the next
+    NoAllocDevice* d = nullptr;
+    std::unique_lock<std::mutex> lock(g_noalloc[std::max(0, std::min(device, kNoAllocMaxDevices - 1))].mu);
+    PAMO_TRY(noalloc_setup(device, d));
+    // the flags come back to this call's own host words
+    int h_flags[2] = {0, 0};
+    Work w = d->w;
+    w.h_flags = h_flags;
+    PAMO_TRY(begin(w, stream));
+    PAMO_TRY(build_tree(w, stream, V, n_v, F, n_f, "pamo_point_mesh_sqdist_noalloc"));
+    PAMO_TRY(query(w, stream, P, n_points, V, F, sqr_d, face_index, closest));
+    return finish(w, stream, "pamo_point_mesh_sqdist_noalloc");
 }
 
 }  // extern "C"
