@@ -9,7 +9,19 @@ min_verts 0, as PaMO(mesh, use_stage1=False, use_stage3=True).run(...).
 One step = one full PaMO.run from device-resident input arrays to the numpy
 result the reference returns: centring, every stage-2 round with its guard
 and undo loop, and stage 3 (five steps of ten Newton iterations against the
-2M-face input surface).  Prints one JSON line.
+2M-face input surface), each step from a fresh object's state.  Before the
+timed steps the launch-shape autotuner picks the guard-tree reuse and guard
+split on this mesh (results unchanged).  Prints one JSON line with:
+
+- roofline: the dominant kernel's algorithmic bytes over its HIP-event time,
+  and its HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE,
+  WRITE_SIZE) over one stage-2 run, scaled by a plain copy's counters;
+- cpu_baseline: the CPU oracle on this box's cores -- the whole stage-2 run
+  (also compared bit for bit with this run's stage 2: stage2_vs_oracle) and
+  stage 3 timed per Newton iteration and scaled;
+- legs: the icosphere workload (stage 2, against the oracle), a batch of
+  meshes against the plain form, and a short served Poisson stream against
+  plain FIFO.
 """
 from __future__ import annotations
 
@@ -47,6 +59,7 @@ def parse():
     p.add_argument("--serve-seconds", type=float, default=5.0, help="serve: length of the arrival stream")
     p.add_argument("--serve-rate", type=float, default=40.0, help="serve: Poisson arrival rate (jobs/s)")
     p.add_argument("--no-legs", action="store_true", help="headline only (no icosphere / batch / serve legs)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     p.add_argument("--leg-batch-n", type=int, default=24, help="legs: meshes in the batch leg")
     p.add_argument("--leg-serve-seconds", type=float, default=1.5, help="legs: length of the served stream")
     p.add_argument("--no-traffic", action="store_true",
@@ -500,6 +513,20 @@ def main():
     pm = PaMO((V, F), use_stage1=False, use_stage3=use_s3, exact_guard=args.exact_guard)
     ctx = pm.ctx
     prof = _hip.Profiler(ctx)
+    tune = None
+    if not args.no_autotune:
+        # launch shapes (guard-tree reuse, guard subtree split) picked on this
+        # mesh before the timed runs; they change timings, never results
+        from pamo_amd import autotune
+        from pamo_amd.pamo import face_centroid_mean
+        mean = face_centroid_mean(ctx, Vd, Fd)
+        Vc = (Vd - torch.from_numpy(mean).to(dev)).contiguous()
+        tune = autotune.tune_stage2(Vc, Fd, ratio, tol, thr, repeats=2,
+                                    log=lambda m: print(m, file=sys.stderr, flush=True))
+        autotune.apply(ctx, tune["settings"])
+        tune = {"settings": tune["settings"],
+                "stage2_ms": {k: {str(v): round(t * 1e3, 2) for v, t in d.items()} for k, d in tune["timings"].items()}}
+        del Vc
 
     def one_run():
         # each run starts from a fresh object's stage-2 state: the context
@@ -604,6 +631,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stage2_vs_oracle": parity,
+        "autotune": tune,
         "quality": quality,
         "kernels": stats,
     }

@@ -61,9 +61,16 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * loop keeps the guard tree of one round for the next ones -- its leaves are
  * renumbered at each compaction, removed faces keep empty boxes, and each
  * round refits the boxes -- and builds a new one once fewer than this
- * percentage of its leaves are live (default 50; 100 builds every round).
+ * percentage of its leaves are live (default 80; 100 builds every round).
  * The flags, hence every result, do not depend on the tree. */
 #define PAMO_OPT_TREE_KEEP 5
+/* Guard launch shape (no counterpart in the reference; results do not
+ * change): on meshes of at most PAMO_OPT_GUARD_SPLIT_FACES faces (default
+ * 262144) each group of queries is served by 2^PAMO_OPT_GUARD_SPLIT_LOG
+ * waves (default 3, at most 4), one per subtree under the root, so that one
+ * query with a large box does not hold a late-round launch in one wave. */
+#define PAMO_OPT_GUARD_SPLIT_FACES 6
+#define PAMO_OPT_GUARD_SPLIT_LOG 7
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).

@@ -18,6 +18,8 @@ PAMO_OPT_EXACT_GUARD = 2
 PAMO_OPT_WALK_LIMIT = 3
 PAMO_OPT_PLACEMENT = 4
 PAMO_OPT_TREE_KEEP = 5
+PAMO_OPT_GUARD_SPLIT_FACES = 6
+PAMO_OPT_GUARD_SPLIT_LOG = 7
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3

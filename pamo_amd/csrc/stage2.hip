@@ -140,10 +140,14 @@ struct Ctx {
     // guard tree carried across rounds (pamo_simplify's loop): the leaves of
     // the last build, renumbered at each compaction, dead ones with empty
     // boxes; rebuilt once fewer than tree_keep_pct % of them are live
-    int tree_keep_pct = 50;     // 100: rebuild every round
+    int tree_keep_pct = 80;     // 100: rebuild every round
     bool tree_valid = false;    // bval / hierarchy describe the current faces
     bool tree_fresh = false;    // the next round may refit instead of building
     int tree_n = 0;             // leaves of the last tree, as last read back
+    // guard launch shape: meshes of at most guard_split_faces faces serve
+    // each query group with 2^guard_split_log waves, one per subtree
+    int guard_split_faces = 1 << 18;
+    int guard_split_log = 3;
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -304,11 +308,12 @@ static int grow_tris(Ctx& c, size_t n) {
 
 // edge_cost keeps stale values between rounds; it restarts at zero when the
 // logical edge capacity grows (reference: cusimp_free.cu:180-194).
-static int grow_edges(Ctx& c, size_t n) {
+static int grow_edges(Ctx& c, size_t n, bool* reallocated = nullptr) {
     if (n <= c.cap_edges) return 0;
     c.cap_edges = n + n / 5;
     if (c.cap_edges + 1 > c.alloc_edges) {
         PAMO_TRY(alloc_edges(c, c.cap_edges + 1));
+        if (reallocated) *reallocated = true;
     } else {
         PAMO_HIP(hipMemsetAsync(c.edge_cost, 0, sizeof(uint32_t) * (c.alloc_edges + 1), c.stream));
     }
@@ -432,7 +437,7 @@ static int run_guard(Ctx& c, int nT, bool refit) {
     // meshes up to 256k faces split every query group over 8 subtrees: late
     // rounds hold few queries, and one query with a large box would otherwise
     // hold a whole launch in one wave
-    const int part_log = nT <= (1 << 18) ? 3 : 0;
+    const int part_log = nT <= c.guard_split_faces ? c.guard_split_log : 0;
     const long long guard_threads = ((((long long)nT + kGuardQ - 1) / kGuardQ) << part_log) * kWave;
     const double guard_bytes = 4.0 * nT + 12.0 * nT + 16.0 * nP + 32.0 * nT + 64.0 * nT + 8.0 * nT + 4.0 * nT + 1.0 * nT;
     Undecided u{nullptr, c.dcnt + C_UNDECIDED, 0, c.exact_guard == 2 ? 1 : 0};
@@ -487,23 +492,28 @@ static int undo_candidates(Ctx& c, int& npushed) {
 // c.n_edges.  Shared by the round and the legacy round.
 // ring_off and ring_fill must hold zeros (round_init_k).
 static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
-    LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP, ring_count_k, nT, m, nT, c.ring_off);
+    // per-vertex ring counts and per-face edge counts in one pass
+    LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP + 4.0 * nT, topo_count_k, nT + 1, m, nT, c.ring_off, c.edge_off);
     PAMO_TRY(scan_exclusive(c.ring_off, c.ring_off, nP + 1, c.ws, c.stream));
     LAUNCHP(K_RING_FILL, 12.0 * nT + 12.0 * nT + 32.0 * nT, ring_fill_k, nT, m, nT, c.ring_off, c.ring_fill,
             c.ring_raw, c.ring_own);
     LAUNCHP(K_RING_SORT, 4.0 * nP + 24.0 * nT + 12.0 * nT + 60.0 * nT, ring_rank_k, 3LL * nT, m, c.ring_off,
             c.ring_raw, c.ring_own, 3 * nT, c.ring_off + nP, c.ring, c.rv);
-    // edges
-    LAUNCHP(K_EDGE_COUNT, 16.0 * nT, edge_count_k, nT + 1, m, nT, c.edge_off);
+    // edges: at most two per face, which the edge buffers always hold, so the
+    // fill and the quadrics run before the host reads the count back (the
+    // read then waits on them instead of stalling the stream)
     PAMO_TRY(scan_exclusive(c.edge_off, c.edge_off, nT + 1, c.ws, c.stream));
-    PAMO_HIP(hipMemcpyAsync(c.dcnt + C_NEDGE, c.edge_off + nT, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+    PAMO_CHECK((size_t)2 * nT <= c.alloc_edges, "edge buffers smaller than two edges per face");
+    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt);
+    // quadrics
+    LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
     PAMO_TRY(sync_counters(c));
     const int nE = c.hcnt[C_NEDGE];
     c.n_edges = nE;
-    PAMO_TRY(grow_edges(c, (size_t)nE));
-    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev);
-    // quadrics
-    LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
+    bool realloc = false;
+    PAMO_TRY(grow_edges(c, (size_t)nE, &realloc));
+    if (realloc)  // fresh edge buffers: fill them again
+        LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt);
     // edge-cost launch order; ring_fill is free once the rings are sorted
     PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
     LAUNCH(edge_order_count_k, nE, c.eu, nE, c.ring_fill);
@@ -576,9 +586,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
                 c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
                 c.esel, (const float4*)c.Q);
     PAMO_TRY(scan_exclusive(c.esel, c.pflag, nE + 1, c.ws, c.stream));
-    LAUNCH(compact_k, nE, c.esel, c.pflag, nE, c.recs, 0);
-    hipLaunchKernelGGL(store_count_k, dim3(1), dim3(1), 0, c.stream, c.pflag, nE, c.dcnt, (int)C_NREG);
-    PAMO_HIP(hipGetLastError());
+    LAUNCH(compact_k, nE, c.esel, c.pflag, nE, c.recs, 0, c.dcnt, (int)C_NREG);
     // cleanup
     LAUNCH(remove_invalid_k, nT, m, nT);
     LAUNCHP(K_LINE_FLAG, 8.0 * nE + 16.0 * nP + 4.0 * nE, line_flag_k, nE + 1, m, c.eu, c.ev, nE, c.esel);
@@ -589,20 +597,21 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_HIP(hipGetLastError());
     LAUNCHP(K_MARK_CHANGED, 1.0 * nT, mark_changed_k, nE, c.recs, c.dcnt, c.ring_off, c.ring, c.eu, c.ev,
             c.changed);
+    // guard + undo loop (:1014-1040).  The guard is queued before the record
+    // counts are read back, so the read waits on the guard's kernels instead
+    // of stalling the stream; a round without undo candidates ignores its
+    // flags.  A tree carried from the last round is refit, otherwise built.
+    PAMO_TRY(run_guard(c, nT, c.tree_valid && c.tree_fresh));
+    c.tree_valid = true;
     PAMO_TRY(sync_counters(c));
     c.n_regular = c.hcnt[C_NREG];
     c.n_rec = c.n_regular + c.hcnt[C_LINE];
     c.n_cand = std::min(c.n_rec, (c.n_regular + 511) / 512 * 512);
-    // guard + undo loop (:1014-1040)
+    c.tree_n = c.hcnt[C_NLIVE];
     int first = c.n_vertices_undo;  // entries already placed by the prologue
     int npushed = 0;
     c.undo_iters = 0;
-    if (c.n_cand > 0) {
-        // a tree carried from the last round is refit; otherwise built
-        PAMO_TRY(run_guard(c, nT, c.tree_valid && c.tree_fresh));
-        c.tree_valid = true;
-        PAMO_TRY(undo_candidates(c, npushed));
-    }
+    if (c.n_cand > 0) PAMO_TRY(undo_candidates(c, npushed));
     c.n_vertices_undo += 2 * npushed;
     int it = 0;
     while (npushed != 0) {
@@ -754,6 +763,16 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     if (option == PAMO_OPT_GUARD_SMALL_STACK) {
         PAMO_CHECK(value == 0 || value == 1, "pamo_ctx_set_option: value must be 0 or 1");
         c.guard_small_stack = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_GUARD_SPLIT_FACES) {
+        PAMO_CHECK(value >= 0 && value <= 0x7fffffff, "pamo_ctx_set_option: split face count out of range");
+        c.guard_split_faces = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_GUARD_SPLIT_LOG) {
+        PAMO_CHECK(value >= 0 && value <= 4, "pamo_ctx_set_option: split depth must be in [0, 4]");
+        c.guard_split_log = (int)value;
         return 0;
     }
     if (option == PAMO_OPT_TREE_KEEP) {

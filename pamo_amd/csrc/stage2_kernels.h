@@ -160,15 +160,6 @@ __global__ void invalid_set_k(const int* __restrict__ undo_in, int n, uint8_t* i
 }
 
 // ---------------------------------------------------------------- rings and edges (A4, A5)
-__global__ void ring_count_k(Mesh m, int nT, int* __restrict__ cnt) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nT) return;
-    int a = m.i[f], b = m.j[f], c = m.k[f];
-    if (face_dead(a, b, c)) return;
-    atomicAdd(&cnt[a], 1);
-    atomicAdd(&cnt[b], 1);
-    atomicAdd(&cnt[c], 1);
-}
 
 __global__ void ring_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ fill,
                             int* __restrict__ raw, int* __restrict__ owner) {
@@ -208,19 +199,32 @@ __global__ void __launch_bounds__(kBlock) ring_rank_k(Mesh m, const int* __restr
     rv[b + r] = make_int4(m.i[x], m.j[x], m.k[x], x);
 }
 
-__global__ void edge_count_k(Mesh m, int nT, int* __restrict__ cnt) {
+
+// Per-face counts in one pass: each live face adds 1 to its three vertices'
+// ring counters (ring_count_k) and stores its emitted-edge count (edge_count_k).
+__global__ void topo_count_k(Mesh m, int nT, int* __restrict__ ring_cnt, int* __restrict__ edge_cnt) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f > nT) return;
     if (f == nT) {
-        cnt[nT] = 0;
+        edge_cnt[nT] = 0;
         return;
     }
     int a = m.i[f], b = m.j[f], c = m.k[f];
-    cnt[f] = face_dead(a, b, c) ? 0 : (int)(a > b) + (int)(b > c) + (int)(c > a);
+    if (face_dead(a, b, c)) {
+        edge_cnt[f] = 0;
+        return;
+    }
+    atomicAdd(&ring_cnt[a], 1);
+    atomicAdd(&ring_cnt[b], 1);
+    atomicAdd(&ring_cnt[c], 1);
+    edge_cnt[f] = (int)(a > b) + (int)(b > c) + (int)(c > a);
 }
 
-__global__ void edge_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ eu, int* __restrict__ ev) {
+// Also stores the edge count off[nT] to cnt[C_NEDGE].
+__global__ void edge_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ eu, int* __restrict__ ev,
+                            int* __restrict__ cnt) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f == 0) cnt[C_NEDGE] = off[nT];
     if (f >= nT) return;
     int a = m.i[f], b = m.j[f], c = m.k[f];
     if (face_dead(a, b, c)) return;
@@ -691,9 +695,11 @@ __global__ void __launch_bounds__(kBlock) pasp_collapse_k(Mesh m, int* occ, cons
     rewrite_rings(m, off, ring, u, v);
 }
 
+// out[pos[e]] = e for flagged e; the total pos[n] goes to cnt[slot] when cnt is given.
 __global__ void compact_k(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out,
-                          int base) {
+                          int base, int* cnt = nullptr, int slot = 0) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0 && cnt) cnt[slot] = pos[n];
     if (e >= n) return;
     if (flag[e]) out[base + pos[e]] = e;
 }
@@ -747,7 +753,6 @@ __global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, cons
     }
 }
 
-__global__ void store_count_k(const int* __restrict__ pos, int n, int* cnt, int slot) { cnt[slot] = pos[n]; }
 
 __global__ void mark_changed_k(const int* __restrict__ recs, const int* __restrict__ cnt,
                                const int* __restrict__ off, const int* __restrict__ ring,

@@ -163,11 +163,13 @@ class PaMO(torch.nn.Module):
     reference, which always collapses an edge to its midpoint) moves a
     collapsed edge to the minimiser of its summed quadric, solved in FP64,
     with the midpoint kept where that system is singular; the default
-    "midpoint" keeps the reference's results.
+    "midpoint" keeps the reference's results.  ``tuning`` takes launch-shape
+    settings found by ``pamo_amd.autotune.tune_stage2`` (they change timings,
+    never results).
     """
 
     def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True, exact_guard: bool = False,
-                 placement: str = "midpoint"):
+                 placement: str = "midpoint", tuning: dict | None = None):
         super().__init__()
         if use_stage1:
             raise NotImplementedError(
@@ -191,6 +193,10 @@ class PaMO(torch.nn.Module):
         self.placement = placement
         if placement == "optimal":
             _hip.check(_hip.lib().pamo_ctx_set_option(self.ctx.ptr, _hip.PAMO_OPT_PLACEMENT, 1), "set_option")
+        if tuning:
+            # launch-shape settings from pamo_amd.autotune (results unchanged)
+            from . import autotune
+            autotune.apply(self.ctx, tuning)
         if use_stage3:
             from . import stage3
             self.config = stage3.Stage3Config()

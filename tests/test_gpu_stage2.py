@@ -378,3 +378,29 @@ def test_tree_carry_matches_rebuild_every_round(torch_cuda, oracle_lib, case):
         assert np.array_equal(hg, ho)
         assert np.array_equal(fg, fo)
         assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32))
+
+
+def test_autotuned_settings_give_the_same_result(torch_cuda, oracle_lib):
+    """The autotuner's knobs (guard-tree keep percentage, guard subtree
+    split) change the launch shape only: every candidate gives the oracle's
+    result bit for bit, and the tuner returns one of them."""
+    from pamo_amd import synth, _hip, autotune
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(6, 3)
+    Vc = centred(V, F)
+    Vd, Fd = torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda()
+    print(f"case: autotune knobs, icosphere L6, {len(F)} faces", flush=True)
+    vo, fo, ho = oracle_lib.run(Vc, F, 0.02, min_verts=0)
+    for settings in ({"guard_split_log": 0}, {"guard_split_log": 2}, {"guard_split_log": 4},
+                     {"guard_split_faces": 0}, {"tree_keep": 30, "guard_split_faces": 1 << 20}):
+        ctx = _hip.Context(0)
+        autotune.apply(ctx, settings)
+        vg, fg, hg = simplify_device(ctx, Vd, Fd, 0.02, min_verts=0)
+        assert np.array_equal(hg, ho), settings
+        assert np.array_equal(fg.cpu().numpy(), fo), settings
+        assert np.array_equal(vg.cpu().numpy().view(np.uint32), vo.view(np.uint32)), settings
+    res = autotune.tune_stage2(Vd, Fd, 0.02, repeats=1,
+                               candidates={"tree_keep": [50, 100], "guard_split_log": [2, 3]})
+    assert res["settings"]["tree_keep"] in (50, 100) and res["settings"]["guard_split_log"] in (2, 3)
+    assert autotune.cached(len(F)) == res["settings"]
