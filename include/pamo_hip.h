@@ -71,6 +71,11 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * query with a large box does not hold a late-round launch in one wave. */
 #define PAMO_OPT_GUARD_SPLIT_FACES 6
 #define PAMO_OPT_GUARD_SPLIT_LOG 7
+/* Guard register budget (no counterpart in the reference; results do not
+ * change): 0 (default) runs the guard kernel as the compiler allocates its
+ * registers, 4 runs a variant capped at 128 vector registers so that 4 waves
+ * fit per SIMD. */
+#define PAMO_OPT_GUARD_WAVES 8
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).
@@ -278,6 +283,8 @@ int pamo_s3_set_dense(pamo_s3* s3, const double* h_A, const double* h_b, double 
 #define PAMO_S3_DBG_CCD 8      /* detect + _clamp_p + _ccd on in: p (n_v,3);
                                   out: clamped p (n_v,3) then ccd_step                 */
 #define PAMO_S3_DBG_HINGE 9    /* after DIFF: projected hinge blocks (n_h,144)        */
+#define PAMO_S3_DBG_RESOLVE 10 /* CGSolver.solve again on the gradient and diagonal of
+                                  the last DIFF / SOLVE; out: p (n_v,3)              */
 int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, double* h_out, long long cap,
                   long long* n_out);
 
@@ -286,6 +293,15 @@ int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, dou
  * [3] CG iterations, [4] line-search evaluations, [5] current detection
  * radius, [6] do_nothing flag, [7] number of hinges. */
 int pamo_s3_stats(pamo_s3* s3, double* out8);
+
+/* Run-time options (no counterpart in the reference's interface, which
+ * fixes its CUDA-graph capture of the PCG loop in config.py:21).
+ * PAMO_S3_OPT_CG_GRAPH: 1 (default) captures each solve's PCG kernels from
+ * the stream into a HIP graph, updated in place between solves, and
+ * launches it as one unit; 0 launches them one by one.  Results do not
+ * change. */
+#define PAMO_S3_OPT_CG_GRAPH 1
+int pamo_s3_set_option(pamo_s3* s3, int option, long long value);
 
 /* ---------------------------------------------------------------- batched and served forms
  * No counterpart in the reference, which runs one mesh per PaMO.run call

@@ -20,6 +20,7 @@ PAMO_OPT_PLACEMENT = 4
 PAMO_OPT_TREE_KEEP = 5
 PAMO_OPT_GUARD_SPLIT_FACES = 6
 PAMO_OPT_GUARD_SPLIT_LOG = 7
+PAMO_OPT_GUARD_WAVES = 8
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3
@@ -51,6 +52,8 @@ PAMO_S3_DBG_HVP = 6
 PAMO_S3_DBG_SOLVE = 7
 PAMO_S3_DBG_CCD = 8
 PAMO_S3_DBG_HINGE = 9
+PAMO_S3_DBG_RESOLVE = 10
+PAMO_S3_OPT_CG_GRAPH = 1
 
 
 class S3Params(ctypes.Structure):
@@ -94,6 +97,7 @@ SIGNATURES = [
     ("pamo_s3_set_dense", c_int, [c_vp, c_vp, c_vp, c_double, c_int]),
     ("pamo_s3_debug", c_int, [c_vp, c_int, c_vp, c_ll, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_s3_stats", c_int, [c_vp, c_vp]),
+    ("pamo_s3_set_option", c_int, [c_vp, c_int, c_ll]),
     ("pamo_s3_prepare_host", c_int, [c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double), c_vp,
                                      ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_double)]),
     ("pamo_s3_normalization", c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),

@@ -9,6 +9,8 @@ faces and vertices, bit for bit):
   stale trees (cheaper rounds, looser boxes); 100 builds every round.
 - PAMO_OPT_GUARD_SPLIT_FACES / PAMO_OPT_GUARD_SPLIT_LOG: below this face count
   each group of guard queries is spread over 2^log waves, one per subtree.
+- PAMO_OPT_GUARD_WAVES: 0 runs the guard kernel with the compiler's register
+  allocation, 4 a variant capped for 4 waves per SIMD.
 
 `tune_stage2` times the fused loop (pamo_simplify) on a caller's mesh for
 each candidate value of one knob at a time (coordinate descent, the other
@@ -26,14 +28,15 @@ import torch
 
 from . import _hip
 
-DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 18, "guard_split_log": 3}
+DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 18, "guard_split_log": 3, "guard_waves": 0}
 CANDIDATES = {
     "tree_keep": [50, 65, 80, 90, 100],
     "guard_split_faces": [1 << 16, 1 << 18, 1 << 20],
     "guard_split_log": [2, 3, 4],
+    "guard_waves": [0, 4],
 }
 _OPTION = {"tree_keep": _hip.PAMO_OPT_TREE_KEEP, "guard_split_faces": _hip.PAMO_OPT_GUARD_SPLIT_FACES,
-           "guard_split_log": _hip.PAMO_OPT_GUARD_SPLIT_LOG}
+           "guard_split_log": _hip.PAMO_OPT_GUARD_SPLIT_LOG, "guard_waves": _hip.PAMO_OPT_GUARD_WAVES}
 _CACHE: dict = {}
 
 

@@ -26,7 +26,8 @@ struct Tree {
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2* range = nullptr;
     float4 *box = nullptr, *pre = nullptr, *suf = nullptr, *blk = nullptr, *blkpre = nullptr, *blksuf = nullptr,
-           *sblk = nullptr;
+           *sblk = nullptr, *blktab = nullptr;
+    long long tab_stride = 0;  // entries per level of blktab
     float* part = nullptr;
     int* cnt = nullptr;  // [C_COUNT]; cnt[C_NLIVE] = n
 };
@@ -57,6 +58,8 @@ static int tree_alloc(Tree& t, int n, hipStream_t st) {
     size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
     PAMO_TRY(dalloc(&t.blk, 2 * nb, st)); PAMO_TRY(dalloc(&t.blkpre, 2 * nb, st)); PAMO_TRY(dalloc(&t.blksuf, 2 * nb, st));
     PAMO_TRY(dalloc(&t.sblk, 2 * ns, st));
+    PAMO_TRY(dalloc(&t.blktab, 2 * kTabLevels * nb, st));
+    t.tab_stride = (long long)nb;
     PAMO_TRY(dalloc(&t.part, 6 * (a / kBlock + 4), st));
     if (!t.cnt) PAMO_TRY(dalloc(&t.cnt, C_COUNT, st));
     return 0;
@@ -64,7 +67,7 @@ static int tree_alloc(Tree& t, int n, hipStream_t st) {
 
 static void tree_free(Tree& t) {
     void* ptrs[] = {t.item, t.key, t.val, t.key2, t.val2, t.left, t.right, t.parent, t.range, t.box,
-                    t.pre,  t.suf, t.blk, t.blkpre, t.blksuf, t.sblk, t.part, t.cnt};
+                    t.pre,  t.suf, t.blk, t.blkpre, t.blksuf, t.sblk, t.blktab, t.part, t.cnt};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     t = Tree{};
@@ -98,12 +101,31 @@ __global__ void items_morton_k(const float4* __restrict__ item, int n, const flo
     val[i] = (uint32_t)i;
 }
 
+// Leaf and node boxes of a built hierarchy (box_tab_k, node_box_k) and,
+// unless refitting a tree whose links are already written, its rope links.
+static int tree_boxes(Tree& t, int n, bool refit, hipStream_t st) {
+    const unsigned nb = grid_for(n, kBlock), nb1 = grid_for(nb, kBlock);
+    hipLaunchKernelGGL(box_tab_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, (const uint32_t*)t.val,
+                       (const int*)t.cnt, 0, t.box, (const int2*)t.range, t.pre, t.suf, t.blk, (float4*)nullptr, 0LL);
+    hipLaunchKernelGGL(box_tab_k, dim3(nb1), dim3(kBlock), 0, st, (const float4*)t.blk, (const uint32_t*)nullptr,
+                       (const int*)t.cnt, 1, (float4*)nullptr, (const int2*)nullptr, t.blkpre, t.blksuf, t.sblk,
+                       t.blktab, t.tab_stride);
+    hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
+                       (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blktab, t.tab_stride,
+                       (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
+    if (!refit)
+        hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * n, kBlock)), dim3(kBlock), 0, st, (const int*)t.left,
+                           (const int2*)t.range, (const int*)t.cnt, (const uint32_t*)t.val, t.box);
+    PAMO_HIP(hipGetLastError());
+    return 0;
+}
+
 // Builds the tree over t.item[0..n) (boxes already written).
 static int tree_build(Tree& t, int n, int* ws, hipStream_t st) {
     t.n = n;
     PAMO_TRY(fill(t.cnt + C_NLIVE, n, 1, st));
     if (n <= 0) return 0;
-    const unsigned nb = grid_for(n, kBlock), nb1 = grid_for(nb, kBlock);
+    const unsigned nb = grid_for(n, kBlock);
     hipLaunchKernelGGL(items_box_part_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, n, t.part);
     hipLaunchKernelGGL(whole_box_k, dim3(1), dim3(kBlock), 0, st, t.part, (int)nb);
     hipLaunchKernelGGL(items_morton_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, n,
@@ -112,39 +134,18 @@ static int tree_build(Tree& t, int n, int* ws, hipStream_t st) {
     PAMO_TRY(radix_sort_pairs(t.key, t.val, t.key2, t.val2, n, 32, ws, st));
     hipLaunchKernelGGL(karras_k, dim3(nb), dim3(kBlock), 0, st, (const uint32_t*)t.key, (const uint32_t*)t.val,
                        (const int*)t.cnt, t.left, t.right, t.parent, t.range);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, (const uint32_t*)t.val,
-                       (const int*)t.cnt, 0, t.box, t.pre, t.suf, t.blk);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, st, (const float4*)t.blk, (const uint32_t*)nullptr,
-                       (const int*)t.cnt, 1, (float4*)nullptr, t.blkpre, t.blksuf, t.sblk);
-    hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
-                       (const float4*)t.box, (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blk,
-                       (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
-    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * n, kBlock)), dim3(kBlock), 0, st, (const int*)t.left,
-                       (const int*)t.right, (const int*)t.parent, (const int*)t.cnt, (const uint32_t*)t.val, t.box);
-    PAMO_HIP(hipGetLastError());
-    return 0;
+    return tree_boxes(t, n, false, st);
 }
 
 // Recomputes every box of a tree built by tree_build over the same n items,
 // after the items' boxes changed: leaf boxes in the built leaf order, node
-// boxes over the built hierarchy, and the rope links (which share the box
-// records).  The tree stays exact for any item boxes; its quality degrades
+// boxes over the built hierarchy; the rope links in the records' .w fields
+// stay as built.  The tree stays exact for any item boxes; its quality degrades
 // only as far as the items moved since the build.
 static int tree_refit(Tree& t, int n, hipStream_t st) {
     PAMO_CHECK(n == t.n, "tree_refit: item count differs from the built tree");
     if (n <= 0) return 0;
-    const unsigned nb = grid_for(n, kBlock), nb1 = grid_for(nb, kBlock);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb), dim3(kBlock), 0, st, (const float4*)t.item, (const uint32_t*)t.val,
-                       (const int*)t.cnt, 0, t.box, t.pre, t.suf, t.blk);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, st, (const float4*)t.blk, (const uint32_t*)nullptr,
-                       (const int*)t.cnt, 1, (float4*)nullptr, t.blkpre, t.blksuf, t.sblk);
-    hipLaunchKernelGGL(node_box_k, dim3(nb), dim3(kBlock), 0, st, (const int2*)t.range, (const int*)t.cnt,
-                       (const float4*)t.box, (const float4*)t.pre, (const float4*)t.suf, (const float4*)t.blk,
-                       (const float4*)t.blkpre, (const float4*)t.blksuf, (const float4*)t.sblk, t.box);
-    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * n, kBlock)), dim3(kBlock), 0, st, (const int*)t.left,
-                       (const int*)t.right, (const int*)t.parent, (const int*)t.cnt, (const uint32_t*)t.val, t.box);
-    PAMO_HIP(hipGetLastError());
-    return 0;
+    return tree_boxes(t, n, true, st);
 }
 
 struct TreeView {

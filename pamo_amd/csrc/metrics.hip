@@ -323,6 +323,8 @@ static size_t noalloc_layout(unsigned char* base, int max_faces, Work& w) {
     const size_t nb = a / kBlock + 2, ns = nb / kBlock + 2;
     take(t.blk, 2 * nb); take(t.blkpre, 2 * nb); take(t.blksuf, 2 * nb);
     take(t.sblk, 2 * ns);
+    take(t.blktab, 2 * kTabLevels * nb);
+    t.tab_stride = (long long)nb;
     take(t.part, 6 * (a / kBlock + 4));
     take(t.cnt, (size_t)C_COUNT);
     t.cap = (int)a;

@@ -6,6 +6,8 @@
 //     (match of the 8 digit bits) and an LDS prefix across the 4 waves, so
 //     the order of equal keys is preserved exactly.
 #pragma once
+#include <algorithm>
+
 #include "hip_common.h"
 
 namespace pamo {
@@ -176,6 +178,136 @@ __global__ void __launch_bounds__(kBlockScan) scan_block_k(const int* in, int* o
     }
 }
 
+// Single-launch scans by decoupled look-back.  Each workgroup takes a tile
+// number from a ticket counter (so a tile only waits on tiles whose
+// workgroups have already started), scans its items, publishes its total,
+// and wave 0 then reads the states of up to 64 preceding tiles at a time:
+// it adds their totals back to the nearest tile that has published its
+// inclusive prefix, and publishes its own inclusive prefix.  A state word
+// packs the call's epoch (bits 34-63), a status (bit 32: total, bit 33:
+// inclusive prefix) and the 32-bit value, so words left by earlier calls
+// never match and the states need no clearing between calls.  Every wait is
+// bounded: after kLbSpinCap reads the tile reports bit 2 in *err (the host
+// turns it into an error status) and goes on.
+//
+// The item values and what is done with their prefixes come from an
+// operation `Op`: Op::count(i) is item i's value (i < n), Op::emit(i, v,
+// pos) receives its exclusive prefix, Op::done(total) runs once, in the last
+// tile, with the sum of all values, and Op::kIpt is the number of
+// consecutive items per thread.  So a flag computation, the scan and the
+// compaction it feeds are one launch.
+constexpr unsigned long long kLbTotal = 1ull << 32, kLbInclusive = 2ull << 32;
+constexpr int kLbSpinCap = 1 << 20;
+
+struct LookbackScan {
+    unsigned long long* state = nullptr;  // one word per tile
+    unsigned* ticket = nullptr;           // tiles handed out so far (mod 2^32)
+    int* err = nullptr;
+    long long cap = 0;                    // tiles
+    unsigned epoch = 0, base = 0;         // this call's epoch; ticket value at its start
+};
+
+__device__ __forceinline__ unsigned long long lb_word(unsigned epoch, unsigned long long status, int v) {
+    return ((unsigned long long)epoch << 34) | status | (unsigned long long)(unsigned)v;
+}
+
+// Exclusive prefix of this tile's total among the tiles before it (wave 0
+// of the workgroup; lane 0's return value is the one used).
+__device__ __forceinline__ int lb_prefix(int tile, int total, unsigned long long* __restrict__ state,
+                                         int* __restrict__ err, unsigned epoch) {
+    const int lane = threadIdx.x;
+    int excl = 0;
+    if (tile == 0) {
+        if (lane == 0)
+            __hip_atomic_store(state, lb_word(epoch, kLbInclusive, total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (lane == 0)
+        __hip_atomic_store(state + tile, lb_word(epoch, kLbTotal, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    for (int end = tile - 1;; end -= kWave) {
+        const int j = end - lane;
+        unsigned long long w = lb_word(epoch, kLbInclusive, 0);  // before tile 0: prefix 0
+        if (j >= 0) {
+            for (int spins = 0;; ++spins) {
+                w = __hip_atomic_load(state + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((unsigned)(w >> 34) == epoch && (w & (kLbTotal | kLbInclusive)) != 0) break;
+                if (spins >= kLbSpinCap) {
+                    atomicOr(err, 2);
+                    w = lb_word(epoch, kLbInclusive, 0);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        const unsigned long long inc = __ballot((w & kLbInclusive) != 0);
+        const int stop = inc ? __ffsll((long long)inc) - 1 : kWave - 1;
+        int part = lane <= stop ? (int)(unsigned)w : 0;
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) part += __shfl_xor(part, d, kWave);
+        excl += part;
+        if (inc) break;
+    }
+    if (lane == 0)
+        __hip_atomic_store(state + tile, lb_word(epoch, kLbInclusive, excl + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kScanBlock) scan_op_k(Op op, long long n, unsigned long long* __restrict__ state,
+                                                        unsigned* __restrict__ ticket, int* __restrict__ err,
+                                                        unsigned epoch, unsigned base) {
+    constexpr int kIpt = Op::kIpt;
+    __shared__ int lds[4];
+    __shared__ int sh[2];
+    if (threadIdx.x == 0) sh[0] = (int)(atomicAdd(ticket, 1u) - base);
+    __syncthreads();
+    const int tile = sh[0];
+    if ((unsigned)tile >= gridDim.x) {  // ticket out of step with the host's count
+        if (threadIdx.x == 0) atomicOr(err, 2);
+        return;
+    }
+    const long long first = ((long long)tile * kScanBlock + threadIdx.x) * kIpt;
+    int v[kIpt];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kIpt; ++i) {
+        const long long idx = first + i;
+        v[i] = idx < n ? op.count(idx) : 0;
+        s += v[i];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, lds, total);
+    if (threadIdx.x < kWave) {
+        const int excl = lb_prefix(tile, total, state, err, epoch);
+        if (threadIdx.x == 0) {
+            sh[1] = excl;
+            if (tile == (int)gridDim.x - 1) op.done(excl + total);
+        }
+    }
+    __syncthreads();
+    ex += sh[1];
+#pragma unroll
+    for (int i = 0; i < kIpt; ++i) {
+        const long long idx = first + i;
+        if (idx < n) op.emit(idx, v[i], ex);
+        ex += v[i];
+    }
+}
+
+// out[i] = sum_{j<i} in[j] (in may alias out: a tile reads all its items
+// before writing any).
+struct ScanOp {
+    static constexpr int kIpt = kScanIpt;
+    const int* in;
+    int* out;
+    __device__ int count(long long i) const { return in[i]; }
+    __device__ void emit(long long i, int, int pos) const { out[i] = pos; }
+    __device__ void done(int) const {}
+};
+
 // Workspace (ints) needed by scan_exclusive for n elements.
 inline long long scan_workspace(long long n) { return (n + kScanTile - 1) / kScanTile + 16; }
 
@@ -212,6 +344,62 @@ inline int scan_exclusive(const int* in, int* out, long long n, int* ws, hipStre
     prof_end(K_SCAN_TILES, st);
     PAMO_HIP(hipGetLastError());
     return 0;
+}
+
+// Room for scan_op launches of up to n items at any items-per-thread count
+// (tile states zeroed when they grow; the ticket counter and its host copy
+// stay in step).
+inline int lookback_reserve(LookbackScan& lb, long long n, int* err, hipStream_t st) {
+    lb.err = err;
+    if (!lb.ticket) {
+        PAMO_HIP(hipMalloc((void**)&lb.ticket, sizeof(unsigned)));
+        PAMO_HIP(hipMemsetAsync(lb.ticket, 0, sizeof(unsigned), st));
+        lb.base = 0;
+    }
+    const long long tiles = (n + kScanBlock - 1) / kScanBlock + 16;  // room for one item per thread
+    if (tiles > lb.cap) {
+        if (lb.state) PAMO_HIP(hipFree(lb.state));
+        lb.state = nullptr;
+        const long long cap = tiles + tiles / 4;
+        PAMO_HIP(hipMalloc((void**)&lb.state, sizeof(unsigned long long) * cap));
+        PAMO_HIP(hipMemsetAsync(lb.state, 0, sizeof(unsigned long long) * cap, st));
+        lb.cap = cap;
+        lb.epoch = 0;
+    }
+    return 0;
+}
+
+inline void lookback_free(LookbackScan& lb) {
+    if (lb.state) (void)hipFree(lb.state);
+    if (lb.ticket) (void)hipFree(lb.ticket);
+    lb = LookbackScan{};
+}
+
+// One launch of scan_op_k over n >= 0 items (a single tile needs no waiting).
+template <class Op>
+inline int scan_op(const Op& op, long long n, LookbackScan& lb, hipStream_t st, int prof_id = K_SCAN_TILES,
+                   double bytes = 0.0) {
+    if (n < 0) return 0;
+    constexpr long long kTile = (long long)kScanBlock * Op::kIpt;
+    const long long nb = std::max(1LL, (n + kTile - 1) / kTile);  // n = 0 still runs Op::done(0)
+    PAMO_CHECK(lb.state && nb <= lb.cap, "look-back scan: tile states not reserved for this size");
+    if (lb.epoch + 1 >= (1u << 30)) {  // epochs exhausted: clear the states and start again
+        PAMO_HIP(hipMemsetAsync(lb.state, 0, sizeof(unsigned long long) * lb.cap, st));
+        lb.epoch = 0;
+    }
+    ++lb.epoch;
+    prof_begin(prof_id, bytes, st);
+    hipLaunchKernelGGL(scan_op_k<Op>, dim3((unsigned)nb), dim3(kScanBlock), 0, st, op, n, lb.state, lb.ticket,
+                       lb.err, lb.epoch, lb.base);
+    prof_end(prof_id, st);
+    PAMO_HIP(hipGetLastError());
+    lb.base += (unsigned)nb;
+    return 0;
+}
+
+// scan_exclusive in one launch for any size.  in may alias out.
+inline int scan_exclusive_lb(const int* in, int* out, long long n, LookbackScan& lb, hipStream_t st) {
+    return scan_op(ScanOp{in, out}, n, lb, st, K_SCAN_TILES, 8.0 * n);
 }
 
 // ---------------------------------------------------------------- radix sort

@@ -102,6 +102,7 @@ struct Ctx {
     float4 *p = nullptr, *op = nullptr, *np = nullptr;
     float4* Q = nullptr;  // 3 per point
     int *occ = nullptr, *pts_map = nullptr, *ring_off = nullptr, *ring_fill = nullptr;
+    int* eord_cnt = nullptr;  // edges per smaller endpoint, then the launch-order cursor
     int *undo_list = nullptr, *tmp_undo = nullptr, *undo_prev = nullptr;  // 4 * alloc_pts
     uint8_t* invalid = nullptr;
     uint64_t* vmin = nullptr;  // per-vertex minimum edge key
@@ -119,7 +120,8 @@ struct Ctx {
     int *bleft = nullptr, *bright = nullptr, *bparent = nullptr;  // parent: 2n-1 nodes
     int2* brange = nullptr;
     float4 *bbox = nullptr, *fbox = nullptr, *bpre = nullptr, *bsuf = nullptr;  // 2 float4 per box
-    float4 *blk = nullptr, *blkpre = nullptr, *blksuf = nullptr, *sblk = nullptr;
+    float4 *blk = nullptr, *blkpre = nullptr, *blksuf = nullptr, *sblk = nullptr, *blktab = nullptr;
+    long long blktab_stride = 0;  // entries per level of blktab
     float* box_part = nullptr;
     // edge sized
     int *eu = nullptr, *ev = nullptr;
@@ -128,6 +130,7 @@ struct Ctx {
     int* eord = nullptr;  // edge ids grouped by smaller endpoint (edge-cost launch order)
     int* shuf = nullptr;
     int* ws = nullptr;
+    LookbackScan lb;  // single-launch scans of the round loop
     int* dcnt = nullptr;
     int* hcnt = nullptr;
     // last round, host view
@@ -148,6 +151,7 @@ struct Ctx {
     // each query group with 2^guard_split_log waves, one per subtree
     int guard_split_faces = 1 << 18;
     int guard_split_log = 3;
+    int guard_waves = 0;        // 0: the compiler's register allocation for the guard; 4: capped for 4 waves
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -189,7 +193,7 @@ static int ensure_ws(Ctx& c, long long n_items) {
         c.alloc_ws = (size_t)need + need / 4;
         PAMO_TRY(dalloc(&c.ws, c.alloc_ws, c.stream));
     }
-    return 0;
+    return lookback_reserve(c.lb, n_items + 1, c.dcnt + C_WALK_ERR, c.stream);
 }
 
 // Point-sized buffers.  The logical capacity follows the reference's growth
@@ -217,6 +221,7 @@ static int grow_pts(Ctx& c, size_t n) {
         PAMO_TRY(zero_async(c, c.Q, 3 * a));
         PAMO_TRY(zero_async(c, c.occ, a + 1)); PAMO_TRY(zero_async(c, c.pts_map, a + 1));
         PAMO_TRY(zero_async(c, c.ring_off, a + 2)); PAMO_TRY(zero_async(c, c.ring_fill, a + 1));
+        PAMO_TRY(zero_async(c, c.eord_cnt, a + 1));
         PAMO_TRY(zero_async(c, c.undo_list, 4 * a)); PAMO_TRY(zero_async(c, c.tmp_undo, 4 * a));
         PAMO_TRY(zero_async(c, c.undo_prev, 4 * a));
         PAMO_TRY(zero_async(c, c.invalid, a)); PAMO_TRY(zero_async(c, c.vmin, a));
@@ -231,6 +236,7 @@ static int grow_pts(Ctx& c, size_t n) {
     PAMO_TRY(dalloc(&c.pts_map, a + 1, c.stream));
     PAMO_TRY(dalloc(&c.ring_off, a + 2, c.stream));
     PAMO_TRY(dalloc(&c.ring_fill, a + 1, c.stream));
+    PAMO_TRY(dalloc(&c.eord_cnt, a + 1, c.stream));
     PAMO_TRY(dalloc(&c.undo_list, 4 * a, c.stream));
     PAMO_TRY(dalloc(&c.tmp_undo, 4 * a, c.stream));
     PAMO_TRY(dalloc(&c.undo_prev, 4 * a, c.stream));
@@ -300,6 +306,8 @@ static int grow_tris(Ctx& c, size_t n) {
     PAMO_TRY(dalloc(&c.blkpre, 2 * nb, c.stream));
     PAMO_TRY(dalloc(&c.blksuf, 2 * nb, c.stream));
     PAMO_TRY(dalloc(&c.sblk, 2 * ns, c.stream));
+    PAMO_TRY(dalloc(&c.blktab, 2 * kTabLevels * nb, c.stream));
+    c.blktab_stride = (long long)nb;
     PAMO_TRY(ensure_parts(c, a));
     PAMO_TRY(dalloc(&c.shuf, 4 * a + 64, c.stream));
     if (2 * a + 1 > c.alloc_edges) PAMO_TRY(alloc_edges(c, 2 * a + 1));  // at most 2 edges per face
@@ -321,14 +329,15 @@ static int grow_edges(Ctx& c, size_t n, bool* reallocated = nullptr) {
 }
 
 static void free_all(Ctx& c) {
-    void* ptrs[] = {c.p, c.op, c.np, c.Q, c.occ, c.pts_map, c.ring_off, c.ring_fill, c.undo_list, c.tmp_undo,
+    void* ptrs[] = {c.p, c.op, c.np, c.Q, c.occ, c.pts_map, c.ring_off, c.ring_fill, c.eord_cnt, c.undo_list, c.tmp_undo,
                     c.undo_prev, c.invalid, c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk, c.ring, c.ring_raw, c.ring_own, c.rv,
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
-                    c.blksuf, c.sblk, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
+                    c.blksuf, c.sblk, c.blktab, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
                     c.shuf, c.ws, c.dcnt, c.cent_out, c.ulist};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    lookback_free(c.lb);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
 }
 
@@ -375,10 +384,11 @@ static int sync_counters(Ctx& c) {
     PAMO_HIP(hipMemcpyAsync(c.hcnt, c.dcnt, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
     if (c.hcnt[C_WALK_ERR] || c.hcnt[C_BAD_UNDO]) {
-        const bool walk = c.hcnt[C_WALK_ERR] != 0;
+        const bool walk = (c.hcnt[C_WALK_ERR] & 1) != 0, scan = (c.hcnt[C_WALK_ERR] & 2) != 0;
         c.hcnt[C_WALK_ERR] = c.hcnt[C_BAD_UNDO] = 0;
         PAMO_HIP(hipMemsetAsync(c.dcnt + C_WALK_ERR, 0, 2 * sizeof(int), c.stream));
         PAMO_CHECK(!walk, "guard: a tree walk exceeded its bound (corrupted LBVH)");
+        PAMO_CHECK(!scan, "scan: a look-back wait exceeded its bound");
         PAMO_CHECK(false, "undo list: entries outside the points of the previous call");
     }
     return 0;
@@ -416,24 +426,25 @@ static int run_guard(Ctx& c, int nT, bool refit) {
     // node boxes from leaf ranges: no communication between workgroups
     const unsigned nb0 = grid_for(nL, kBlock), nb1 = grid_for(nb0, kBlock);
     prof_begin(K_BVH_BOXES, 4.0 * nT + 32.0 * nT + 32.0 * nT + 64.0 * nT, c.stream);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb0), dim3(kBlock), 0, c.stream, (const float4*)c.fbox,
-                       (const uint32_t*)c.bval, (const int*)c.dcnt, 0, c.bbox, c.bpre, c.bsuf, c.blk);
-    hipLaunchKernelGGL(box_scan_k, dim3(nb1), dim3(kBlock), 0, c.stream, (const float4*)c.blk,
-                       (const uint32_t*)nullptr, (const int*)c.dcnt, 1, (float4*)nullptr, c.blkpre, c.blksuf,
-                       c.sblk);
+    hipLaunchKernelGGL(box_tab_k, dim3(nb0), dim3(kBlock), 0, c.stream, (const float4*)c.fbox,
+                       (const uint32_t*)c.bval, (const int*)c.dcnt, 0, c.bbox, (const int2*)c.brange, c.bpre, c.bsuf,
+                       c.blk, (float4*)nullptr, 0LL);
+    hipLaunchKernelGGL(box_tab_k, dim3(nb1), dim3(kBlock), 0, c.stream, (const float4*)c.blk,
+                       (const uint32_t*)nullptr, (const int*)c.dcnt, 1, (float4*)nullptr, (const int2*)nullptr,
+                       c.blkpre, c.blksuf, c.sblk, c.blktab, c.blktab_stride);
     hipLaunchKernelGGL(node_box_k, dim3(grid_for(nL, kBlock)), dim3(kBlock), 0, c.stream, (const int2*)c.brange,
-                       (const int*)c.dcnt, (const float4*)c.bbox + 0, (const float4*)c.bpre, (const float4*)c.bsuf,
-                       (const float4*)c.blk, (const float4*)c.blkpre, (const float4*)c.blksuf,
-                       (const float4*)c.sblk, c.bbox);
-    hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * nL, kBlock)), dim3(kBlock), 0, c.stream, (const int*)c.bleft,
-                       (const int*)c.bright, (const int*)c.bparent, (const int*)c.dcnt, (const uint32_t*)c.bval,
+                       (const int*)c.dcnt, (const float4*)c.bpre, (const float4*)c.bsuf, (const float4*)c.blktab,
+                       c.blktab_stride, (const float4*)c.blkpre, (const float4*)c.blksuf, (const float4*)c.sblk,
                        c.bbox);
+    // a refit keeps the hierarchy, so the links in the records' .w fields
+    // stand (box_tab_k rewrites each leaf's face id, which a carry renumbers)
+    if (!refit)
+        hipLaunchKernelGGL(rope_k, dim3(grid_for(2LL * nL, kBlock)), dim3(kBlock), 0, c.stream, (const int*)c.bleft,
+                           (const int2*)c.brange, (const int*)c.dcnt, (const uint32_t*)c.bval, c.bbox);
     prof_end(K_BVH_BOXES, c.stream);
     PAMO_HIP(hipGetLastError());
     // queries: changed live faces in Morton order
-    LAUNCH(query_flag_k, nL + 1, c.bval, c.dcnt, c.changed, (const int*)c.ti, nT, nL, c.face_scan);
-    PAMO_TRY(scan_exclusive(c.face_scan, c.pflag, nL + 1, c.ws, c.stream));
-    LAUNCH(query_list_k, nL, c.bval, c.face_scan, c.pflag, nL, c.qlist, c.dcnt);
+    PAMO_TRY(scan_op(QueryListOp{c.bval, c.changed, c.ti, nT, c.qlist, c.dcnt}, nL, c.lb, c.stream));
     // meshes up to 256k faces split every query group over 8 subtrees: late
     // rounds hold few queries, and one query with a large box would otherwise
     // hold a whole launch in one wave
@@ -460,6 +471,9 @@ static int run_guard(Ctx& c, int nT, bool refit) {
     } else if (c.guard_small_stack) {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStackSmall, false>), guard_threads, m, c.qlist,
                 c.dcnt, c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
+    } else if (c.guard_waves == 4) {
+        LAUNCHP(K_GUARD_QUERY, guard_bytes, guard_query_w4_k, guard_threads, m, c.qlist, c.dcnt, c.changed, c.fbox,
+                (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
     } else {
         LAUNCHP(K_GUARD_QUERY, guard_bytes, (guard_query_k<kGuardStack, false>), guard_threads, m, c.qlist, c.dcnt,
                 c.changed, c.fbox, (const float4*)c.bbox, c.bright, 1e-3f, c.flagged, u, part_log, c.dcnt + C_WALK_ERR, c.walk_cap);
@@ -473,10 +487,8 @@ static int run_guard(Ctx& c, int nT, bool refit) {
 // candidates).
 static int undo_candidates(Ctx& c, int& npushed) {
     const int n = c.n_cand;
-    LAUNCHP(K_UNDO_CAND, 16.0 * n + 1.0 * c.n_tris, undo_cand_k, n + 1, c.recs, n, c.ring_off, c.ring, c.eu, c.ev,
-            c.flagged, c.pflag);
-    PAMO_TRY(scan_exclusive(c.pflag, c.esel, n + 1, c.ws, c.stream));
-    LAUNCH(undo_compact_k, std::max(n, 1), c.recs, n, c.pflag, c.esel, c.pushed, c.dcnt);
+    PAMO_TRY(scan_op(UndoListOp{c.recs, c.ring_off, c.ring, c.eu, c.ev, c.flagged, c.pushed, c.dcnt}, n, c.lb,
+                     c.stream, K_UNDO_CAND, 16.0 * n + 1.0 * c.n_tris));
     PAMO_TRY(sync_counters(c));
     npushed = c.hcnt[C_PUSHED];
     c.tree_n = c.hcnt[C_NLIVE];  // leaves of the tree this guard used
@@ -490,11 +502,11 @@ static int undo_candidates(Ctx& c, int& npushed) {
 // Vertex -> face rings in ascending face order (:276-304), the edge list
 // (:306-335), the quadrics (:337-366) and the edge-cost launch order; sets
 // c.n_edges.  Shared by the round and the legacy round.
-// ring_off and ring_fill must hold zeros (round_init_k).
+// ring_off, ring_fill and eord_cnt must hold zeros (round_init_k).
 static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     // per-vertex ring counts and per-face edge counts in one pass
     LAUNCHP(K_RING_COUNT, 12.0 * nT + 8.0 * nP + 4.0 * nT, topo_count_k, nT + 1, m, nT, c.ring_off, c.edge_off);
-    PAMO_TRY(scan_exclusive(c.ring_off, c.ring_off, nP + 1, c.ws, c.stream));
+    PAMO_TRY(scan_exclusive_lb(c.ring_off, c.ring_off, nP + 1, c.lb, c.stream));
     LAUNCHP(K_RING_FILL, 12.0 * nT + 12.0 * nT + 32.0 * nT, ring_fill_k, nT, m, nT, c.ring_off, c.ring_fill,
             c.ring_raw, c.ring_own);
     LAUNCHP(K_RING_SORT, 4.0 * nP + 24.0 * nT + 12.0 * nT + 60.0 * nT, ring_rank_k, 3LL * nT, m, c.ring_off,
@@ -502,9 +514,10 @@ static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     // edges: at most two per face, which the edge buffers always hold, so the
     // fill and the quadrics run before the host reads the count back (the
     // read then waits on them instead of stalling the stream)
-    PAMO_TRY(scan_exclusive(c.edge_off, c.edge_off, nT + 1, c.ws, c.stream));
+    PAMO_TRY(scan_exclusive_lb(c.edge_off, c.edge_off, nT + 1, c.lb, c.stream));
     PAMO_CHECK((size_t)2 * nT <= c.alloc_edges, "edge buffers smaller than two edges per face");
-    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt);
+    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
+            c.eord_cnt);
     // quadrics
     LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
     PAMO_TRY(sync_counters(c));
@@ -512,13 +525,15 @@ static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     c.n_edges = nE;
     bool realloc = false;
     PAMO_TRY(grow_edges(c, (size_t)nE, &realloc));
-    if (realloc)  // fresh edge buffers: fill them again
-        LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt);
-    // edge-cost launch order; ring_fill is free once the rings are sorted
-    PAMO_HIP(hipMemsetAsync(c.ring_fill, 0, sizeof(int) * (nP + 1), c.stream));
-    LAUNCH(edge_order_count_k, nE, c.eu, nE, c.ring_fill);
-    PAMO_TRY(scan_exclusive(c.ring_fill, c.ring_fill, nP + 1, c.ws, c.stream));
-    LAUNCH(edge_order_scatter_k, nE, c.eu, nE, c.ring_fill, c.eord);
+    if (realloc) {  // fresh edge buffers: fill and count them again
+        PAMO_HIP(hipMemsetAsync(c.eord_cnt, 0, sizeof(int) * (nP + 1), c.stream));
+        LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
+                c.eord_cnt);
+    }
+    // edge-cost launch order: edges grouped by smaller endpoint (counted by
+    // edge_fill_k)
+    PAMO_TRY(scan_exclusive_lb(c.eord_cnt, c.eord_cnt, nP + 1, c.lb, c.stream));
+    LAUNCH(edge_order_scatter_k, nE, c.eu, nE, c.eord_cnt, c.eord);
     return 0;
 }
 
@@ -540,7 +555,7 @@ static int pasp_round_device(Ctx& c, int nP, int nT, float scale, float thr) {
     c.n_pts = nP;
     c.n_tris = nT;
     LAUNCH(round_init_k, std::max(nP + 1, nT), m, Mesh{nullptr, nullptr, nullptr, nullptr}, nP, nT, c.occ, c.ring_off,
-           c.ring_fill, (unsigned long long*)c.vmin, (uint8_t*)nullptr);
+           c.ring_fill, c.eord_cnt, (unsigned long long*)c.vmin, (uint8_t*)nullptr);
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     LAUNCHP(K_EDGE_COST, 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 4.0 * nE, pasp_cost_k, nE, m,
@@ -550,7 +565,7 @@ static int pasp_round_device(Ctx& c, int nP, int nT, float scale, float thr) {
     LAUNCHP(K_SELECT_COLLAPSE, 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP, pasp_collapse_k, nE, m, c.occ,
             c.ring_off, c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin,
             collapse_t);
-    PAMO_TRY(scan_exclusive(c.occ, c.pts_map, nP + 1, c.ws, c.stream));
+    PAMO_TRY(scan_exclusive_lb(c.occ, c.pts_map, nP + 1, c.lb, c.stream));
     return 0;
 }
 
@@ -562,7 +577,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     c.n_tris = nT;
     // snapshot for undo, occupancy, ring counters, minimum keys, changed flags
     LAUNCH(round_init_k, std::max(nP + 1, nT), m, Mesh{c.op, c.oi, c.oj, c.ok}, nP, nT, c.occ, c.ring_off, c.ring_fill,
-           (unsigned long long*)c.vmin, c.changed);
+           c.eord_cnt, (unsigned long long*)c.vmin, c.changed);
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
@@ -585,15 +600,12 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
         LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, select_collapse_k<false>, nE + 1, m, orig_mesh(c), c.occ, c.ring_off,
                 c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
                 c.esel, (const float4*)c.Q);
-    PAMO_TRY(scan_exclusive(c.esel, c.pflag, nE + 1, c.ws, c.stream));
-    LAUNCH(compact_k, nE, c.esel, c.pflag, nE, c.recs, 0, c.dcnt, (int)C_NREG);
+    PAMO_TRY(scan_op(CollapseListOp{c.esel, c.recs, c.dcnt}, nE, c.lb, c.stream));
     // cleanup
-    LAUNCH(remove_invalid_k, nT, m, nT);
-    LAUNCHP(K_LINE_FLAG, 8.0 * nE + 16.0 * nP + 4.0 * nE, line_flag_k, nE + 1, m, c.eu, c.ev, nE, c.esel);
-    PAMO_TRY(scan_exclusive(c.esel, c.pflag, nE + 1, c.ws, c.stream));
-    LAUNCH(line_list_k, nE, c.esel, c.pflag, nE, c.recs, c.dcnt);
+    PAMO_TRY(scan_op(CleanupOp{m, c.eu, c.ev, nE, nT, c.recs, c.dcnt}, (long long)std::max(nE, nT), c.lb, c.stream,
+                     K_LINE_FLAG, 8.0 * nE + 16.0 * nP + 4.0 * nE + 12.0 * nT));
     hipLaunchKernelGGL(line_apply_k, dim3(1), dim3(1), 0, c.stream, m, c.occ, c.ring_off, c.ring, c.eu, c.ev,
-                       c.pflag, nE, c.recs, c.dcnt);
+                       (const int*)c.recs, c.dcnt);
     PAMO_HIP(hipGetLastError());
     LAUNCHP(K_MARK_CHANGED, 1.0 * nT, mark_changed_k, nE, c.recs, c.dcnt, c.ring_off, c.ring, c.eu, c.ev,
             c.changed);
@@ -618,9 +630,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
         ++it;
         ++c.undo_iters;
         LAUNCH(undo_apply_k, npushed, m, orig_mesh(c), c.occ, c.ring_off, c.ring, c.eu, c.ev, c.pushed, npushed,
-               c.tmp_undo);
-        LAUNCH(rearrange_k, c.n_vertices_undo - first, c.tmp_undo, c.n_vertices_undo - first, c.pts_map,
-               c.undo_list, first);
+               (const int*)c.pts_map, c.undo_list, first);
         first += 2 * npushed;
         PAMO_TRY(run_guard(c, nT, true));
         PAMO_TRY(undo_candidates(c, npushed));
@@ -628,7 +638,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
         if (it == 5) break;
     }
     // new vertex ids
-    PAMO_TRY(scan_exclusive(c.occ, c.pts_map, nP + 1, c.ws, c.stream));
+    PAMO_TRY(scan_exclusive_lb(c.occ, c.pts_map, nP + 1, c.lb, c.stream));
     return 0;
 }
 
@@ -654,15 +664,12 @@ static int prologue(Ctx& c, const int* undo_in, int n_undo_in, bool is_stuck) {
 // tree_keep_pct % of its leaves are still live.
 static int compact_round(Ctx& c, int nP, int nT, bool carry = false) {
     Mesh m = cur_mesh(c);
-    LAUNCH(live_flag_k, nT + 1, m, nT, c.face_scan);
-    PAMO_TRY(scan_exclusive(c.face_scan, c.face_scan, nT + 1, c.ws, c.stream));
-    LAUNCHP(K_COMPACT_MESH, 4.0 * nP + 4.0 * nP + 16.0 * nP + 16.0 * nP + 12.0 * nT + 4.0 * nT + 12.0 * nT,
-            compact_mesh_k, std::max(nP, nT), m, nP, nT, c.occ, c.pts_map, c.face_scan, c.np, c.ni, c.nj, c.nk,
-            c.dcnt);
+    PAMO_TRY(scan_op(CompactMeshOp{m, nP, nT, c.occ, c.pts_map, c.face_scan, c.np, c.ni, c.nj, c.nk, c.dcnt},
+                     (long long)std::max(nP, nT) + 1, c.lb, c.stream, K_COMPACT_MESH,
+                     4.0 * nP + 4.0 * nP + 16.0 * nP + 16.0 * nP + 12.0 * nT + 4.0 * nT + 12.0 * nT));
     carry = carry && c.tree_valid && c.tree_keep_pct < 100;
     if (carry) {
-        hipLaunchKernelGGL(fill_k<int>, dim3(1), dim3(64), 0, c.stream, c.dcnt + C_LIVE_LEAVES, 0, 1LL);
-        // over every leaf: a carried tree can have more leaves than faces
+        // over every leaf (CompactMeshOp has zeroed the live-leaf count): a carried tree can have more leaves than faces
         LAUNCH(carry_leaves_k, std::max(nT, c.tree_n), c.bval, (const int*)c.ti, (const int*)c.face_scan, nT,
                c.dcnt);
     }
@@ -690,7 +697,7 @@ static int shuffle_faces(Ctx& c, int nT) {
     int* flag = c.shuf;
     uint32_t* img = (uint32_t*)(c.shuf + domain + 1);
     LAUNCH(shuffle_flags_k, domain + 1, fs, domain, (uint32_t)nT, flag, img);
-    PAMO_TRY(scan_exclusive(flag, flag, domain + 1, c.ws, c.stream));
+    PAMO_TRY(scan_exclusive_lb(flag, flag, domain + 1, c.lb, c.stream));
     PAMO_HIP(hipMemcpyAsync(c.ni, c.ti, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
     PAMO_HIP(hipMemcpyAsync(c.nj, c.tj, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
     PAMO_HIP(hipMemcpyAsync(c.nk, c.tk, sizeof(int) * nT, hipMemcpyDeviceToDevice, c.stream));
@@ -773,6 +780,11 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     if (option == PAMO_OPT_GUARD_SPLIT_LOG) {
         PAMO_CHECK(value >= 0 && value <= 4, "pamo_ctx_set_option: split depth must be in [0, 4]");
         c.guard_split_log = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_GUARD_WAVES) {
+        PAMO_CHECK(value == 0 || value == 4, "pamo_ctx_set_option: guard waves must be 0 (compiler's choice) or 4");
+        c.guard_waves = (int)value;
         return 0;
     }
     if (option == PAMO_OPT_TREE_KEEP) {

@@ -199,7 +199,6 @@ __global__ void __launch_bounds__(kBlock) ring_rank_k(Mesh m, const int* __restr
     rv[b + r] = make_int4(m.i[x], m.j[x], m.k[x], x);
 }
 
-
 // Per-face counts in one pass: each live face adds 1 to its three vertices'
 // ring counters (ring_count_k) and stores its emitted-edge count (edge_count_k).
 __global__ void topo_count_k(Mesh m, int nT, int* __restrict__ ring_cnt, int* __restrict__ edge_cnt) {
@@ -221,17 +220,19 @@ __global__ void topo_count_k(Mesh m, int nT, int* __restrict__ ring_cnt, int* __
 }
 
 // Also stores the edge count off[nT] to cnt[C_NEDGE].
+// Also counts the edges per smaller endpoint into order_cnt (zeroed by
+// round_init_k), the histogram of the edge-cost launch order.
 __global__ void edge_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ eu, int* __restrict__ ev,
-                            int* __restrict__ cnt) {
+                            int* __restrict__ cnt, int* __restrict__ order_cnt) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f == 0) cnt[C_NEDGE] = off[nT];
     if (f >= nT) return;
     int a = m.i[f], b = m.j[f], c = m.k[f];
     if (face_dead(a, b, c)) return;
     int q = off[f];
-    if (a > b) { eu[q] = b; ev[q] = a; ++q; }
-    if (b > c) { eu[q] = c; ev[q] = b; ++q; }
-    if (c > a) { eu[q] = a; ev[q] = c; ++q; }
+    if (a > b) { eu[q] = b; ev[q] = a; ++q; atomicAdd(&order_cnt[b], 1); }
+    if (b > c) { eu[q] = c; ev[q] = b; ++q; atomicAdd(&order_cnt[c], 1); }
+    if (c > a) { eu[q] = a; ev[q] = c; ++q; atomicAdd(&order_cnt[a], 1); }
 }
 
 // ---------------------------------------------------------------- quadrics and costs (A6, A7)
@@ -365,11 +366,6 @@ __device__ __forceinline__ void edge_quadric(const float4* __restrict__ Q, int u
 // order, so in this order the lanes of a wave read neighbouring rings,
 // positions and quadrics.  The order inside one group depends on atomic
 // timing; it changes only which lane computes an edge, never a result.
-__global__ void edge_order_count_k(const int* __restrict__ eu, int nE, int* __restrict__ cnt) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nE) atomicAdd(&cnt[eu[e]], 1);
-}
-
 __global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __restrict__ cursor,
                                      int* __restrict__ order) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,55 +691,16 @@ __global__ void __launch_bounds__(kBlock) pasp_collapse_k(Mesh m, int* occ, cons
     rewrite_rings(m, off, ring, u, v);
 }
 
-// out[pos[e]] = e for flagged e; the total pos[n] goes to cnt[slot] when cnt is given.
-__global__ void compact_k(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out,
-                          int base, int* cnt = nullptr, int slot = 0) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e == 0 && cnt) cnt[slot] = pos[n];
-    if (e >= n) return;
-    if (flag[e]) out[base + pos[e]] = e;
-}
-
 // ---------------------------------------------------------------- cleanup (A10)
-__global__ void remove_invalid_k(Mesh m, int nT) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nT) return;
-    int a = m.i[f], b = m.j[f], c = m.k[f];
-    if (face_dead(a, b, c)) return;
-    if (a == b || b == c || c == a) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
-}
-
-__global__ void line_flag_k(Mesh m, const int* __restrict__ eu, const int* __restrict__ ev, int nE,
-                            int* __restrict__ flag) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e > nE) return;
-    if (e == nE) {
-        flag[nE] = 0;
-        return;
-    }
-    F3 p = ld(m, eu[e]), q = ld(m, ev[e]);
-    flag[e] = (p.x == q.x && p.y == q.y && p.z == q.z) ? 1 : 0;
-}
-
-// The flagged line edges in edge order, appended to the records after the
-// regular collapses (recs[cnt[C_NREG] + pos[e]] = e).
-__global__ void line_list_k(const int* __restrict__ flag, const int* __restrict__ pos, int nE,
-                            int* __restrict__ recs, const int* __restrict__ cnt) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nE || !flag[e]) return;
-    recs[cnt[C_NREG] + pos[e]] = e;
-}
-
 // Line collapses are rare; applying them in edge order in one lane keeps
 // the result deterministic where the reference races on shared faces.  The
-// lane walks the list line_list_k built, not the whole edge array.
+// lane walks the list CleanupOp built, not the whole edge array.
 __global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
-                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ pos,
-                             int nE, const int* __restrict__ recs, int* cnt) {
+                             const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ recs,
+                             const int* cnt) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int base = cnt[C_NREG];
-    const int nl = pos[nE];
-    cnt[C_LINE] = nl;
+    const int nl = cnt[C_LINE];
     for (int w = 0; w < nl; ++w) {
         const int e = recs[base + w];
         const int u = eu[e], v = ev[e];
@@ -752,7 +709,6 @@ __global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, cons
         rewrite_rings(m, off, ring, u, v);
     }
 }
-
 
 __global__ void mark_changed_k(const int* __restrict__ recs, const int* __restrict__ cnt,
                                const int* __restrict__ off, const int* __restrict__ ring,
@@ -892,8 +848,8 @@ __global__ void __launch_bounds__(kBlock) whole_box_k(float* __restrict__ part, 
 // (when snap.p is set), occupancy 1 (0 at nP), zeroed ring counters and
 // changed flags (when given), per-vertex minimum keys at the maximum.
 __global__ void round_init_k(Mesh m, Mesh snap, int nP, int nT, int* __restrict__ occ, int* __restrict__ ring_off,
-                             int* __restrict__ ring_fill, unsigned long long* __restrict__ vmin,
-                             uint8_t* __restrict__ changed) {
+                             int* __restrict__ ring_fill, int* __restrict__ order_cnt,
+                             unsigned long long* __restrict__ vmin, uint8_t* __restrict__ changed) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nP) {
         if (snap.p) snap.p[i] = m.p[i];
@@ -904,6 +860,7 @@ __global__ void round_init_k(Mesh m, Mesh snap, int nP, int nT, int* __restrict_
     if (i <= nP) {
         ring_off[i] = 0;
         ring_fill[i] = 0;
+        order_cnt[i] = 0;
     }
     if (i < nT) {
         if (snap.p) {
@@ -1008,62 +965,107 @@ __global__ void karras_k(const uint32_t* __restrict__ key, const uint32_t* __res
     range[i] = make_int2(first, last);
 }
 
-// Sorted leaf boxes, and prefix / suffix boxes within aligned blocks of 256
-// leaves (LDS Hillis-Steele scans).  blk[b] receives each block's box.
-// The same kernel runs over block boxes to get superblock prefix / suffix.
-// level 0: n = live faces, boxes gathered through `gather` and also stored
-// as tree leaves (node n-1+s of dst_leaf); level 1: n = number of blocks.
-__global__ void __launch_bounds__(kBlock) box_scan_k(const float4* __restrict__ src, const uint32_t* __restrict__ gather,
-                                                     const int* __restrict__ cnt, int level,
-                                                     float4* __restrict__ dst_leaf, float4* __restrict__ pre,
-                                                     float4* __restrict__ suf, float4* __restrict__ blk) {
-    __shared__ float lo[3][kBlock], hi[3][kBlock];
-    __shared__ float lo2[3][kBlock], hi2[3][kBlock];
+// Box unions inside aligned blocks of 256 leaves, from a sparse table in
+// LDS: level k holds the union of entries [t, t + 2^k), so any range of the
+// block is the union of two table entries (min / max are exact and
+// idempotent, so overlapping halves give the same box).  One launch writes
+// the sorted leaf boxes (level 0: node n-1+s of `node`, lo.w = item id,
+// hi.w left as it is), the in-block prefix and suffix boxes, each block's
+// box, and (given `range`) the box of every internal node whose leaf range
+// lies inside the block: internal node i covers [i, j] or [j, i], so those
+// nodes are exactly the ones numbered inside the block.  Level 1 runs the
+// same kernel over the block boxes and also stores its table (`tab`, level
+// k at tab + 2 k stride) for the nodes that span blocks.
+constexpr int kTabLevels = 8;  // 2^0 .. 2^7 entries
+
+struct BoxTab {
+    float v[kTabLevels][6][kBlock];
+};
+
+__device__ __forceinline__ int tab_level(int len) { return min(31 - __clz(len), kTabLevels - 1); }
+
+// Union of table entries over [l, r] (0 <= l <= r < kBlock) into lo / hi.
+__device__ __forceinline__ void tab_union(const BoxTab& T, int l, int r, float* lo, float* hi) {
+    const int k = tab_level(r - l + 1), r0 = r - (1 << k) + 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = fminf(T.v[k][c][l], T.v[k][c][r0]);
+        hi[c] = fmaxf(T.v[k][3 + c][l], T.v[k][3 + c][r0]);
+    }
+}
+
+__device__ __forceinline__ void store_xyz(float4* q, const float* v) {
+    float* f = reinterpret_cast<float*>(q);
+    f[0] = v[0];
+    f[1] = v[1];
+    f[2] = v[2];
+}
+
+__global__ void __launch_bounds__(kBlock) box_tab_k(const float4* __restrict__ src, const uint32_t* __restrict__ gather,
+                                                    const int* __restrict__ cnt, int level,
+                                                    float4* __restrict__ node, const int2* __restrict__ range,
+                                                    float4* __restrict__ pre, float4* __restrict__ suf,
+                                                    float4* __restrict__ blk, float4* __restrict__ tab,
+                                                    long long tab_stride) {
+    __shared__ BoxTab T;
     const int nl = cnt[C_NLIVE];
     const int n = level == 0 ? nl : (nl + kBlock - 1) / kBlock;
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if ((long long)blockIdx.x * kBlock >= n) return;
+    const int base = blockIdx.x * kBlock;
+    if (base >= n) return;
+    const int t = threadIdx.x, s = base + t;
     const float inf = __builtin_huge_valf();
     float4 a = make_float4(inf, inf, inf, 0.0f), b = make_float4(-inf, -inf, -inf, 0.0f);
     if (s < n) {
-        long long g = gather ? (long long)gather[s] : s;
+        const long long g = gather ? (long long)gather[s] : s;
         a = src[2 * g];
         b = src[2 * g + 1];
-        if (dst_leaf) {
-            dst_leaf[2 * ((long long)nl - 1 + s)] = a;
-            dst_leaf[2 * ((long long)nl - 1 + s) + 1] = b;
+        if (node) {
+            float4* q = node + 2 * ((long long)nl - 1 + s);
+            q[0] = make_float4(a.x, a.y, a.z, __int_as_float((int)g));
+            const float hv[3] = {b.x, b.y, b.z};
+            store_xyz(q + 1, hv);
         }
     }
-    const int t = threadIdx.x;
-    lo[0][t] = a.x; lo[1][t] = a.y; lo[2][t] = a.z;
-    hi[0][t] = b.x; hi[1][t] = b.y; hi[2][t] = b.z;
-    lo2[0][t] = a.x; lo2[1][t] = a.y; lo2[2][t] = a.z;
-    hi2[0][t] = b.x; hi2[1][t] = b.y; hi2[2][t] = b.z;
+    T.v[0][0][t] = a.x; T.v[0][1][t] = a.y; T.v[0][2][t] = a.z;
+    T.v[0][3][t] = b.x; T.v[0][4][t] = b.y; T.v[0][5][t] = b.z;
+    for (int k = 1; k < kTabLevels; ++k) {
+        __syncthreads();
+        const int h = 1 << (k - 1);
+        const int u = t + h < kBlock ? t + h : t;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T.v[k][c][t] = fminf(T.v[k - 1][c][t], T.v[k - 1][c][u]);
+            T.v[k][3 + c][t] = fmaxf(T.v[k - 1][3 + c][t], T.v[k - 1][3 + c][u]);
+        }
+    }
     __syncthreads();
-    for (int d = 1; d < kBlock; d <<= 1) {
-        float pl[3], ph[3], sl[3], sh[3];
-        for (int c = 0; c < 3; ++c) {
-            pl[c] = t >= d ? fminf(lo[c][t], lo[c][t - d]) : lo[c][t];
-            ph[c] = t >= d ? fmaxf(hi[c][t], hi[c][t - d]) : hi[c][t];
-            sl[c] = t + d < kBlock ? fminf(lo2[c][t], lo2[c][t + d]) : lo2[c][t];
-            sh[c] = t + d < kBlock ? fmaxf(hi2[c][t], hi2[c][t + d]) : hi2[c][t];
-        }
-        __syncthreads();
-        for (int c = 0; c < 3; ++c) {
-            lo[c][t] = pl[c]; hi[c][t] = ph[c];
-            lo2[c][t] = sl[c]; hi2[c][t] = sh[c];
-        }
-        __syncthreads();
-    }
+    float lo[3], hi[3];
     if (s < n) {
-        pre[2 * (long long)s] = make_float4(lo[0][t], lo[1][t], lo[2][t], 0.0f);
-        pre[2 * (long long)s + 1] = make_float4(hi[0][t], hi[1][t], hi[2][t], 0.0f);
-        suf[2 * (long long)s] = make_float4(lo2[0][t], lo2[1][t], lo2[2][t], 0.0f);
-        suf[2 * (long long)s + 1] = make_float4(hi2[0][t], hi2[1][t], hi2[2][t], 0.0f);
+        tab_union(T, 0, t, lo, hi);
+        pre[2 * (long long)s] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        pre[2 * (long long)s + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        tab_union(T, t, kBlock - 1, lo, hi);
+        suf[2 * (long long)s] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        suf[2 * (long long)s + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        if (t == 0) {
+            blk[2 * (long long)blockIdx.x] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+            blk[2 * (long long)blockIdx.x + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        }
     }
-    if (t == 0) {
-        blk[2 * (long long)blockIdx.x] = make_float4(lo2[0][0], lo2[1][0], lo2[2][0], 0.0f);
-        blk[2 * (long long)blockIdx.x + 1] = make_float4(hi2[0][0], hi2[1][0], hi2[2][0], 0.0f);
+    if (range && s < nl - 1) {
+        const int2 r = range[s];
+        if ((r.x >> 8) == (r.y >> 8)) {
+            tab_union(T, r.x - base, r.y - base, lo, hi);
+            store_xyz(node + 2 * (long long)s, lo);
+            store_xyz(node + 2 * (long long)s + 1, hi);
+        }
+    }
+    if (tab && s < n) {
+        for (int k = 0; k < kTabLevels; ++k) {
+            float4* q = tab + 2 * (k * tab_stride + s);
+            q[0] = make_float4(T.v[k][0][t], T.v[k][1][t], T.v[k][2][t], 0.0f);
+            q[1] = make_float4(T.v[k][3][t], T.v[k][4][t], T.v[k][5][t], 0.0f);
+        }
     }
 }
 
@@ -1094,12 +1096,14 @@ __device__ __forceinline__ void grow_range(float4& lo, float4& hi, const float4*
     for (; q <= q1; ++q) grow(lo, hi, bx, base + q);
 }
 
-// Box of every internal node from its leaf range [a, b]: direct loop inside
-// one 256-leaf block, otherwise suffix(a) + prefix(b) + the full blocks in
-// between (again via block-level suffix / prefix and full superblocks).
+// Box of every internal node whose leaf range [a, b] spans blocks (box_tab_k
+// has written the others): suffix(a) + prefix(b) + the full blocks between,
+// which inside one superblock are two entries of the level-1 table, and
+// otherwise block-level suffix / prefix and the full superblocks.  Only the
+// xyz fields are written; the .w fields hold the hierarchy's links.
 __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ range, const int* __restrict__ cnt,
-                                                     const float4* __restrict__ leaf, const float4* __restrict__ pre,
-                                                     const float4* __restrict__ suf, const float4* __restrict__ blk,
+                                                     const float4* __restrict__ pre, const float4* __restrict__ suf,
+                                                     const float4* __restrict__ tab, long long tab_stride,
                                                      const float4* __restrict__ bpre,
                                                      const float4* __restrict__ bsuf,
                                                      const float4* __restrict__ sblk, float4* __restrict__ box) {
@@ -1107,28 +1111,28 @@ __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ ra
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int2 r = range[i];
+    const int A = r.x >> 8, B = r.y >> 8;
+    if (A == B) return;
     const float inf = __builtin_huge_valf();
     float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
-    const int A = r.x >> 8, B = r.y >> 8;
-    if (A == B) {
-        grow_range(lo, hi, leaf, (long long)(n - 1), r.x, r.y);
-    } else {
-        grow(lo, hi, suf, r.x);
-        grow(lo, hi, pre, r.y);
-        const int A1 = A + 1, B1 = B - 1;
-        if (A1 <= B1) {
-            const int SA = A1 >> 8, SB = B1 >> 8;
-            if (SA == SB) {
-                grow_range(lo, hi, blk, 0, A1, B1);
-            } else {
-                grow(lo, hi, bsuf, A1);
-                grow(lo, hi, bpre, B1);
-                grow_range(lo, hi, sblk, 0, SA + 1, SB - 1);
-            }
+    grow(lo, hi, suf, r.x);
+    grow(lo, hi, pre, r.y);
+    const int A1 = A + 1, B1 = B - 1;
+    if (A1 <= B1) {
+        const int SA = A1 >> 8, SB = B1 >> 8;
+        if (SA == SB) {
+            const int k = tab_level(B1 - A1 + 1);
+            grow(lo, hi, tab + 2 * k * tab_stride, A1);
+            grow(lo, hi, tab + 2 * k * tab_stride, B1 - (1 << k) + 1);
+        } else {
+            grow(lo, hi, bsuf, A1);
+            grow(lo, hi, bpre, B1);
+            grow_range(lo, hi, sblk, 0, SA + 1, SB - 1);
         }
     }
-    box[2 * (long long)i] = lo;
-    box[2 * (long long)i + 1] = hi;
+    const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
+    store_xyz(box + 2 * (long long)i, l3);
+    store_xyz(box + 2 * (long long)i + 1, h3);
 }
 
 // ---------------------------------------------------------------- guard (A12, A13)
@@ -1138,22 +1142,26 @@ __global__ void __launch_bounds__(kBlock) node_box_k(const int2* __restrict__ ra
 // done: the right sibling of the first ancestor-or-self that is a left
 // child, -1 past the end.  Traversal then needs one 32-byte record per step
 // and no stack or parent walk.
-__global__ void __launch_bounds__(kBlock) rope_k(const int* __restrict__ left, const int* __restrict__ right,
-                                                 const int* __restrict__ parent, const int* __restrict__ cnt,
-                                                 const uint32_t* __restrict__ val, float4* __restrict__ node) {
+__global__ void __launch_bounds__(kBlock) rope_k(const int* __restrict__ left, const int2* __restrict__ range,
+                                                 const int* __restrict__ cnt, const uint32_t* __restrict__ val,
+                                                 float4* __restrict__ node) {
     const int n = cnt[C_NLIVE];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (n <= 0 || x >= 2 * n - 1) return;
+    // x's subtree ends at leaf b; the next node in depth-first order is the
+    // right child of the node that splits between leaves b and b+1.  That
+    // child is internal node b+1 when that node's range starts at b+1 (an
+    // internal node is numbered by the end of its range next to its
+    // parent's split, so it is a right child exactly when it starts its
+    // range), and otherwise the leaf b+1.
+    const bool leaf = x >= n - 1;
+    const int b = leaf ? x - (n - 1) : range[x].y;
     int esc = -1;
-    for (int y = x; y != 0;) {
-        const int p = parent[y];
-        if (left[p] == y) {
-            esc = right[p];
-            break;
-        }
-        y = p;
+    if (b < n - 1) {
+        const int c = b + 1;
+        esc = (c <= n - 2 && range[c].x == c) ? c : c + (n - 1);
     }
-    const int lw = x >= n - 1 ? (int)val[x - (n - 1)] : left[x];
+    const int lw = leaf ? (int)val[x - (n - 1)] : left[x];
     float* r = reinterpret_cast<float*>(node + 2 * (long long)x);
     r[3] = __int_as_float(lw);
     r[7] = __int_as_float(esc);
@@ -1211,21 +1219,6 @@ __device__ __forceinline__ void load_tri9(Mesh m, int f, float T[3][3], int idx[
     }
 }
 
-// Changed live faces in Morton order (neighbouring lanes query neighbouring
-// regions of the tree).
-// Queries: the changed live faces among the tree's leaves, in leaf order.
-// nL >= the tree's leaf count (a carried tree can have more leaves than
-// there are faces); flag[0..nL] is written.
-__global__ void query_flag_k(const uint32_t* __restrict__ val, const int* __restrict__ cnt,
-                             const uint8_t* __restrict__ changed, const int* __restrict__ ti, int nT, int nL,
-                             int* __restrict__ flag) {
-    const int n = cnt[C_NLIVE];
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > nL) return;
-    const int f = s < n ? (int)val[s] : nT;  // a carried tree's dead leaves hold nT
-    flag[s] = (f < nT && changed[f] && ti[f] != -1) ? 1 : 0;
-}
-
 // Carries the guard tree across a compaction: leaf s keeps its place in the
 // hierarchy and takes its face's new index, or nT_new (the empty sentinel
 // box) if the face was removed; counts the leaves still live.  Every face
@@ -1247,14 +1240,6 @@ __global__ void carry_leaves_k(uint32_t* __restrict__ val, const int* __restrict
     if ((threadIdx.x & 63) == 0 && lm) atomicAdd(&nlive, (int)__popcll(lm));
     __syncthreads();
     if (threadIdx.x == 0 && nlive) atomicAdd(&cnt[C_LIVE_LEAVES], nlive);
-}
-
-__global__ void query_list_k(const uint32_t* __restrict__ val, const int* __restrict__ flag,
-                             const int* __restrict__ pos, int nT, int* __restrict__ list, int* cnt) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) cnt[C_NQUERY] = pos[nT];
-    if (s >= nT) return;
-    if (flag[s]) list[pos[s]] = (int)val[s];
 }
 
 // Wave-level ordering point for LDS traffic between lanes of one wave.
@@ -1364,9 +1349,11 @@ __global__ void __launch_bounds__(kBlock) exact_pairs_k(Mesh m, const int2* __re
 }
 
 // Stackless walk of the subtree below internal node x: from its left child
-// until x's escape, evaluating candidates in this lane.
+// until x's escape, evaluating candidates in this lane.  Inlined: as a
+// called function its register needs add to the caller's (242 registers,
+// 2 waves per SIMD, against 129 inlined).
 template <bool kExact>
-__device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restrict__ changed,
+__device__ __forceinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restrict__ changed,
                                                 const float4* __restrict__ node, int n, int x, int stop,
                                                 float4 qlo, float4 qhi, int a, int ia0, int ia1, int ia2,
                                                 float epsilon, uint8_t* __restrict__ flagged, Undecided u,
@@ -1405,14 +1392,12 @@ __device__ __noinline__ void guard_walk_subtree(Mesh m, const uint8_t* __restric
 // links (exact, only slower).  The flags are idempotent, so the visiting
 // order does not change the result.
 template <int kStackCap, bool kExact>
-__global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __restrict__ list,
-                                                        const int* __restrict__ cnt,
-                                                        const uint8_t* __restrict__ changed,
-                                                        const float4* __restrict__ fbox,
-                                                        const float4* __restrict__ node,
-                                                        const int* __restrict__ right, float epsilon,
-                                                        uint8_t* __restrict__ flagged, Undecided u, int part_log,
-                                                        int* err, long long walk_cap) {
+__device__ __forceinline__ void guard_query_body(Mesh m, const int* __restrict__ list, const int* __restrict__ cnt,
+                                                 const uint8_t* __restrict__ changed,
+                                                 const float4* __restrict__ fbox, const float4* __restrict__ node,
+                                                 const int* __restrict__ right, float epsilon,
+                                                 uint8_t* __restrict__ flagged, Undecided u, int part_log, int* err,
+                                                 long long walk_cap) {
     constexpr int kW = kBlock / kWave;
     __shared__ int2 stk[kW][kStackCap];
     __shared__ int2 pq[kW][kQueue];
@@ -1511,49 +1496,40 @@ __global__ void __launch_bounds__(kBlock) guard_query_k(Mesh m, const int* __res
     }
 }
 
+#define PAMO_GUARD_PARAMS                                                                                  \
+    Mesh m, const int* __restrict__ list, const int* __restrict__ cnt, const uint8_t* __restrict__ changed,   \
+        const float4* __restrict__ fbox, const float4* __restrict__ node, const int* __restrict__ right,       \
+        float epsilon, uint8_t* __restrict__ flagged, Undecided u, int part_log, int* err, long long walk_cap
+#define PAMO_GUARD_ARGS m, list, cnt, changed, fbox, node, right, epsilon, flagged, u, part_log, err, walk_cap
+
+template <int kStackCap, bool kExact>
+__global__ void __launch_bounds__(kBlock) guard_query_k(PAMO_GUARD_PARAMS) {
+    guard_query_body<kStackCap, kExact>(PAMO_GUARD_ARGS);
+}
+
+// The same guard with its registers capped for 4 waves per SIMD (the
+// compiler's own allocation is one register over); the results do not
+// change.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) guard_query_w4_k(PAMO_GUARD_PARAMS) {
+    guard_query_body<kGuardStack, false>(PAMO_GUARD_ARGS);
+}
+#undef PAMO_GUARD_PARAMS
+#undef PAMO_GUARD_ARGS
+
 // ---------------------------------------------------------------- undo (A14)
-__global__ void undo_cand_k(const int* __restrict__ recs, int n_rec, const int* __restrict__ off,
-                            const int* __restrict__ ring, const int* __restrict__ eu, const int* __restrict__ ev,
-                            const uint8_t* __restrict__ flagged, int* __restrict__ pflag) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r > n_rec) return;
-    if (r == n_rec) {
-        pflag[n_rec] = 0;
-        return;
-    }
-    int e = recs[r];
-    int u = eu[e], v = ev[e];
-    int hit = 0;
-    for (int a = off[u]; a < off[u + 1] && !hit; ++a) hit = flagged[ring[a]];
-    for (int a = off[v]; a < off[v + 1] && !hit; ++a) hit = flagged[ring[a]];
-    pflag[r] = hit ? 1 : 0;
-}
-
-__global__ void undo_compact_k(const int* __restrict__ recs, int n_rec, const int* __restrict__ pflag,
-                               const int* __restrict__ pos, int* __restrict__ pushed, int* cnt) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r == 0) cnt[C_PUSHED] = pos[n_rec];
-    if (r >= n_rec) return;
-    if (pflag[r]) pushed[pos[r]] = recs[r];
-}
-
+// Restores each pushed collapse and appends its two end vertices, through
+// the point map of the previous call (rearrange_undo_list_kernel,
+// cusimp_free.cu:869-878), to the undo list at `first`.
 __global__ void undo_apply_k(Mesh m, Orig o, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
                              const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ pushed,
-                             int npushed, int* __restrict__ tmp_undo) {
+                             int npushed, const int* __restrict__ pts_map, int* __restrict__ undo_list, int first) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= npushed) return;
     int e = pushed[r];
     int u = eu[e], v = ev[e];
-    tmp_undo[2 * r] = u;
-    tmp_undo[2 * r + 1] = v;
+    undo_list[first + 2 * r] = pts_map[u];
+    undo_list[first + 2 * r + 1] = pts_map[v];
     restore_edge(m, o, occ, off, ring, u, v);
-}
-
-__global__ void rearrange_k(const int* __restrict__ tmp_undo, int n, const int* __restrict__ pts_map,
-                            int* __restrict__ undo_list, int first) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    undo_list[first + i] = pts_map[tmp_undo[i]];
 }
 
 // ---------------------------------------------------------------- outputs / compaction (A15)
@@ -1572,29 +1548,6 @@ __global__ void store_tris_k(Mesh m, int n, int* __restrict__ aos) {
     aos[3 * (long long)f] = m.i[f];
     aos[3 * (long long)f + 1] = m.j[f];
     aos[3 * (long long)f + 2] = m.k[f];
-}
-
-__global__ void live_flag_k(Mesh m, int nT, int* __restrict__ flag) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f > nT) return;
-    flag[f] = (f < nT && m.i[f] >= 0) ? 1 : 0;
-}
-
-__global__ void compact_mesh_k(Mesh m, int nP, int nT, const int* __restrict__ occ,
-                               const int* __restrict__ pts_map, const int* __restrict__ fpos,
-                               float4* __restrict__ np, int* ni, int* nj, int* nk, int* cnt) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
-        cnt[C_NV] = pts_map[nP];
-        cnt[C_NF] = fpos[nT];
-    }
-    if (t < nP && occ[t]) np[pts_map[t]] = m.p[t];
-    if (t < nT && m.i[t] >= 0) {
-        int d = fpos[t];
-        ni[d] = pts_map[m.i[t]];
-        nj[d] = pts_map[m.j[t]];
-        nk[d] = pts_map[m.k[t]];
-    }
 }
 
 // Centring offset of PaMO.run (pamo/__init__.py:60-72,144): numpy's
@@ -1689,6 +1642,135 @@ __global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __re
     }
     if (t < 3) out3[t] = (float)((double)acc / (double)nF);
 }
+
+// ---------------------------------------------------------------- fused compactions
+// Operations for scan_op (prim.h): each computes an item's flag, receives
+// the item's position among the flagged ones from the same launch's
+// look-back scan, and writes the compacted list and its count, where the
+// round used a flag kernel, a scan and a compaction kernel.
+
+// Guard queries: the changed live faces among the tree's leaves, in leaf
+// order (a carried tree's dead leaves hold the sentinel nT).
+struct QueryListOp {
+    static constexpr int kIpt = 8;
+    const uint32_t* val;
+    const uint8_t* changed;
+    const int* ti;
+    int nT;
+    int* list;
+    int* cnt;
+    __device__ int count(long long s) const {
+        const int f = s < cnt[C_NLIVE] ? (int)val[s] : nT;
+        return (f < nT && changed[f] && ti[f] != -1) ? 1 : 0;
+    }
+    __device__ void emit(long long s, int v, int pos) const {
+        if (v) list[pos] = (int)val[s];
+    }
+    __device__ void done(int total) const { cnt[C_NQUERY] = total; }
+};
+
+// Regular collapse records in edge order (select_collapse_k's flags).
+struct CollapseListOp {
+    static constexpr int kIpt = 8;
+    const int* flag;
+    int* recs;
+    int* cnt;
+    __device__ int count(long long e) const { return flag[e]; }
+    __device__ void emit(long long e, int v, int pos) const {
+        if (v) recs[pos] = (int)e;
+    }
+    __device__ void done(int total) const { cnt[C_NREG] = total; }
+};
+
+// Cleanup (cusimp_free.cu:218-274): item t removes face t if two of its
+// corners coincide (t < nT) and flags edge t if its ends coincide (t < nE);
+// the line collapses go, in edge order, after the regular collapses in the
+// records.  Items cover max(nE, nT); the two parts touch disjoint data.
+struct CleanupOp {
+    static constexpr int kIpt = 8;
+    Mesh m;
+    const int* eu;
+    const int* ev;
+    int nE, nT;
+    int* recs;
+    int* cnt;
+    __device__ int count(long long t) const {
+        if (t < nT) {
+            const int a = m.i[t], b = m.j[t], c = m.k[t];
+            if (!face_dead(a, b, c) && (a == b || b == c || c == a)) {
+                m.i[t] = -1;
+                m.j[t] = -1;
+                m.k[t] = -1;
+            }
+        }
+        if (t >= nE) return 0;
+        const F3 p = ld(m, eu[t]), q = ld(m, ev[t]);
+        return (p.x == q.x && p.y == q.y && p.z == q.z) ? 1 : 0;
+    }
+    __device__ void emit(long long e, int v, int pos) const {
+        if (v) recs[cnt[C_NREG] + pos] = (int)e;
+    }
+    __device__ void done(int total) const { cnt[C_LINE] = total; }
+};
+
+// Undo candidates (get_undo_candidate_kernel, cusimp_free.cu:776-827): the
+// records one of whose end vertices touches a flagged face.  Two records
+// per thread: each walks two vertex rings.
+struct UndoListOp {
+    static constexpr int kIpt = 2;
+    const int* recs;
+    const int* off;
+    const int* ring;
+    const int* eu;
+    const int* ev;
+    const uint8_t* flagged;
+    int* pushed;
+    int* cnt;
+    __device__ int count(long long r) const {
+        const int e = recs[r];
+        const int u = eu[e], v = ev[e];
+        int hit = 0;
+        for (int a = off[u]; a < off[u + 1] && !hit; ++a) hit = flagged[ring[a]];
+        for (int a = off[v]; a < off[v + 1] && !hit; ++a) hit = flagged[ring[a]];
+        return hit ? 1 : 0;
+    }
+    __device__ void emit(long long r, int v, int pos) const {
+        if (v) pushed[pos] = recs[r];
+    }
+    __device__ void done(int total) const { cnt[C_PUSHED] = total; }
+};
+
+// Compaction of the mesh (pamo/__init__.py:163-167): item t moves point t
+// to pts_map[t] if it is occupied, and face t to its position among the
+// live faces with renumbered corners; fpos[t] receives that position for
+// t <= nT (fpos[nT] = live faces), which the guard-tree carry reads.  Items
+// cover max(nP, nT) + 1.
+struct CompactMeshOp {
+    static constexpr int kIpt = 8;
+    Mesh m;
+    int nP, nT;
+    const int* occ;
+    const int* pts_map;
+    int* fpos;
+    float4* np;
+    int* ni;
+    int* nj;
+    int* nk;
+    int* cnt;
+    __device__ int count(long long t) const { return (t < nT && m.i[t] >= 0) ? 1 : 0; }
+    __device__ void emit(long long t, int v, int pos) const {
+        if (t == 0) cnt[C_LIVE_LEAVES] = 0;  // counted by carry_leaves_k next
+        if (t < nP && occ[t]) np[pts_map[t]] = m.p[t];
+        if (t == nP) cnt[C_NV] = pts_map[nP];
+        if (t <= nT) fpos[t] = pos;
+        if (v) {
+            ni[pos] = pts_map[m.i[t]];
+            nj[pos] = pts_map[m.j[t]];
+            nk[pos] = pts_map[m.k[t]];
+        }
+    }
+    __device__ void done(int total) const { cnt[C_NF] = total; }
+};
 
 }  // namespace
 }  // namespace pamo

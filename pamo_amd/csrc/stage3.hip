@@ -111,6 +111,9 @@ struct Ctx {
     long long cap_ws = 0;
     // statistics
     long long stat_contacts = 0, stat_newton = 0, stat_cg = 0, stat_ls = 0;
+    // the CG loop replayed from a captured graph (updated each solve)
+    int cg_graph = 1;
+    hipGraphExec_t cg_exec = nullptr;
     double last_energy = 0.0;
 };
 
@@ -128,6 +131,8 @@ static void ctx_free(Ctx& c) {
         if (q) (void)hipFree(q);
     if (c.hcount) (void)hipHostFree(c.hcount);
     if (c.hsc) (void)hipHostFree(c.hsc);
+    if (c.cg_exec) (void)hipGraphExecDestroy(c.cg_exec);
+    c.cg_exec = nullptr;
     tree_free(c.gt_tree);
     tree_free(c.tri_tree);
     tree_free(c.edge_tree);
@@ -334,7 +339,7 @@ __device__ __forceinline__ bool det_pair(const float4* __restrict__ x, const int
 // Stackless walk below internal node x (from its left child until x's
 // escape), classifying and appending in this lane.
 template <bool kPT>
-__device__ __noinline__ void det_walk_subtree(const float4* __restrict__ x, const int4* __restrict__ tri,
+__device__ __forceinline__ void det_walk_subtree(const float4* __restrict__ x, const int4* __restrict__ tri,
                                               const int2* __restrict__ edge, const float4* __restrict__ node,
                                               int n_leaves, int cur, int stop, V3 lo, V3 hi, int q, float thr,
                                               float ee_thres, int* counter, int cap, int4* __restrict__ cidx,
@@ -1450,7 +1455,7 @@ static int update_target(Ctx& c) {
 
 // CGSolver.solve (cg_solver.py:96-158): Jacobi PCG with n_cg fixed
 // iterations, p = H^-1 grad; three launches per iteration.
-static int cg_solve(Ctx& c) {
+static int cg_record(Ctx& c) {
     const int nV = c.nV, mask = c.P.mask;
     const int np = (int)grid_for(nV, kB);
     L3(cg_init_k, nV, (const float4*)c.grad, (const float4*)c.diag, nV, c.r, c.z, c.p, c.zr_part);
@@ -1464,6 +1469,39 @@ static int cg_solve(Ctx& c) {
         L3(cg_update_k, nV, (const float4*)c.v, (const float4*)c.Av, (const float4*)c.diag, (const double*)c.vav_part,
            (int)grid_for((long long)nV * kGather, kB), (const double*)zr_cur, np, nV, c.p, c.r, c.z, zr_oth);
     }
+    return 0;
+}
+
+// The reference captures its PCG loop in a CUDA graph (cg_solver.py:150-156,
+// config.py:21).  Here the loop's 3 n_cg + 1 kernels are captured from the
+// stream each solve and the executable graph is updated in place (the
+// active-contact count, hence one grid size, changes between Newton
+// iterations), then launched as one unit.  No memset or copy is captured.
+static int cg_solve(Ctx& c) {
+    if (!c.cg_graph) return cg_record(c);
+    hipGraph_t g = nullptr;
+    PAMO_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = cg_record(c);
+    const hipError_t ec = hipStreamEndCapture(c.stream, &g);
+    if (rc != 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    PAMO_HIP(ec);
+    if (c.cg_exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+        if (hipGraphExecUpdate(c.cg_exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();  // not updatable (shape changed): instantiate anew
+            (void)hipGraphExecDestroy(c.cg_exec);
+            c.cg_exec = nullptr;
+        }
+    }
+    hipError_t ei = hipSuccess;
+    if (!c.cg_exec) ei = hipGraphInstantiate(&c.cg_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PAMO_HIP(ei);
+    PAMO_HIP(hipGraphLaunch(c.cg_exec, c.stream));
     return 0;
 }
 
@@ -1999,6 +2037,14 @@ int pamo_s3_debug(pamo_s3* s, int what, const double* h_in, long long n_in, doub
         *n_out = 6;
         return 0;
     }
+    case PAMO_S3_DBG_RESOLVE: {
+        PAMO_TRY(check_ready(c, "pamo_s3_debug"));
+        PAMO_CHECK(need_out(n3), "pamo_s3_debug: output too small");
+        PAMO_TRY(cg_solve(c));
+        PAMO_TRY(read_vec(c, c.p, h_out));
+        *n_out = n3;
+        return 0;
+    }
     case PAMO_S3_DBG_DIFF:
     case PAMO_S3_DBG_HVP:
     case PAMO_S3_DBG_SOLVE:
@@ -2116,6 +2162,18 @@ int pamo_s3_set_mesh_device(pamo_s3* s, const float* d_v, int n_v, const int* d_
     const int do_nothing = hp.min_angle_deg <= c.min_angle_thres ? 1 : 0;
     return pamo_s3_set_mesh(s, d_v, n_v, d_f, n_f, scale, t3, hp.edges.data(), n_e,
                             n_h ? hp.hinges.data() : nullptr, n_h, hp.vor.data(), do_nothing);
+}
+
+int pamo_s3_set_option(pamo_s3* s, int option, long long value) {
+    pamo::clear_error();
+    PAMO_CHECK(s, "pamo_s3_set_option: null argument");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    if (option == PAMO_S3_OPT_CG_GRAPH) {
+        PAMO_CHECK(value == 0 || value == 1, "pamo_s3_set_option: value must be 0 or 1");
+        c.cg_graph = (int)value;
+        return 0;
+    }
+    PAMO_CHECK(false, "pamo_s3_set_option: unknown option");
 }
 
 int pamo_s3_stats(pamo_s3* s, double* out8) {

@@ -308,6 +308,12 @@ class Stage3System:
         return {"energy": o[0], "contacts": int(o[1]), "newton": int(o[2]), "cg": int(o[3]), "line_search": int(o[4]),
                 "radius": o[5], "do_nothing": bool(o[6]), "n_hinges": int(o[7])}
 
+    def set_cg_graph(self, on: bool):
+        """Replay the PCG loop from a captured HIP graph (default) or launch
+        its kernels one by one; results do not change."""
+        _hip.check(_hip.lib().pamo_s3_set_option(self.ptr, _hip.PAMO_S3_OPT_CG_GRAPH, 1 if on else 0),
+                   "pamo_s3_set_option")
+
     # -- test access (no counterpart in the reference's API)
     def set_dense(self, A, b, c0):
         A = np.ascontiguousarray(A, dtype=np.float64)

@@ -382,8 +382,9 @@ def test_tree_carry_matches_rebuild_every_round(torch_cuda, oracle_lib, case):
 
 def test_autotuned_settings_give_the_same_result(torch_cuda, oracle_lib):
     """The autotuner's knobs (guard-tree keep percentage, guard subtree
-    split) change the launch shape only: every candidate gives the oracle's
-    result bit for bit, and the tuner returns one of them."""
+    split, guard register budget) change the launch shape only: every
+    candidate gives the oracle's result bit for bit, and the tuner returns
+    one of them."""
     from pamo_amd import synth, _hip, autotune
     from pamo_amd.pamo import simplify_device
     torch = torch_cuda
@@ -393,7 +394,8 @@ def test_autotuned_settings_give_the_same_result(torch_cuda, oracle_lib):
     print(f"case: autotune knobs, icosphere L6, {len(F)} faces", flush=True)
     vo, fo, ho = oracle_lib.run(Vc, F, 0.02, min_verts=0)
     for settings in ({"guard_split_log": 0}, {"guard_split_log": 2}, {"guard_split_log": 4},
-                     {"guard_split_faces": 0}, {"tree_keep": 30, "guard_split_faces": 1 << 20}):
+                     {"guard_split_faces": 0}, {"tree_keep": 30, "guard_split_faces": 1 << 20},
+                     {"guard_waves": 4}, {"guard_waves": 4, "guard_split_log": 0}):
         ctx = _hip.Context(0)
         autotune.apply(ctx, settings)
         vg, fg, hg = simplify_device(ctx, Vd, Fd, 0.02, min_verts=0)

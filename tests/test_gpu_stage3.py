@@ -137,6 +137,26 @@ def test_pcg_direction_and_ccd(kind):
 
 
 @pytest.mark.parametrize("kind", ["plain", "sheets"])
+def test_cg_graph_replays_match_eager_launches(kind):
+    """The PCG loop replayed from its captured HIP graph (cg_solver.py:150-156
+    captures the reference's) gives the direction of the kernels launched one
+    by one, bit for bit, on several replays with eager solves and host reads
+    in between."""
+    s = scene(kind)
+    print(f"[stage3] cg graph {kind}", flush=True)
+    g = gpu_system(s)
+    g.set_cg_graph(False)
+    p_eager = g.debug(_hip.PAMO_S3_DBG_SOLVE)  # detection, gradient, eager PCG
+    assert np.isfinite(p_eager).all() and np.abs(p_eager).max() > 0
+    for rep in range(3):
+        g.set_cg_graph(True)
+        p_graph = g.debug(_hip.PAMO_S3_DBG_RESOLVE)  # same gradient, graph replay
+        assert np.array_equal(p_graph, p_eager), rep
+        g.set_cg_graph(False)
+        assert np.array_equal(g.debug(_hip.PAMO_S3_DBG_RESOLVE), p_eager), rep
+
+
+@pytest.mark.parametrize("kind", ["plain", "sheets"])
 def test_one_step_matches_oracle(kind):
     s = scene(kind)
     print(f"[stage3] step {kind}", flush=True)
