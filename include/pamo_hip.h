@@ -76,6 +76,10 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * registers, 4 runs a variant capped at 128 vector registers so that 4 waves
  * fit per SIMD. */
 #define PAMO_OPT_GUARD_WAVES 8
+/* Edge-cost launch shape (no counterpart in the reference; results do not
+ * change): 1 (default) lane per edge, or 4 / 8 lanes sharing one edge's
+ * ring faces and adding their terms in the single-lane order. */
+#define PAMO_OPT_EDGE_LANES 9
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).
@@ -296,9 +300,9 @@ int pamo_s3_stats(pamo_s3* s3, double* out8);
 
 /* Run-time options (no counterpart in the reference's interface, which
  * fixes its CUDA-graph capture of the PCG loop in config.py:21).
- * PAMO_S3_OPT_CG_GRAPH: 1 (default) captures each solve's PCG kernels from
- * the stream into a HIP graph, updated in place between solves, and
- * launches it as one unit; 0 launches them one by one.  Results do not
+ * PAMO_S3_OPT_CG_GRAPH: 1 captures each solve's PCG kernels from the
+ * stream into a HIP graph, updated in place between solves, and launches it
+ * as one unit; 0 (default) launches them one by one.  Results do not
  * change. */
 #define PAMO_S3_OPT_CG_GRAPH 1
 int pamo_s3_set_option(pamo_s3* s3, int option, long long value);

@@ -104,7 +104,8 @@ struct Ctx {
     int *occ = nullptr, *pts_map = nullptr, *ring_off = nullptr, *ring_fill = nullptr;
     int* eord_cnt = nullptr;  // edges per smaller endpoint, then the launch-order cursor
     int *undo_list = nullptr, *tmp_undo = nullptr, *undo_prev = nullptr;  // 4 * alloc_pts
-    uint8_t* invalid = nullptr;
+    uint32_t* invalid = nullptr;  // epochs (invalid_set_k)
+    uint32_t inv_epoch = 1;       // the invalid table's current epoch
     uint64_t* vmin = nullptr;  // per-vertex minimum edge key
     // face sized
     int *ti = nullptr, *tj = nullptr, *tk = nullptr, *oi = nullptr, *oj = nullptr, *ok = nullptr;
@@ -152,6 +153,7 @@ struct Ctx {
     int guard_split_faces = 1 << 18;
     int guard_split_log = 3;
     int guard_waves = 0;        // 0: the compiler's register allocation for the guard; 4: capped for 4 waves
+    int edge_lanes = 1;         // lanes per edge in the edge-cost kernel (1, 4 or 8)
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -240,7 +242,7 @@ static int grow_pts(Ctx& c, size_t n) {
     PAMO_TRY(dalloc(&c.undo_list, 4 * a, c.stream));
     PAMO_TRY(dalloc(&c.tmp_undo, 4 * a, c.stream));
     PAMO_TRY(dalloc(&c.undo_prev, 4 * a, c.stream));
-    PAMO_TRY(dalloc(&c.invalid, a, c.stream));
+    PAMO_TRY(dalloc(&c.invalid, a, c.stream));  // epochs: zero is never current
     PAMO_TRY(dalloc(&c.vmin, a, c.stream));
     PAMO_TRY(ensure_parts(c, a));
     return ensure_ws(c, (long long)a + 1);
@@ -580,14 +582,22 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
            c.eord_cnt, (unsigned long long*)c.vmin, c.changed);
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
-    CostParams cp{edge_s, tres, is_stuck ? 1 : 0};
+    CostParams cp{edge_s, tres, is_stuck ? 1 : 0, c.inv_epoch};
     const double cost_bytes = 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE;
-    if (c.placement)
-        LAUNCHP(K_EDGE_COST, cost_bytes, edge_cost_k<true>, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE, c.invalid,
-                c.Q, cp, c.edge_cost);
-    else
-        LAUNCHP(K_EDGE_COST, cost_bytes, edge_cost_k<false>, nE, m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE,
-                c.invalid, c.Q, cp, c.edge_cost);
+#define PAMO_EDGE_COST(K, lanes)                                                                              \
+    LAUNCHP(K_EDGE_COST, cost_bytes, K, (long long)nE * (lanes), m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE,     \
+            c.invalid, c.Q, cp, c.edge_cost)
+    if (c.edge_lanes == 8) {
+        if (c.placement) PAMO_EDGE_COST((edge_cost_group_k<true, 8>), 8);
+        else PAMO_EDGE_COST((edge_cost_group_k<false, 8>), 8);
+    } else if (c.edge_lanes == 4) {
+        if (c.placement) PAMO_EDGE_COST((edge_cost_group_k<true, 4>), 4);
+        else PAMO_EDGE_COST((edge_cost_group_k<false, 4>), 4);
+    } else {
+        if (c.placement) PAMO_EDGE_COST(edge_cost_k<true>, 1);
+        else PAMO_EDGE_COST(edge_cost_k<false>, 1);
+    }
+#undef PAMO_EDGE_COST
     // independent set + collapse
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
             (unsigned long long*)c.vmin);
@@ -615,15 +625,24 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     // flags.  A tree carried from the last round is refit, otherwise built.
     PAMO_TRY(run_guard(c, nT, c.tree_valid && c.tree_fresh));
     c.tree_valid = true;
+    // the first undo candidates follow the guard without a host read: their
+    // count comes from the device counters, over a launch bounded by the
+    // edge count and by half the points plus 512 (regular collapses share
+    // no vertex)
+    const long long cand_bound = std::min<long long>(nE, nP / 2 + 512);
+    PAMO_TRY(scan_op(UndoListOp{c.recs, c.ring_off, c.ring, c.eu, c.ev, c.flagged, c.pushed, c.dcnt}, cand_bound,
+                     c.lb, c.stream, K_UNDO_CAND, 16.0 * cand_bound + 1.0 * nT));
     PAMO_TRY(sync_counters(c));
     c.n_regular = c.hcnt[C_NREG];
     c.n_rec = c.n_regular + c.hcnt[C_LINE];
     c.n_cand = std::min(c.n_rec, (c.n_regular + 511) / 512 * 512);
     c.tree_n = c.hcnt[C_NLIVE];
+    PAMO_CHECK(c.n_cand <= cand_bound, "undo candidates beyond their launch bound");
+    PAMO_CHECK(!c.exact_guard || c.hcnt[C_UNDECIDED] <= (long long)c.alloc_ulist,
+               "exact guard: more pairs left undecided than the list holds");
     int first = c.n_vertices_undo;  // entries already placed by the prologue
-    int npushed = 0;
+    int npushed = c.hcnt[C_PUSHED];
     c.undo_iters = 0;
-    if (c.n_cand > 0) PAMO_TRY(undo_candidates(c, npushed));
     c.n_vertices_undo += 2 * npushed;
     int it = 0;
     while (npushed != 0) {
@@ -651,9 +670,15 @@ static int prologue(Ctx& c, const int* undo_in, int n_undo_in, bool is_stuck) {
                    (int)c.alloc_pts, c.dcnt + C_BAD_UNDO);
         c.n_vertices_undo += n_undo_in;
     } else if (n_undo_in != 0) {
-        PAMO_HIP(hipMemsetAsync(c.invalid, 0, c.alloc_pts, c.stream));
+        // a new epoch clears the table
+        if (c.inv_epoch == 0xFFFFFFFFu) {
+            PAMO_HIP(hipMemsetAsync(c.invalid, 0, sizeof(uint32_t) * c.alloc_pts, c.stream));
+            c.inv_epoch = 0;
+        }
+        ++c.inv_epoch;
     }
-    if (n_undo_in != 0) LAUNCH(invalid_set_k, n_undo_in, undo_in, n_undo_in, c.invalid, (int)c.alloc_pts);
+    if (n_undo_in != 0)
+        LAUNCH(invalid_set_k, n_undo_in, undo_in, n_undo_in, c.invalid, (int)c.alloc_pts, c.inv_epoch);
     return 0;
 }
 
@@ -780,6 +805,11 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     if (option == PAMO_OPT_GUARD_SPLIT_LOG) {
         PAMO_CHECK(value >= 0 && value <= 4, "pamo_ctx_set_option: split depth must be in [0, 4]");
         c.guard_split_log = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_EDGE_LANES) {
+        PAMO_CHECK(value == 1 || value == 4 || value == 8, "pamo_ctx_set_option: edge lanes must be 1, 4 or 8");
+        c.edge_lanes = (int)value;
         return 0;
     }
     if (option == PAMO_OPT_GUARD_WAVES) {
@@ -1051,9 +1081,7 @@ static int simplify_loop(Ctx& c, const float* d_verts, int n_verts, const int* d
         PAMO_TRY(pamo::round_device(c, nV, nF, scale, threshold, is_stuck));
         ++rounds;
         n_undo = c.n_vertices_undo;
-        if (n_undo > 0)
-            PAMO_HIP(hipMemcpyAsync(c.undo_prev, c.undo_list, sizeof(int) * n_undo, hipMemcpyDeviceToDevice,
-                                    c.stream));
+        std::swap(c.undo_prev, c.undo_list);  // this round's list feeds the next prologue
         PAMO_TRY(pamo::compact_round(c, nV, nF, true));
         nV = c.hcnt[pamo::C_NV];
         nF = c.hcnt[pamo::C_NF];

@@ -152,11 +152,14 @@ __global__ void stuck_prologue_k(const int* __restrict__ undo_in, int n, int* tm
     undo_list[i] = pts_map[v];
 }
 
-__global__ void invalid_set_k(const int* __restrict__ undo_in, int n, uint8_t* invalid, int cap) {
+// The invalid table holds epochs: vertex v is invalid when invalid[v] equals
+// the current epoch.  Starting a new epoch clears the table (the
+// reference's memset, cusimp_free.cu:905-908) without touching it.
+__global__ void invalid_set_k(const int* __restrict__ undo_in, int n, uint32_t* invalid, int cap, uint32_t epoch) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int v = undo_in[i];
-    if (v >= 0 && v < cap) invalid[v] = 1;
+    if (v >= 0 && v < cap) invalid[v] = epoch;
 }
 
 // ---------------------------------------------------------------- rings and edges (A4, A5)
@@ -315,6 +318,7 @@ __device__ __forceinline__ int link_dup(const int4* __restrict__ rv, int u, int 
 struct CostParams {
     float edge_s, tres;
     int is_stuck;
+    uint32_t inv_epoch;  // the invalid table's current epoch
 };
 
 // Opt-in optimal placement (the reference always collapses to the midpoint,
@@ -379,14 +383,14 @@ template <bool kOptimal>
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
                                                       const int4* __restrict__ rv, const int* __restrict__ eu,
                                                       const int* __restrict__ ev, const int* __restrict__ order,
-                                                      int nE, const uint8_t* __restrict__ invalid,
+                                                      int nE, const uint32_t* __restrict__ invalid,
                                                       const float4* __restrict__ Q, CostParams cp,
                                                       uint32_t* __restrict__ cost) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nE) return;
     const int e = order[t];
     const int u = eu[e], v = ev[e];
-    if (invalid[u] && invalid[v]) {
+    if (invalid[u] == cp.inv_epoch && invalid[v] == cp.inv_epoch) {
         cost[e] = 0xFFFFFFFFu;
         return;
     }
@@ -436,6 +440,137 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
     qa = 5.0f * qa;
     // M = Q_u + Q_v; storage 0:00 1:01 2:02 3:03 4:11 5:12 6:13 7:22 8:23 9:33
+    const float M0 = M[0], M1 = M[1], M2 = M[2], M3 = M[3], M4 = M[4], M5 = M[5], M6 = M[6], M7 = M[7];
+    const float M8 = M[8], M9 = M[9];
+    const float x0 = vm.x, x1 = vm.y, x2 = vm.z, x3 = 1.0f;
+    float r0 = M0 * x0 + M1 * x1 + M2 * x2 + M3 * x3;
+    float r1 = M1 * x0 + M4 * x1 + M5 * x2 + M6 * x3;
+    float r2 = M2 * x0 + M5 * x1 + M7 * x2 + M8 * x3;
+    float r3 = M3 * x0 + M6 * x1 + M8 * x2 + M9 * x3;
+    float qc = (r0 * x0 + r1 * x1 + r2 * x2 + r3 * x3) / (cp.edge_s * cp.edge_s);
+    float tot = qc + elen + qa / (float)ntri * cp.tres;
+    cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+}
+
+// link_dup's count over the outer-ring faces g, g + kG, ... (one lane's
+// share of a kG-lane group); the shares add up to link_dup's count, and a
+// share above 2 is reported as 3.
+template <int kG>
+__device__ __forceinline__ int link_dup_share(const int4* __restrict__ rv, int u, int ub, int ue, int v, int vb,
+                                              int ve, int g) {
+    const bool swap = (ve - vb) > (ue - ub);
+    const int ib = swap ? ub : vb, ie = swap ? ue : ve, iw = swap ? u : v;
+    const int ob = swap ? vb : ub, oe = swap ? ve : ue, ow = swap ? v : u;
+    int dup = 0;
+    for (int c0 = ib; c0 < ie; c0 += kLinkRing) {
+        int sv[kLinkRing];
+#pragma unroll
+        for (int b = 0; b < kLinkRing; ++b) sv[b] = c0 + b < ie ? succ_in(rv[c0 + b], iw) : -2;
+        for (int a = ob + g; a < oe; a += kG) {
+            const int su = succ_in(rv[a], ow);
+#pragma unroll
+            for (int b = 0; b < kLinkRing; ++b) dup += (su == sv[b]) ? 1 : 0;
+        }
+        if (dup > 2) return 3;
+    }
+    return dup;
+}
+
+// edge_cost_k with kG lanes per edge (kG divides the wave).  The link count
+// is split over the lanes (an integer sum); each lane then evaluates every
+// kG-th face of the two rings (positions, lengths, normals, flip test,
+// quality term) and all lanes of the group add the terms up face by face
+// in the single-lane order, taking each face's terms from its lane by
+// shuffles, so every float sum is formed in the same order as in
+// edge_cost_k and the costs are equal bit for bit.  A flip anywhere sets
+// the maximum cost, as the single-lane loop's first flip does.  The ring
+// loads of one edge then take about 1/kG of the dependent round trips.
+template <bool kOptimal, int kG>
+__global__ void __launch_bounds__(kBlock) edge_cost_group_k(Mesh m, const int* __restrict__ off,
+                                                            const int4* __restrict__ rv, const int* __restrict__ eu,
+                                                            const int* __restrict__ ev, const int* __restrict__ order,
+                                                            int nE, const uint32_t* __restrict__ invalid,
+                                                            const float4* __restrict__ Q, CostParams cp,
+                                                            uint32_t* __restrict__ cost) {
+    static_assert(kWave % kG == 0, "a group must not straddle waves");
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(threadIdx.x & (kG - 1));
+    const long long t = gt / kG;
+    if (t >= nE) return;  // the whole group
+    const int e = order[t];
+    const int u = eu[e], v = ev[e];
+    if (invalid[u] == cp.inv_epoch && invalid[v] == cp.inv_epoch) {
+        if (g == 0) cost[e] = 0xFFFFFFFFu;
+        return;
+    }
+    if (cp.is_stuck) return;
+    const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
+    int dup = link_dup_share<kG>(rv, u, ub, ue, v, vb, ve, g);
+#pragma unroll
+    for (int d = kG / 2; d > 0; d >>= 1) dup += __shfl_xor(dup, d, kG);
+    if (dup > 2) {
+        if (g == 0) cost[e] = 0xFFFFFFFFu;
+        return;
+    }
+    if (dup != 2) return;
+    const F3 p0 = ld(m, u), p1 = ld(m, v);
+    float M[10];
+    edge_quadric(Q, u, v, M);
+    const F3 vm = kOptimal ? place_optimal(M, p0, p1) : f3divs(f3add(p0, p1), 2.0f);
+    const float k4s3 = 4.0f * sqrtf(3.0f);
+    const int nu = ue - ub, nf = nu + (ve - vb);
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long gmask = ((kG == 64) ? ~0ull : ((1ull << kG) - 1)) << (lane & ~(kG - 1));
+    float elen = 0.0f, qa = 0.0f;
+    int nedge = 0, ntri = 0;
+    for (int r0 = 0; r0 < nf; r0 += kG) {  // the same trip count in every lane of the group
+        const int i = r0 + g;
+        float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, qt = 0.0f;
+        int bits = 0;  // 1, 2, 4: edge-length terms present; 8: quality term present
+        bool flip = false;
+        if (i < nf) {
+            const bool side = i >= nu;
+            const int w = side ? v : u, other = side ? u : v;
+            const int4 tt = rv[side ? vb + (i - nu) : ub + i];
+            const F3 pi = ld(m, tt.x), pj = ld(m, tt.y), pk = ld(m, tt.z);
+            if (tt.x > tt.y) { l0 = f3norm(f3sub(pi, pj)); bits |= 1; }
+            if (tt.y > tt.z) { l1 = f3norm(f3sub(pj, pk)); bits |= 2; }
+            if (tt.z > tt.x) { l2 = f3norm(f3sub(pk, pi)); bits |= 4; }
+            if (!(tt.x == other || tt.y == other || tt.z == other)) {
+                const F3 n0 = tt.x == w ? vm : pi, n1 = tt.y == w ? vm : pj, n2 = tt.z == w ? vm : pk;
+                F3 on = f3cross(f3sub(pj, pi), f3sub(pk, pi));
+                on = f3divs(on, f3norm(on));
+                F3 nn = f3cross(f3sub(n1, n0), f3sub(n2, n0));
+                nn = f3divs(nn, f3norm(nn));
+                if (f3dot(on, nn) < 0) {
+                    flip = true;
+                } else {
+                    float l01 = f3norm(f3sub(n0, n1)), l12 = f3norm(f3sub(n1, n2)), l20 = f3norm(f3sub(n2, n0));
+                    float qv = k4s3 * heron(n0, n1, n2) / (l01 * l01) + l12 * l12 + l20 * l20 + 0.0000001f;
+                    qt = 1.0f - tclamp(qv, 0.0f, 1.0f);
+                    bits |= 8;
+                }
+            }
+        }
+        if (__ballot(flip) & gmask) {
+            if (g == 0) cost[e] = 0xFFFFFFFFu;
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            const int bj = __shfl(bits, j, kG);
+            const float a0 = __shfl(l0, j, kG), a1 = __shfl(l1, j, kG), a2 = __shfl(l2, j, kG);
+            const float qj = __shfl(qt, j, kG);
+            if (bj & 1) { elen += a0; ++nedge; }
+            if (bj & 2) { elen += a1; ++nedge; }
+            if (bj & 4) { elen += a2; ++nedge; }
+            if (bj & 8) { qa += qj; ++ntri; }
+        }
+    }
+    if (g != 0) return;
+    elen = elen / (float)nedge / cp.edge_s * cp.tres;
+    elen = elen + f3norm(f3sub(p0, p1)) / cp.edge_s * cp.tres;
+    qa = 5.0f * qa;
     const float M0 = M[0], M1 = M[1], M2 = M[2], M3 = M[3], M4 = M[4], M5 = M[5], M6 = M[6], M7 = M[7];
     const float M8 = M[8], M9 = M[9];
     const float x0 = vm.x, x1 = vm.y, x2 = vm.z, x3 = 1.0f;
@@ -1714,8 +1849,10 @@ struct CleanupOp {
 };
 
 // Undo candidates (get_undo_candidate_kernel, cusimp_free.cu:776-827): the
-// records one of whose end vertices touches a flagged face.  Two records
-// per thread: each walks two vertex rings.
+// records one of whose end vertices touches a flagged face, among the
+// first min(records, regular collapses rounded up to 512) records, a count
+// taken from the device counters.  Two records per thread: each walks two
+// vertex rings.
 struct UndoListOp {
     static constexpr int kIpt = 2;
     const int* recs;
@@ -1727,6 +1864,10 @@ struct UndoListOp {
     int* pushed;
     int* cnt;
     __device__ int count(long long r) const {
+        // candidates: the first min(records, regular rounded up to 512)
+        // records (the reference's grid); the launch covers a bound
+        const int nreg = cnt[C_NREG], nrec = nreg + cnt[C_LINE];
+        if (r >= min(nrec, (nreg + 511) / 512 * 512)) return 0;
         const int e = recs[r];
         const int u = eu[e], v = ev[e];
         int hit = 0;

@@ -348,6 +348,14 @@ def test_optimal_placement_matches_oracle(torch_cuda, oracle_lib, level):
     assert synth.is_watertight_manifold(fo)
     vm, fm, hm = oracle_lib.run(Vc, F, 0.05, min_verts=0)
     assert not (len(fm) == len(fo) and np.array_equal(vm, vo))  # the option changes the result
+    # the 8-lanes-per-edge cost kernel with the solve gives the same result
+    ctx8 = _hip.Context(0)
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx8.ptr, _hip.PAMO_OPT_PLACEMENT, 1), "set_option")
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx8.ptr, _hip.PAMO_OPT_EDGE_LANES, 8), "set_option")
+    v8, f8, h8 = simplify_device(ctx8, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.05, min_verts=0)
+    assert np.array_equal(h8, ho)
+    assert np.array_equal(f8.cpu().numpy(), fo)
+    assert np.array_equal(v8.cpu().numpy().view(np.uint32), vo.view(np.uint32))
 
 
 @pytest.mark.parametrize("case", ["icosphere_l6", "sheets_l5"])
@@ -395,7 +403,8 @@ def test_autotuned_settings_give_the_same_result(torch_cuda, oracle_lib):
     vo, fo, ho = oracle_lib.run(Vc, F, 0.02, min_verts=0)
     for settings in ({"guard_split_log": 0}, {"guard_split_log": 2}, {"guard_split_log": 4},
                      {"guard_split_faces": 0}, {"tree_keep": 30, "guard_split_faces": 1 << 20},
-                     {"guard_waves": 4}, {"guard_waves": 4, "guard_split_log": 0}):
+                     {"guard_waves": 4}, {"guard_waves": 4, "guard_split_log": 0},
+                     {"edge_lanes": 4}, {"edge_lanes": 8, "tree_keep": 100}):
         ctx = _hip.Context(0)
         autotune.apply(ctx, settings)
         vg, fg, hg = simplify_device(ctx, Vd, Fd, 0.02, min_verts=0)
