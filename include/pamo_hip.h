@@ -80,6 +80,11 @@ int pamo_ctx_reset(pamo_ctx* ctx);
  * change): 1 (default) lane per edge, or 4 / 8 lanes sharing one edge's
  * ring faces and adding their terms in the single-lane order. */
 #define PAMO_OPT_EDGE_LANES 9
+/* Ring-walk launch shape (no counterpart in the reference; results do not
+ * change): 1 (default) lane per vertex / edge / record in the quadric,
+ * selection-and-collapse and undo kernels, or 4 / 8 lanes splitting the
+ * one-ring faces (float sums kept in the single-lane order). */
+#define PAMO_OPT_RING_LANES 10
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 
 /* Last error message of the calling thread ("" if none).

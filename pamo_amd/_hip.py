@@ -22,6 +22,7 @@ PAMO_OPT_GUARD_SPLIT_FACES = 6
 PAMO_OPT_GUARD_SPLIT_LOG = 7
 PAMO_OPT_GUARD_WAVES = 8
 PAMO_OPT_EDGE_LANES = 9
+PAMO_OPT_RING_LANES = 10
 PAMO_DBG_EDGES = 1
 PAMO_DBG_EDGE_COST = 2
 PAMO_DBG_RECORDS = 3

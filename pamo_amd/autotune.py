@@ -12,6 +12,8 @@ faces and vertices, bit for bit):
 - PAMO_OPT_GUARD_WAVES: 0 runs the guard kernel with the compiler's register
   allocation, 4 a variant capped for 4 waves per SIMD.
 - PAMO_OPT_EDGE_LANES: 1, 4 or 8 lanes per edge in the edge-cost kernel.
+- PAMO_OPT_RING_LANES: 1, 4 or 8 lanes per item in the quadric, selection
+  and undo kernels.
 
 `tune_stage2` times the fused loop (pamo_simplify) on a caller's mesh for
 each candidate value of one knob at a time (coordinate descent, the other
@@ -29,17 +31,18 @@ import torch
 
 from . import _hip
 
-DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 18, "guard_split_log": 3, "guard_waves": 0, "edge_lanes": 1}
+DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 18, "guard_split_log": 3, "guard_waves": 0, "edge_lanes": 1, "ring_lanes": 1}
 CANDIDATES = {
     "tree_keep": [50, 65, 80, 90, 100],
     "guard_split_faces": [1 << 16, 1 << 18, 1 << 20],
     "guard_split_log": [2, 3, 4],
     "guard_waves": [0, 4],
     "edge_lanes": [1, 4, 8],
+    "ring_lanes": [1, 4, 8],
 }
 _OPTION = {"tree_keep": _hip.PAMO_OPT_TREE_KEEP, "guard_split_faces": _hip.PAMO_OPT_GUARD_SPLIT_FACES,
            "guard_split_log": _hip.PAMO_OPT_GUARD_SPLIT_LOG, "guard_waves": _hip.PAMO_OPT_GUARD_WAVES,
-           "edge_lanes": _hip.PAMO_OPT_EDGE_LANES}
+           "edge_lanes": _hip.PAMO_OPT_EDGE_LANES, "ring_lanes": _hip.PAMO_OPT_RING_LANES}
 _CACHE: dict = {}
 
 

@@ -154,6 +154,7 @@ struct Ctx {
     int guard_split_log = 3;
     int guard_waves = 0;        // 0: the compiler's register allocation for the guard; 4: capped for 4 waves
     int edge_lanes = 1;         // lanes per edge in the edge-cost kernel (1, 4 or 8)
+    int ring_lanes = 1;         // lanes per item in the quadric, selection and undo kernels (1, 4 or 8)
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -521,7 +522,13 @@ static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
             c.eord_cnt);
     // quadrics
-    LAUNCHP(K_QUADRIC, 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
+    const double quad_bytes = 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP;
+    if (c.ring_lanes == 8)
+        LAUNCHP(K_QUADRIC, quad_bytes, quadric_group_k<8>, 8LL * nP, m, c.ring_off, c.rv, nP, c.Q);
+    else if (c.ring_lanes == 4)
+        LAUNCHP(K_QUADRIC, quad_bytes, quadric_group_k<4>, 4LL * nP, m, c.ring_off, c.rv, nP, c.Q);
+    else
+        LAUNCHP(K_QUADRIC, quad_bytes, quadric_k, nP, m, c.ring_off, c.rv, nP, c.Q);
     PAMO_TRY(sync_counters(c));
     const int nE = c.hcnt[C_NEDGE];
     c.n_edges = nE;
@@ -602,14 +609,21 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
             (unsigned long long*)c.vmin);
     const double sel_bytes = 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP;
-    if (c.placement)
-        LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, select_collapse_k<true>, nE + 1, m, orig_mesh(c), c.occ, c.ring_off,
-                c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
-                c.esel, (const float4*)c.Q);
-    else
-        LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, select_collapse_k<false>, nE + 1, m, orig_mesh(c), c.occ, c.ring_off,
-                c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,
-                c.esel, (const float4*)c.Q);
+#define PAMO_SELECT(K, lanes)                                                                                 \
+    LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, K, (long long)(nE + 1) * (lanes), m, orig_mesh(c), c.occ, c.ring_off,   \
+            c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,  \
+            c.esel, (const float4*)c.Q)
+    if (c.ring_lanes == 8) {
+        if (c.placement) PAMO_SELECT((select_collapse_group_k<true, 8>), 8);
+        else PAMO_SELECT((select_collapse_group_k<false, 8>), 8);
+    } else if (c.ring_lanes == 4) {
+        if (c.placement) PAMO_SELECT((select_collapse_group_k<true, 4>), 4);
+        else PAMO_SELECT((select_collapse_group_k<false, 4>), 4);
+    } else {
+        if (c.placement) PAMO_SELECT(select_collapse_k<true>, 1);
+        else PAMO_SELECT(select_collapse_k<false>, 1);
+    }
+#undef PAMO_SELECT
     PAMO_TRY(scan_op(CollapseListOp{c.esel, c.recs, c.dcnt}, nE, c.lb, c.stream));
     // cleanup
     PAMO_TRY(scan_op(CleanupOp{m, c.eu, c.ev, nE, nT, c.recs, c.dcnt}, (long long)std::max(nE, nT), c.lb, c.stream,
@@ -648,8 +662,15 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     while (npushed != 0) {
         ++it;
         ++c.undo_iters;
-        LAUNCH(undo_apply_k, npushed, m, orig_mesh(c), c.occ, c.ring_off, c.ring, c.eu, c.ev, c.pushed, npushed,
-               (const int*)c.pts_map, c.undo_list, first);
+        if (c.ring_lanes == 8)
+            LAUNCH(undo_apply_group_k<8>, 8LL * npushed, m, orig_mesh(c), c.occ, c.ring_off, c.ring, c.eu, c.ev,
+                   c.pushed, npushed, (const int*)c.pts_map, c.undo_list, first);
+        else if (c.ring_lanes == 4)
+            LAUNCH(undo_apply_group_k<4>, 4LL * npushed, m, orig_mesh(c), c.occ, c.ring_off, c.ring, c.eu, c.ev,
+                   c.pushed, npushed, (const int*)c.pts_map, c.undo_list, first);
+        else
+            LAUNCH(undo_apply_k, npushed, m, orig_mesh(c), c.occ, c.ring_off, c.ring, c.eu, c.ev, c.pushed, npushed,
+                   (const int*)c.pts_map, c.undo_list, first);
         first += 2 * npushed;
         PAMO_TRY(run_guard(c, nT, true));
         PAMO_TRY(undo_candidates(c, npushed));
@@ -805,6 +826,11 @@ int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value) {
     if (option == PAMO_OPT_GUARD_SPLIT_LOG) {
         PAMO_CHECK(value >= 0 && value <= 4, "pamo_ctx_set_option: split depth must be in [0, 4]");
         c.guard_split_log = (int)value;
+        return 0;
+    }
+    if (option == PAMO_OPT_RING_LANES) {
+        PAMO_CHECK(value == 1 || value == 4 || value == 8, "pamo_ctx_set_option: ring lanes must be 1, 4 or 8");
+        c.ring_lanes = (int)value;
         return 0;
     }
     if (option == PAMO_OPT_EDGE_LANES) {

@@ -265,6 +265,69 @@ __global__ void __launch_bounds__(kBlock) quadric_k(Mesh m, const int* __restric
     Q[3 * (long long)v + 2] = make_float4(q[8], q[9], 0.0f, 0.0f);
 }
 
+// Makes the stores of a group's lanes visible to the other lanes of the
+// workgroup (the group shares one wave) before they read the same faces.
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Group mask of the calling lane's kG-lane group within its wave.
+template <int kG>
+__device__ __forceinline__ unsigned long long group_mask() {
+    const int lane = threadIdx.x & (kWave - 1);
+    return ((kG == 64) ? ~0ull : ((1ull << kG) - 1)) << (lane & ~(kG - 1));
+}
+
+// quadric_k with kG lanes per vertex: each lane forms the plane of every
+// kG-th ring face and all lanes add the ten products face by face in the
+// single-lane order (the planes come from their lanes by shuffles), so the
+// sums are bit-identical to quadric_k's.
+template <int kG>
+__global__ void __launch_bounds__(kBlock) quadric_group_k(Mesh m, const int* __restrict__ off,
+                                                          const int4* __restrict__ rv, int nP,
+                                                          float4* __restrict__ Q) {
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(threadIdx.x & (kG - 1));
+    const long long vv = gt / kG;
+    if (vv >= nP) return;  // the whole group
+    const int v = (int)vv;
+    float q[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) q[c] = 0.0f;
+    const int b = off[v], nf = off[v + 1] - b;
+    for (int r0 = 0; r0 < nf; r0 += kG) {  // the same trip count in every lane of the group
+        float p[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (r0 + g < nf) {
+            const int4 t = rv[b + r0 + g];
+            F3 a = ld(m, t.x), bb = ld(m, t.y), d = ld(m, t.z);
+            F3 n = f3cross(f3sub(bb, a), f3sub(d, a));
+            n = f3divs(n, f3norm(n));
+            p[0] = n.x;
+            p[1] = n.y;
+            p[2] = n.z;
+            p[3] = -f3dot(n, a);
+        }
+        const int nj = nf - r0 < kG ? nf - r0 : kG;
+        for (int j = 0; j < kG; ++j) {
+            float pj[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) pj[k] = __shfl(p[k], j, kG);
+            if (j < nj) {
+                int c = 0;
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+                    for (int jj = ii; jj < 4; ++jj) q[c++] += pj[ii] * pj[jj];
+            }
+        }
+    }
+    if (g != 0) return;
+    Q[3 * (long long)v] = make_float4(q[0], q[1], q[2], q[3]);
+    Q[3 * (long long)v + 1] = make_float4(q[4], q[5], q[6], q[7]);
+    Q[3 * (long long)v + 2] = make_float4(q[8], q[9], 0.0f, 0.0f);
+}
+
 __device__ __forceinline__ int succ_in(int4 t, int x) {
     if (t.x == x) return t.y;
     if (t.y == x) return t.z;
@@ -716,6 +779,107 @@ __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int*
         }
     }
     sel[e] = s;
+}
+
+// select_collapse_k with kG lanes per edge.  The lanes split the faces of
+// the two rings: the selection test and the skinny test are
+// order-independent (every face must hold the edge's key; any face
+// skinny), and each face's rewrite and restore touch that face only.  The
+// rewrite keeps the single-lane order between the rings: every face of
+// ring(u) is rewritten before any face of ring(v) is read, since a face in
+// both rings (it holds u and v) must be removed by the first pass.
+template <bool kOptimal, int kG>
+__global__ void __launch_bounds__(kBlock) select_collapse_group_k(Mesh m, Orig o, int* occ,
+                                                                  const int* __restrict__ off,
+                                                                  const int* __restrict__ ring,
+                                                                  const int4* __restrict__ rv,
+                                                                  const int* __restrict__ eu,
+                                                                  const int* __restrict__ ev, int nE,
+                                                                  const uint32_t* __restrict__ cost,
+                                                                  const unsigned long long* __restrict__ vmin,
+                                                                  uint32_t collapse_t, int* __restrict__ sel,
+                                                                  const float4* __restrict__ Q) {
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(threadIdx.x & (kG - 1));
+    const long long ee = gt / kG;
+    if (ee > nE) return;  // the whole group
+    const int e = (int)ee;
+    if (e == nE) {
+        if (g == 0) sel[nE] = 0;
+        return;
+    }
+    const uint32_t c = cost[e];
+    if (c > collapse_t) {
+        if (g == 0) sel[e] = 0;
+        return;
+    }
+    const unsigned long long key = edge_key(c, e);
+    const int u = eu[e], v = ev[e];
+    const int ub = off[u], vb = off[v];
+    const int nu = off[u + 1] - ub, nv = off[v + 1] - vb, nf = nu + nv;
+    const unsigned long long gm = group_mask<kG>();
+    bool bad = false;
+    for (int i = g; i < nf && !bad; i += kG) {
+        const int4 t = rv[i < nu ? ub + i : vb + (i - nu)];
+        unsigned long long mn = vmin[t.x];
+        const unsigned long long y = vmin[t.y], z = vmin[t.z];
+        mn = y < mn ? y : mn;
+        mn = z < mn ? z : mn;
+        bad = mn != key;
+    }
+    if (__ballot(bad) & gm) {
+        if (g == 0) sel[e] = 0;
+        return;
+    }
+    F3 vm;
+    if (kOptimal) {
+        float M[10];
+        edge_quadric(Q, u, v, M);
+        vm = place_optimal(M, ld(m, u), ld(m, v));
+    } else {
+        vm = f3divs(f3add(ld(m, u), ld(m, v)), 2.0f);
+    }
+    if (g == 0) {
+        st(m, u, vm);
+        st(m, v, F3{0.0f, 0.0f, 0.0f});
+        occ[v] = 0;
+    }
+    for (int i = g; i < nu; i += kG) {
+        const int f = ring[ub + i];
+        if (tri_has(m.i[f], m.j[f], m.k[f], v)) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
+    }
+    group_sync();
+    for (int i = g; i < nv; i += kG) {
+        const int f = ring[vb + i];
+        const int ii = m.i[f], jj = m.j[f], kk = m.k[f];
+        if (tri_has(ii, jj, kk, u)) { m.i[f] = -1; m.j[f] = -1; m.k[f] = -1; }
+        else if (ii == v) m.i[f] = u;
+        else if (jj == v) m.j[f] = u;
+        else if (kk == v) m.k[f] = u;
+    }
+    group_sync();
+    bool sk = false;
+    for (int i = g; i < nf && !sk; i += kG) {
+        const int f = ring[i < nu ? ub + i : vb + (i - nu)];
+        const int ii = m.i[f];
+        if (ii == -1) continue;
+        sk = skinny(ld(m, ii), ld(m, m.j[f]), ld(m, m.k[f]));
+    }
+    if (__ballot(sk) & gm) {
+        // restore_edge across the group (every face gets its snapshot corners)
+        if (g == 0) {
+            occ[v] = 1;
+            m.p[u] = o.p[u];
+            m.p[v] = o.p[v];
+        }
+        for (int i = g; i < nf; i += kG) {
+            const int f = ring[i < nu ? ub + i : vb + (i - nu)];
+            m.i[f] = o.i[f];
+            m.j[f] = o.j[f];
+            m.k[f] = o.k[f];
+        }
+    }
+    if (g == 0) sel[e] = 1;
 }
 
 // ---------------------------------------------------------------- legacy PaSP round (cusimp.cu)
@@ -1667,6 +1831,37 @@ __global__ void undo_apply_k(Mesh m, Orig o, int* occ, const int* __restrict__ o
     restore_edge(m, o, occ, off, ring, u, v);
 }
 
+// undo_apply_k with kG lanes per record, splitting the ring faces to
+// restore (pushed collapses have disjoint rings).
+template <int kG>
+__global__ void undo_apply_group_k(Mesh m, Orig o, int* occ, const int* __restrict__ off,
+                                   const int* __restrict__ ring, const int* __restrict__ eu,
+                                   const int* __restrict__ ev, const int* __restrict__ pushed, int npushed,
+                                   const int* __restrict__ pts_map, int* __restrict__ undo_list, int first) {
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(threadIdx.x & (kG - 1));
+    const long long rr = gt / kG;
+    if (rr >= npushed) return;
+    const int r = (int)rr;
+    const int e = pushed[r];
+    const int u = eu[e], v = ev[e];
+    if (g == 0) {
+        undo_list[first + 2 * r] = pts_map[u];
+        undo_list[first + 2 * r + 1] = pts_map[v];
+        occ[v] = 1;
+        m.p[u] = o.p[u];
+        m.p[v] = o.p[v];
+    }
+    const int ub = off[u], vb = off[v];
+    const int nu = off[u + 1] - ub, nf = nu + off[v + 1] - vb;
+    for (int i = g; i < nf; i += kG) {
+        const int f = ring[i < nu ? ub + i : vb + (i - nu)];
+        m.i[f] = o.i[f];
+        m.j[f] = o.j[f];
+        m.k[f] = o.k[f];
+    }
+}
+
 // ---------------------------------------------------------------- outputs / compaction (A15)
 __global__ void store_points_k(const float4* __restrict__ p, int n, float* __restrict__ aos) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1870,10 +2065,19 @@ struct UndoListOp {
         if (r >= min(nrec, (nreg + 511) / 512 * 512)) return 0;
         const int e = recs[r];
         const int u = eu[e], v = ev[e];
-        int hit = 0;
-        for (int a = off[u]; a < off[u + 1] && !hit; ++a) hit = flagged[ring[a]];
-        for (int a = off[v]; a < off[v + 1] && !hit; ++a) hit = flagged[ring[a]];
-        return hit ? 1 : 0;
+        // four ring entries, then their four flags, per step
+        for (int side = 0; side < 2; ++side) {
+            const int w = side ? v : u;
+            const int b = off[w], end = off[w + 1];
+            for (int a = b; a < end; a += 4) {
+                const int f0 = ring[a];
+                const int f1 = a + 1 < end ? ring[a + 1] : f0;
+                const int f2 = a + 2 < end ? ring[a + 2] : f0;
+                const int f3 = a + 3 < end ? ring[a + 3] : f0;
+                if (flagged[f0] | flagged[f1] | flagged[f2] | flagged[f3]) return 1;
+            }
+        }
+        return 0;
     }
     __device__ void emit(long long r, int v, int pos) const {
         if (v) pushed[pos] = recs[r];

@@ -41,10 +41,12 @@ def _compare_round(it, a, b):
         assert len(bad) == 0, f"round {it}: {n} differs at {bad[:8]} ({x.ravel()[bad[:4]]} vs {y.ravel()[bad[:4]]})"
 
 
-def _lockstep(V, F, ratio, torch, oracle, check_debug_rounds=3, tolerance=4):
+def _lockstep(V, F, ratio, torch, oracle, check_debug_rounds=3, tolerance=4, options=None):
     from pamo_amd.pamo import CUDSP_Free
     from pamo_amd import _hip
     op = CUDSP_Free(0)
+    for opt, val in (options or {}).items():
+        _hip.check(_hip.lib().pamo_ctx_set_option(op.ctx.ptr, opt, val), "set_option")
     ref = oracle.Round()
     gpu_fwd = _gpu_forward(op, torch)
     stats = {"rounds": 0, "undo_rounds": 0}
@@ -87,14 +89,19 @@ def test_round_parity_icosphere(torch_cuda, oracle_lib):
     assert synth.is_watertight_manifold(Fo)
 
 
-def test_round_parity_undo_path(torch_cuda, oracle_lib):
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_round_parity_undo_path(torch_cuda, oracle_lib, lanes):
     """Closely spaced sheets: collapses on one sheet cross the other, so the
-    guard + undo loop runs; GPU and oracle must take identical decisions."""
-    from pamo_amd import synth
+    guard + undo loop runs; GPU and oracle must take identical decisions,
+    with one lane per item and with the 8-lane cost, quadric, selection and
+    undo kernels."""
+    from pamo_amd import synth, _hip
     V, F = synth.icosphere_sh(4, 3)
     V2, F2 = synth.offset_sheets(V, F, rel_offset=5e-3)
-    print(f"case: offset sheets L4 seed 3, {len(F2)} faces", flush=True)
-    Vo, Fo, hist, stats = _lockstep(centred(V2, F2), F2, 0.05, torch_cuda, oracle_lib, check_debug_rounds=2)
+    print(f"case: offset sheets L4 seed 3, {len(F2)} faces, {lanes} lanes", flush=True)
+    opts = {_hip.PAMO_OPT_EDGE_LANES: lanes, _hip.PAMO_OPT_RING_LANES: lanes}
+    Vo, Fo, hist, stats = _lockstep(centred(V2, F2), F2, 0.05, torch_cuda, oracle_lib, check_debug_rounds=2,
+                                    options=opts)
     assert stats["undo_rounds"] > 0, "case did not exercise the undo loop"
 
 
@@ -348,10 +355,12 @@ def test_optimal_placement_matches_oracle(torch_cuda, oracle_lib, level):
     assert synth.is_watertight_manifold(fo)
     vm, fm, hm = oracle_lib.run(Vc, F, 0.05, min_verts=0)
     assert not (len(fm) == len(fo) and np.array_equal(vm, vo))  # the option changes the result
-    # the 8-lanes-per-edge cost kernel with the solve gives the same result
+    # the 8-lane cost, quadric, selection and undo kernels with the solve
+    # give the same result
     ctx8 = _hip.Context(0)
     _hip.check(_hip.lib().pamo_ctx_set_option(ctx8.ptr, _hip.PAMO_OPT_PLACEMENT, 1), "set_option")
     _hip.check(_hip.lib().pamo_ctx_set_option(ctx8.ptr, _hip.PAMO_OPT_EDGE_LANES, 8), "set_option")
+    _hip.check(_hip.lib().pamo_ctx_set_option(ctx8.ptr, _hip.PAMO_OPT_RING_LANES, 8), "set_option")
     v8, f8, h8 = simplify_device(ctx8, torch.from_numpy(Vc).cuda(), torch.from_numpy(F).cuda(), 0.05, min_verts=0)
     assert np.array_equal(h8, ho)
     assert np.array_equal(f8.cpu().numpy(), fo)
@@ -404,7 +413,8 @@ def test_autotuned_settings_give_the_same_result(torch_cuda, oracle_lib):
     for settings in ({"guard_split_log": 0}, {"guard_split_log": 2}, {"guard_split_log": 4},
                      {"guard_split_faces": 0}, {"tree_keep": 30, "guard_split_faces": 1 << 20},
                      {"guard_waves": 4}, {"guard_waves": 4, "guard_split_log": 0},
-                     {"edge_lanes": 4}, {"edge_lanes": 8, "tree_keep": 100}):
+                     {"edge_lanes": 4}, {"edge_lanes": 8, "tree_keep": 100},
+                     {"ring_lanes": 4}, {"ring_lanes": 8, "edge_lanes": 4}):
         ctx = _hip.Context(0)
         autotune.apply(ctx, settings)
         vg, fg, hg = simplify_device(ctx, Vd, Fd, 0.02, min_verts=0)
