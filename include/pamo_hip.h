@@ -27,6 +27,16 @@ int pamo_ctx_destroy(pamo_ctx* ctx);
  * object, simp_cuda/pamo/__init__.py:21). */
 int pamo_ctx_reset(pamo_ctx* ctx);
 
+/* Stream of the context's calls (the reference runs its kernels on the
+ * current CUDA stream, cusimp_free.cu; here a context has its own
+ * non-blocking stream by default).  own = 0 runs the following calls on
+ * `stream`, a hipStream_t of the context's device (NULL: its null stream),
+ * so they are ordered after the caller's earlier work without a host
+ * synchronisation; own = 1 returns to the context's stream.  Work queued on
+ * the previous stream is finished first.  The calls still return when
+ * their results are ready. */
+int pamo_ctx_set_stream(pamo_ctx* ctx, void* stream, int own);
+
 /* Test option (no counterpart in the reference): with value 1 the guard
  * query runs with a 32-entry frontier per wave, so the exact overflow walk
  * of the frontier traversal is exercised; 0 restores the default.  The
@@ -66,24 +76,24 @@ int pamo_ctx_reset(pamo_ctx* ctx);
 #define PAMO_OPT_TREE_KEEP 5
 /* Guard launch shape (no counterpart in the reference; results do not
  * change): on meshes of at most PAMO_OPT_GUARD_SPLIT_FACES faces (default
- * 262144) each group of queries is served by 2^PAMO_OPT_GUARD_SPLIT_LOG
+ * 65536) each group of queries is served by 2^PAMO_OPT_GUARD_SPLIT_LOG
  * waves (default 3, at most 4), one per subtree under the root, so that one
  * query with a large box does not hold a late-round launch in one wave. */
 #define PAMO_OPT_GUARD_SPLIT_FACES 6
 #define PAMO_OPT_GUARD_SPLIT_LOG 7
 /* Guard register budget (no counterpart in the reference; results do not
- * change): 0 (default) runs the guard kernel as the compiler allocates its
- * registers, 4 runs a variant capped at 128 vector registers so that 4 waves
- * fit per SIMD. */
+ * change): 0 runs the guard kernel as the compiler allocates its registers
+ * (129 vector registers, 3 waves per SIMD), 4 (default) a variant capped at
+ * 128 so that 4 waves fit per SIMD. */
 #define PAMO_OPT_GUARD_WAVES 8
 /* Edge-cost launch shape (no counterpart in the reference; results do not
- * change): 1 (default) lane per edge, or 4 / 8 lanes sharing one edge's
+ * change): 1 lane per edge, or 4 (default) / 8 lanes sharing one edge's
  * ring faces and adding their terms in the single-lane order. */
 #define PAMO_OPT_EDGE_LANES 9
 /* Ring-walk launch shape (no counterpart in the reference; results do not
- * change): 1 (default) lane per vertex / edge / record in the quadric,
- * selection-and-collapse and undo kernels, or 4 / 8 lanes splitting the
- * one-ring faces (float sums kept in the single-lane order). */
+ * change): 1 lane per vertex / edge / record in the quadric,
+ * selection-and-collapse and undo kernels, or 4 / 8 (default) lanes
+ * splitting the one-ring faces (float sums kept in the single-lane order). */
 #define PAMO_OPT_RING_LANES 10
 int pamo_ctx_set_option(pamo_ctx* ctx, int option, long long value);
 

@@ -74,6 +74,7 @@ SIGNATURES = [
     ("pamo_ctx_create", c_int, [ctypes.POINTER(c_vp), c_int]),
     ("pamo_ctx_destroy", c_int, [c_vp]),
     ("pamo_ctx_reset", c_int, [c_vp]),
+    ("pamo_ctx_set_stream", c_int, [c_vp, c_vp, c_int]),
     ("pamo_ctx_set_option", c_int, [c_vp, c_int, c_ll]),
     ("pamo_last_error", ctypes.c_char_p, []),
     ("pamo_round", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_int,

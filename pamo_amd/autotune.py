@@ -31,7 +31,8 @@ import torch
 
 from . import _hip
 
-DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 18, "guard_split_log": 3, "guard_waves": 0, "edge_lanes": 1, "ring_lanes": 1}
+DEFAULTS = {"tree_keep": 80, "guard_split_faces": 1 << 16, "guard_split_log": 3, "guard_waves": 4, "edge_lanes": 4,
+            "ring_lanes": 8}
 CANDIDATES = {
     "tree_keep": [50, 65, 80, 90, 100],
     "guard_split_faces": [1 << 16, 1 << 18, 1 << 20],

@@ -92,7 +92,8 @@ void prof_end(int id, hipStream_t st) {
 
 // ---------------------------------------------------------------- context
 struct Ctx {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every call runs on
+    hipStream_t own_stream = nullptr;  // the context's own non-blocking stream
     int device = 0;
     // logical capacities, grown like the reference (n + n/5)
     size_t cap_pts = 0, cap_tris = 0, cap_edges = 0;
@@ -150,11 +151,11 @@ struct Ctx {
     int tree_n = 0;             // leaves of the last tree, as last read back
     // guard launch shape: meshes of at most guard_split_faces faces serve
     // each query group with 2^guard_split_log waves, one per subtree
-    int guard_split_faces = 1 << 18;
+    int guard_split_faces = 1 << 16;
     int guard_split_log = 3;
-    int guard_waves = 0;        // 0: the compiler's register allocation for the guard; 4: capped for 4 waves
-    int edge_lanes = 1;         // lanes per edge in the edge-cost kernel (1, 4 or 8)
-    int ring_lanes = 1;         // lanes per item in the quadric, selection and undo kernels (1, 4 or 8)
+    int guard_waves = 4;        // 0: the compiler's register allocation for the guard; 4: capped for 4 waves
+    int edge_lanes = 4;         // lanes per edge in the edge-cost kernel (1, 4 or 8)
+    int ring_lanes = 8;         // lanes per item in the quadric, selection and undo kernels (1, 4 or 8)
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
@@ -778,14 +779,15 @@ int pamo_ctx_create(pamo_ctx** out, int device) {
     Ctx* c = new Ctx();
     c->device = device;
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    c->stream = c->own_stream;
     if (e == hipSuccess) e = hipMalloc((void**)&c->dcnt, sizeof(int) * pamo::C_COUNT);
     if (e == hipSuccess) e = hipMemsetAsync(c->dcnt, 0, sizeof(int) * pamo::C_COUNT, c->stream);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcnt, sizeof(int) * pamo::C_COUNT, 0);
     if (e != hipSuccess) {
         pamo::set_error(std::string("pamo_ctx_create: ") + hipGetErrorString(e));
         pamo::free_all(*c);
-        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
         delete c;
         return 1;
     }
@@ -798,10 +800,29 @@ int pamo_ctx_destroy(pamo_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     c->prof.drain();
     pamo::free_all(*c);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
+    return 0;
+}
+
+// Runs the context's calls on a caller's stream of the same device (own = 0;
+// stream may be NULL, the device's null stream), or again on its own stream
+// (own = 1).  The work queued before the switch on the previous stream is
+// finished first, so the context's buffers are never used by two streams at
+// once.
+int pamo_ctx_set_stream(pamo_ctx* ctx, void* stream, int own) {
+    pamo::clear_error();
+    PAMO_CHECK(ctx != nullptr, "pamo_ctx_set_stream: null context");
+    PAMO_CHECK(own == 0 || own == 1, "pamo_ctx_set_stream: own must be 0 or 1");
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    hipStream_t next = own ? c.own_stream : reinterpret_cast<hipStream_t>(stream);
+    if (next == c.stream) return 0;
+    PAMO_HIP(hipSetDevice(c.device));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    c.stream = next;
     return 0;
 }
 

@@ -1982,7 +1982,7 @@ __global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __re
 // Guard queries: the changed live faces among the tree's leaves, in leaf
 // order (a carried tree's dead leaves hold the sentinel nT).
 struct QueryListOp {
-    static constexpr int kIpt = 8;
+    static constexpr int kIpt = 4;
     const uint32_t* val;
     const uint8_t* changed;
     const int* ti;
@@ -2017,7 +2017,7 @@ struct CollapseListOp {
 // the line collapses go, in edge order, after the regular collapses in the
 // records.  Items cover max(nE, nT); the two parts touch disjoint data.
 struct CleanupOp {
-    static constexpr int kIpt = 8;
+    static constexpr int kIpt = 2;
     Mesh m;
     const int* eu;
     const int* ev;
@@ -2046,10 +2046,10 @@ struct CleanupOp {
 // Undo candidates (get_undo_candidate_kernel, cusimp_free.cu:776-827): the
 // records one of whose end vertices touches a flagged face, among the
 // first min(records, regular collapses rounded up to 512) records, a count
-// taken from the device counters.  Two records per thread: each walks two
+// taken from the device counters.  One record per thread: each walks two
 // vertex rings.
 struct UndoListOp {
-    static constexpr int kIpt = 2;
+    static constexpr int kIpt = 1;
     const int* recs;
     const int* off;
     const int* ring;
@@ -2091,7 +2091,7 @@ struct UndoListOp {
 // t <= nT (fpos[nT] = live faces), which the guard-tree carry reads.  Items
 // cover max(nP, nT) + 1.
 struct CompactMeshOp {
-    static constexpr int kIpt = 8;
+    static constexpr int kIpt = 2;
     Mesh m;
     int nP, nT;
     const int* occ;

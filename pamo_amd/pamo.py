@@ -3,8 +3,10 @@
 Mirrors simp_cuda/pamo/__init__.py:18-205 (class PaMO) and
 simp_cuda/src/pybind.cpp:16-103 (class CUDSP_Free) of the reference, on top
 of libpamo_hip.so.  Tensors stay on the GPU and are handed to the library by
-pointer; the library runs on its own HIP stream, so the caller's stream is
-synchronised before each call.
+pointer.  A call runs on the caller's current stream when that is a created
+stream, so it is ordered after the caller's queued work without a host wait;
+from the null stream it runs on the context's own stream after that stream
+has drained.
 """
 from __future__ import annotations
 
@@ -51,6 +53,19 @@ def _check_faces(triangles: torch.Tensor, n_pts: int, name: str = "triangles"):
         raise RuntimeError(f"{name}: face indices outside [0, {n_pts}) (rows must be all -1 or valid)")
 
 
+def _order_after_caller(ctx: _hip.Context, device) -> None:
+    """Orders the context's next call after the work the caller has queued:
+    on the caller's current stream when it is a created stream (no host
+    wait), else on the context's own stream once the caller's null stream
+    has drained."""
+    s = torch.cuda.current_stream(device)
+    if s.cuda_stream:
+        _hip.check(_hip.lib().pamo_ctx_set_stream(ctx.ptr, ctypes.c_void_p(s.cuda_stream), 0), "pamo_ctx_set_stream")
+    else:
+        _hip.check(_hip.lib().pamo_ctx_set_stream(ctx.ptr, None, 1), "pamo_ctx_set_stream")
+        s.synchronize()
+
+
 class CUDSP_Free:
     """One collapse round per ``forward`` call (pybind.cpp:52-103).
 
@@ -86,7 +101,7 @@ class CUDSP_Free:
         cap = 4 * (n_pts + 1) + n_undo + 16
         out_undo = torch.empty(cap, dtype=torch.int32, device=dev)
         n_out = ctypes.c_int(0)
-        torch.cuda.current_stream(dev).synchronize()
+        _order_after_caller(self.ctx, dev)
         _hip.check(_hip.lib().pamo_round(
             self.ctx.ptr, points.data_ptr(), n_pts, triangles.data_ptr(), n_tris,
             verts_undo.data_ptr() if n_undo > 0 else None, n_undo, float(scale), float(threshold),
@@ -116,7 +131,7 @@ def face_centroid_mean(ctx: _hip.Context, points: torch.Tensor, triangles: torch
     _check_mesh_types(points, triangles)
     _check_faces(triangles, int(points.shape[0]))
     out = (ctypes.c_float * 3)()
-    torch.cuda.current_stream(points.device).synchronize()
+    _order_after_caller(ctx, points.device)
     _hip.check(_hip.lib().pamo_centroid(ctx.ptr, points.data_ptr(), int(points.shape[0]), triangles.data_ptr(),
                                         int(triangles.shape[0]), out), "pamo_centroid")
     return np.array([out[0], out[1], out[2]], dtype=np.float32)
@@ -135,7 +150,7 @@ def simplify_device(ctx: _hip.Context, verts: torch.Tensor, faces: torch.Tensor,
     # one entry per round: at most `iters` rounds, and a round that changes
     # the face count removes at least one face
     hist = np.zeros(max(1, min(int(iters), nf + int(tolerance) * (nf + 1) + 1, 1 << 26)), dtype=np.int32)
-    torch.cuda.current_stream(verts.device).synchronize()
+    _order_after_caller(ctx, verts.device)
     _hip.check(_hip.lib().pamo_simplify(
         ctx.ptr, verts.data_ptr(), nv, faces.data_ptr(), nf, float(ratio), int(tolerance), float(threshold),
         int(iters), int(min_verts), out_v.data_ptr(), out_f.data_ptr(), ctypes.byref(onv), ctypes.byref(onf),
@@ -268,7 +283,7 @@ class CUDSP:
         out_tris = torch.empty((n_tris, 3), dtype=torch.int32, device=dev)
         out_occ = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
         out_map = torch.empty((n_pts, 1), dtype=torch.int32, device=dev)
-        torch.cuda.current_stream(dev).synchronize()
+        _order_after_caller(self.ctx, dev)
         _hip.check(_hip.lib().pamo_pasp_round(
             self.ctx.ptr, points.data_ptr(), n_pts, triangles.data_ptr(), n_tris, float(scale), float(threshold),
             int(bool(init)), out_pts.data_ptr(), out_tris.data_ptr(), out_occ.data_ptr(), out_map.data_ptr()),
@@ -302,7 +317,7 @@ class PaSP(torch.nn.Module):
         out_v = torch.empty_like(points)
         out_f = torch.empty_like(triangles)
         onv, onf, rounds = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-        torch.cuda.current_stream(points.device).synchronize()
+        _order_after_caller(self.ctx, points.device)
         _hip.check(_hip.lib().pamo_pasp_run(
             self.ctx.ptr, points.data_ptr(), nv, triangles.data_ptr(), nf, float(threshold), int(iter),
             out_v.data_ptr(), out_f.data_ptr(), ctypes.byref(onv), ctypes.byref(onf), ctypes.byref(rounds)),

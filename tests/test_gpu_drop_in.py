@@ -158,3 +158,35 @@ def test_guard_walk_bound_reports_an_error(torch_cuda, oracle_lib):
     vo, fo, ho = oracle_lib.run(Vc, F, 0.1, min_verts=0)
     assert np.array_equal(hg, ho)
     assert np.array_equal(fg.cpu().numpy(), fo)
+
+
+def test_calls_run_on_the_callers_stream(torch_cuda, oracle_lib):
+    """From a created torch stream the library call runs on that stream
+    (pamo_ctx_set_stream), ordered after the work the caller queued there
+    without a host wait: an input written on the stream just before the call
+    is read correctly, the result equals the oracle's, and the context goes
+    back to its own stream for a call from the null stream."""
+    from pamo_amd import synth, _hip
+    from pamo_amd.pamo import simplify_device
+    torch = torch_cuda
+    V, F = synth.icosphere_sh(5, 4)
+    Vc = (V - tris_mean(V, F)).astype(np.float32)
+    print(f"case: caller's stream, icosphere L5, {len(F)} faces", flush=True)
+    vo, fo, ho = oracle_lib.run(Vc, F, 0.05, min_verts=0)
+    ctx = _hip.Context(0)
+    Fd = torch.from_numpy(F).cuda()
+    src = torch.from_numpy(Vc).cuda()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        Vs = torch.empty_like(src)
+        Vs.copy_(src)  # queued on s; the call must see it
+        Vs.mul_(1.0)
+        vg, fg, hg = simplify_device(ctx, Vs, Fd, 0.05, min_verts=0)
+    s.synchronize()
+    assert np.array_equal(hg, ho)
+    assert np.array_equal(fg.cpu().numpy(), fo)
+    assert np.array_equal(_bits(vg.cpu().numpy()), _bits(vo))
+    _hip.check(_hip.lib().pamo_ctx_reset(ctx.ptr), "pamo_ctx_reset")  # a fresh context's state
+    v2, f2, h2 = simplify_device(ctx, src, Fd, 0.05, min_verts=0)  # null stream: the context's own
+    assert np.array_equal(h2, ho) and np.array_equal(f2.cpu().numpy(), fo)
