@@ -322,6 +322,39 @@ int pamo_s3_stats(pamo_s3* s3, double* out8);
 #define PAMO_S3_OPT_CG_GRAPH 1
 int pamo_s3_set_option(pamo_s3* s3, int option, long long value);
 
+/* ---------------------------------------------------------------- stage 1 (remeshing)
+ * Replaces PaMO.remesh and its inputs (simp_cuda/pamo/__init__.py:60-120,
+ * 130-141): the triangles normalised into the unit cube as
+ * preprocess_mesh does, a narrow-band distance field on a resolution^3
+ * grid (band 3 / resolution, in place of torchcumesh2sdf.get_sdf), signed
+ * by a flood fill of the outside from the grid border, and the level
+ * 0.9 / resolution extracted as a closed manifold triangle mesh by
+ * marching cubes with one face rule for every case (in place of pdmc.DMC
+ * and the quad split).  The reference picks resolution 256, 128 when the
+ * target is at most 1000 faces and 64 at most 50. */
+typedef struct pamo_s1 pamo_s1;
+int pamo_s1_create(pamo_s1** out, int device);
+int pamo_s1_destroy(pamo_s1* s1);
+/* Device float32 points (n_points,3) and int32 faces (n_faces,3) with
+ * indices already checked; mean3 (host) is the centring offset of PaMO.run
+ * (pamo_centroid).  Returns the output sizes; the mesh stays in the context
+ * until pamo_s1_get. */
+int pamo_s1_remesh(pamo_s1* s1, const float* d_points, int n_points, const int* d_faces, int n_faces,
+                   const float* mean3, int resolution, int* n_verts_out, int* n_faces_out);
+/* The remeshed vertices, in the centred frame of PaMO.run (as remesh
+ * returns them, pamo/__init__.py:87), and faces, into device buffers of the
+ * sizes returned. */
+int pamo_s1_get(pamo_s1* s1, float* d_verts, int* d_faces);
+/* Test access (no counterpart in the reference). */
+#define PAMO_S1_DBG_FRAME 1       /* 8 floats: mean xyz, min xyz, max, level           */
+#define PAMO_S1_DBG_CASES 2       /* 256 x 18 int8: crossing edges in patch cycle order (12, -1 padded),
+                                     edges per patch (4), patches, triangles       */
+#define PAMO_S1_DBG_FIELD 3       /* resolution^3 uint32: distance bits (+inf beyond the band) */
+#define PAMO_S1_DBG_OUTSIDE 4     /* resolution^3 uint8: 1 for the outside              */
+#define PAMO_S1_DBG_NORMALIZED 5  /* n_faces x 3 x 4 float: normalised corners (w = 0)   */
+#define PAMO_S1_DBG_VERTS 6       /* n_verts x 4 float: vertices in sample index units   */
+int pamo_s1_debug(pamo_s1* s1, int what, void* h_out, long long cap, long long* n_out);
+
 /* ---------------------------------------------------------------- batched and served forms
  * No counterpart in the reference, which runs one mesh per PaMO.run call
  * (simp_cuda/pamo/__init__.py:122-205).  Each job is that whole call --

@@ -56,6 +56,12 @@ PAMO_S3_DBG_CCD = 8
 PAMO_S3_DBG_HINGE = 9
 PAMO_S3_DBG_RESOLVE = 10
 PAMO_S3_OPT_CG_GRAPH = 1
+PAMO_S1_DBG_FRAME = 1
+PAMO_S1_DBG_CASES = 2
+PAMO_S1_DBG_FIELD = 3
+PAMO_S1_DBG_OUTSIDE = 4
+PAMO_S1_DBG_NORMALIZED = 5
+PAMO_S1_DBG_VERTS = 6
 
 
 class S3Params(ctypes.Structure):
@@ -80,6 +86,12 @@ SIGNATURES = [
     ("pamo_round", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_float, c_float, c_int, c_int,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.POINTER(c_int)]),
     ("pamo_centroid", c_int, [c_vp, c_vp, c_int, c_vp, c_int, ctypes.POINTER(c_float)]),
+    ("pamo_s1_create", c_int, [ctypes.POINTER(c_vp), c_int]),
+    ("pamo_s1_destroy", c_int, [c_vp]),
+    ("pamo_s1_remesh", c_int, [c_vp, c_vp, c_int, c_vp, c_int, ctypes.POINTER(c_float), c_int, ctypes.POINTER(c_int),
+                               ctypes.POINTER(c_int)]),
+    ("pamo_s1_get", c_int, [c_vp, c_vp, c_vp]),
+    ("pamo_s1_debug", c_int, [c_vp, c_int, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_round_debug", c_int, [c_vp, c_int, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_simplify", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, c_int, c_float, c_ll, c_ll, c_vp, c_vp,
                               ctypes.POINTER(c_int), ctypes.POINTER(c_int), c_vp, c_int, ctypes.POINTER(c_int)]),

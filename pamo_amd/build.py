@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-HIP_SOURCES = ["stage2.hip", "stage3.hip", "serve.hip", "metrics.hip"]
+HIP_SOURCES = ["stage1.hip", "stage2.hip", "stage3.hip", "serve.hip", "metrics.hip"]
 
 
 def _run(cmd):

@@ -1,9 +1,10 @@
 """Command-line workflow of the reference's example.py: load an OBJ, run
 PaMO, write the result (`python -m pamo_amd.example -i in.obj -o out.obj`).
 
-Same options as example.py:13-21.  Stage 1 (remeshing) is not part of this
-library, so --disable_stage1 is required; the input must then be a closed
-manifold triangle mesh, as PaMO's stage 2 expects.
+Same options as example.py:13-21.  Without --disable_stage1 the input is
+remeshed first (pamo_amd.stage1), so open or non-manifold inputs work; with
+it the input must be a closed manifold triangle mesh, as PaMO's stage 2
+expects.
 """
 from __future__ import annotations
 
@@ -26,12 +27,11 @@ def main(argv=None):
     p.add_argument("--disable_stage3", action="store_true", help="Disable safe projection")
     p.add_argument("--exact_guard", action="store_true", help="Exact intersection guard")
     args = p.parse_args(argv)
-    if not args.disable_stage1:
-        p.error("stage 1 (remeshing) is not available here: pass --disable_stage1")
     V, F = load_obj(args.input)
     print(f"# of input verts : {len(V)}")
     print(f"# of input faces : {len(F)}")
-    pamo = PaMO((V, F), use_stage1=False, use_stage3=not args.disable_stage3, exact_guard=args.exact_guard)
+    pamo = PaMO((V, F), use_stage1=not args.disable_stage1, use_stage3=not args.disable_stage3,
+                exact_guard=args.exact_guard)
     start = time.time()
     verts, faces = pamo.run(torch.from_numpy(V).float().cuda(), torch.from_numpy(F).int().cuda(),
                             min_verts=args.min_vertex, ratio=args.ratio)

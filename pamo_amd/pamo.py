@@ -170,7 +170,10 @@ class PaMO(torch.nn.Module):
 
     ``input_mesh`` is a trimesh-like object (``vertices``, ``faces`` and,
     optionally, ``bounding_box.bounds``) or a ``(vertices, faces)`` pair.
-    Stage 1 (SDF remeshing) is not part of this package: asking for it raises.
+    Stage 1 (``use_stage1=True``, the reference's default) remeshes the input
+    as the iso-surface of an offset distance field (``pamo_amd.stage1``) at
+    the reference's resolution rule, and stage 2 then simplifies the
+    remeshed surface towards the input's target face count.
     ``exact_guard=True`` (not in the reference's signature) replaces the
     reference's guard predicate by the exact intersection test, so that no
     collapse that makes two faces intersect survives; the default keeps the
@@ -186,10 +189,6 @@ class PaMO(torch.nn.Module):
     def __init__(self, input_mesh, use_stage1: bool = True, use_stage3: bool = True, exact_guard: bool = False,
                  placement: str = "midpoint", tuning: dict | None = None):
         super().__init__()
-        if use_stage1:
-            raise NotImplementedError(
-                "stage-1 remeshing (SDF + dual marching cubes) is not provided by pamo_amd; "
-                "construct PaMO(..., use_stage1=False)")
         self.use_stage1 = use_stage1
         self.use_stage3 = use_stage3
         if isinstance(input_mesh, (tuple, list)):
@@ -222,6 +221,14 @@ class PaMO(torch.nn.Module):
             self.s3_target = stage3.normalized_target(self.gt_vertices)
             self.gt_v_dev = torch.as_tensor(self.gt_vertices, dtype=torch.float32).cuda()
             self.gt_f_dev = torch.as_tensor(self.gt_faces, dtype=torch.int32).cuda()
+        if use_stage1:
+            # the remeshing context (pamo/__init__.py:45-50: mesh2vol and
+            # vol2mesh set up once per PaMO object)
+            from .stage1 import Remesher
+            self.remesher = Remesher(torch.cuda.current_device())
+        self.R = 256
+        self.band = 3 / self.R
+        self.margin = self.band * 2 + 1
         self.target_faces = None
         self.timings = {}
 
@@ -229,11 +236,26 @@ class PaMO(torch.nn.Module):
         """pamo/__init__.py:122-205; returns numpy (vertices, faces)."""
         _check_mesh_types(points, triangles)
         self.target_faces = max(int(ratio * len(triangles)), min_verts)
+        self.timings = {}
         t0 = time.perf_counter()
         tris_mean = face_centroid_mean(self.ctx, points, triangles)
-        verts = points - torch.from_numpy(tris_mean).to(points.device)
-        v2, f2, hist = simplify_device(self.ctx, verts.contiguous(), triangles, ratio, tolerance, threshold, iter,
-                                       min_verts)
+        if self.use_stage1:
+            # pamo/__init__.py:130-143: remesh at a resolution chosen by the
+            # target, then stage 2 on the remeshed (centred) mesh towards
+            # the target of the input face count
+            from .stage1 import resolution_for
+            self.R = resolution_for(self.target_faces)
+            self.band = 3 / self.R
+            self.margin = self.band * 2 + 1
+            v1, f1 = self.remesher.remesh(points, triangles, tris_mean, self.R)
+            torch.cuda.synchronize()
+            self.timings["stage1_s"] = time.perf_counter() - t0
+            self.stage1_out = (v1, f1)
+            v2, f2, hist = simplify_device(self.ctx, v1, f1, 0.0, tolerance, threshold, iter, self.target_faces)
+        else:
+            verts = points - torch.from_numpy(tris_mean).to(points.device)
+            v2, f2, hist = simplify_device(self.ctx, verts.contiguous(), triangles, ratio, tolerance, threshold, iter,
+                                           min_verts)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         self.rounds = len(hist)
@@ -241,7 +263,7 @@ class PaMO(torch.nn.Module):
         # the stage-2 result (device, centred) and the centring offset
         self.stage2_out = (v2, f2)
         self.tris_mean = tris_mean
-        self.timings = {"stage2_s": t1 - t0}
+        self.timings["stage2_s"] = t1 - t0 - self.timings.get("stage1_s", 0.0)
         if self.use_stage3:
             # pamo/__init__.py:191-203: the stage-2 result (+ centring offset,
             # a float32 add) is projected back onto the input surface
