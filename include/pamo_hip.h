@@ -264,6 +264,19 @@ int pamo_s3_normalization(pamo_s3* s3, const float* d_v, int n_v, double* scale,
 int pamo_s3_set_mesh_device(pamo_s3* s3, const float* d_v, int n_v, const int* d_f, int n_f, double scale,
                             const double* t3);
 
+/* The stage-2 -> stage-3 handoff trimesh.Trimesh(vertices, faces) with its
+ * default process=True (pamo/__init__.py:194; trimesh 4.4.0
+ * Trimesh.process -> grouping.merge_vertices): vertices referenced by a face
+ * whose coordinates times 10^8, rounded half to even, are equal merge into
+ * the first of them; the kept vertices, in index order, go to d_v_out
+ * (float32 (n_v,3), not aliasing d_v) and *out_n_v receives their count;
+ * the int32 (n_f,3) faces d_f are re-indexed in place.  Unreferenced
+ * vertices are dropped.  A non-finite referenced vertex is an error (trimesh
+ * would drop it and point its faces at vertex 0).  Runs on the stage-3
+ * context's stream and returns when done. */
+int pamo_s3_merge_vertices(pamo_s3* s3, const float* d_v, int n_v, int* d_f, int n_f, float* d_v_out,
+                           int* out_n_v);
+
 /* n_steps calls of Stage3System.step (system.py:411-470): closest-point
  * targets, then n_newton_iters of {contacts, gradient, Jacobi PCG, clamp,
  * ACCD, halving line search}. */

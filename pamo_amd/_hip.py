@@ -117,6 +117,7 @@ SIGNATURES = [
                                      ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_double)]),
     ("pamo_s3_normalization", c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     ("pamo_s3_set_mesh_device", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double)]),
+    ("pamo_s3_merge_vertices", c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
     ("pamo_batch_run", c_int, [c_int, c_int, ctypes.POINTER(S3Params), c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                c_int, c_float, c_ll, c_ll, c_int, c_vp, c_vp, c_vp, c_vp,
                                ctypes.POINTER(c_double), c_vp, c_vp]),

@@ -187,7 +187,9 @@ static int run_job(Engine& e, Slot& s, Job& j) {
         double scale = 1.0, t[3];
         SV_TRY(pamo_s3_normalization(s.c3, j.dv, j.nv, &scale, t));
         SV_TRY(pamo_s3_set_target(s.c3, j.dv, j.nv, j.df, j.nf, scale, t));
-        SV_TRY(pamo_s3_set_mesh_device(s.c3, s.vbuf, onv, s.fbuf, onf, scale, t));
+        // trimesh.Trimesh(process=True)'s vertex merge of the stage-2 result
+        SV_TRY(pamo_s3_merge_vertices(s.c3, s.vbuf, onv, s.fbuf, onf, s.obuf, &onv));
+        SV_TRY(pamo_s3_set_mesh_device(s.c3, s.obuf, onv, s.fbuf, onf, scale, t));
         SV_TRY(pamo_s3_step(s.c3, 5));
         SV_TRY(pamo_s3_get_vertices(s.c3, j.out_v, scale, t));
     }

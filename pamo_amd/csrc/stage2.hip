@@ -157,6 +157,8 @@ struct Ctx {
     int edge_lanes = 4;         // lanes per edge in the edge-cost kernel (1, 4 or 8)
     int ring_lanes = 8;         // lanes per item in the quadric, selection and undo kernels (1, 4 or 8)
     float* cent_out = nullptr;  // pamo_centroid's result (3 floats)
+    float* cent_means = nullptr;  // pamo_centroid's face means, 3 rows of cent_pitch
+    long long cap_cent = 0;
     int exact_guard = 0;      // 0 reference predicate, 1 exact, 2 exact with every pair in the exact pass
     int2* ulist = nullptr;    // exact guard: pairs left undecided by the filtered predicates
     size_t alloc_ulist = 0;
@@ -338,7 +340,7 @@ static void free_all(Ctx& c) {
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
                     c.blksuf, c.sblk, c.blktab, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
-                    c.shuf, c.ws, c.dcnt, c.cent_out, c.ulist};
+                    c.shuf, c.ws, c.dcnt, c.cent_out, c.ulist, c.cent_means};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     lookback_free(c.lb);
@@ -953,8 +955,15 @@ int pamo_centroid(pamo_ctx* ctx, const float* d_pts, int n_pts, const int* d_tri
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     PAMO_HIP(hipSetDevice(c.device));
     if (!c.cent_out) PAMO_TRY(pamo::dalloc(&c.cent_out, 4, c.stream));
-    hipLaunchKernelGGL(pamo::centroid_seq_k, dim3(1), dim3(pamo::kCentThreads), 0, c.stream, d_pts, d_tris, n_tris,
-                       c.cent_out);
+    const long long pitch = ((long long)n_tris + 3) & ~3LL;
+    if (3 * pitch > c.cap_cent) {
+        c.cap_cent = 3 * pitch + 3 * pitch / 4;
+        PAMO_TRY(pamo::dalloc(&c.cent_means, (size_t)c.cap_cent, c.stream));
+    }
+    hipLaunchKernelGGL(pamo::face_mean_k, dim3(pamo::grid_for(n_tris, 256)), dim3(256), 0, c.stream, d_pts, d_tris,
+                       n_tris, pitch, c.cent_means);
+    hipLaunchKernelGGL(pamo::centroid_seq_k, dim3(1), dim3(pamo::kCentThreads), 0, c.stream, c.cent_means, pitch,
+                       n_tris, c.cent_out);
     PAMO_HIP(hipGetLastError());
     float h[3];
     PAMO_HIP(hipMemcpyAsync(h, c.cent_out, sizeof(h), hipMemcpyDeviceToHost, c.stream));

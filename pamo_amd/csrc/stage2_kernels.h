@@ -1886,25 +1886,42 @@ __global__ void store_tris_k(Mesh m, int n, int* __restrict__ aos) {
 // float64 loop; a float64 quotient rounded to float32 is the correctly
 // rounded float32 quotient, since 53 >= 2 * 24 + 2); the mean over faces is
 // a sequential float32 running sum from the first face's value, in face
-// order, divided the same way.  A float32 running sum cannot be split, so
-// one workgroup runs it: waves 1..15 write the face means of the next chunk
-// to LDS while lanes 0..2 of wave 0 add up the current chunk, one coordinate
-// per lane, reading 32 values ahead of the additions so that the chain of
-// dependent adds is not held up by LDS latency.
+// order, divided the same way.  The face means come from a launch over the
+// whole GPU (face_mean_k, rows of `pitch` floats per coordinate).  A float32
+// running sum cannot be split, so one workgroup adds them up: waves 1..15
+// copy the next chunk of means to LDS while lanes 0..2 of wave 0 add up the
+// current chunk, one coordinate per lane, reading 32 values ahead of the
+// additions so that the chain of dependent adds is not held up by LDS
+// latency.
 constexpr int kCentThreads = 1024;
 constexpr int kCentChunk = 2048;            // multiple of 32
 constexpr int kCentPitch = kCentChunk + 4;  // rows start in different LDS banks
 
-__device__ inline void centroid_fill(const float* __restrict__ pts, const int* __restrict__ tris, long long base,
-                                     int cnt, float* __restrict__ dst, int t, int nt) {
-    for (int j = t; j < cnt; j += nt) {
-        const long long f = base + j;
-        const int i0 = tris[3 * f], i1 = tris[3 * f + 1], i2 = tris[3 * f + 2];
+__global__ void face_mean_k(const float* __restrict__ pts, const int* __restrict__ tris, int nF, long long pitch,
+                            float* __restrict__ means) {
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const int i0 = tris[3 * f], i1 = tris[3 * f + 1], i2 = tris[3 * f + 2];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
-            dst[d * kCentPitch + j] = s / 3.0f;
-        }
+    for (int d = 0; d < 3; ++d) {
+        const float s = (pts[3 * (long long)i0 + d] + pts[3 * (long long)i1 + d]) + pts[3 * (long long)i2 + d];
+        means[d * pitch + f] = s / 3.0f;
+    }
+}
+
+// means[d * pitch + base ...] -> dst rows; base and pitch are multiples of 4
+__device__ inline void centroid_fill(const float* __restrict__ means, long long pitch, long long base, int cnt,
+                                     float* __restrict__ dst, int t, int nt) {
+    const int n4 = cnt >> 2;
+    for (int j = t; j < 3 * n4; j += nt) {
+        const int d = j / n4, q = j - d * n4;
+        const float4 v = *reinterpret_cast<const float4*>(means + d * pitch + base + 4 * q);
+        *reinterpret_cast<float4*>(dst + d * kCentPitch + 4 * q) = v;
+    }
+    const int tail = cnt - 4 * n4;
+    if (t < 3 * tail) {
+        const int d = t / tail, q = 4 * n4 + t - d * tail;
+        dst[d * kCentPitch + q] = means[d * pitch + base + q];
     }
 }
 
@@ -1936,13 +1953,12 @@ __device__ __forceinline__ float centroid_chain(float acc, const float* __restri
     return acc;
 }
 
-__global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __restrict__ pts,
-                                                               const int* __restrict__ tris, int nF,
-                                                               float* __restrict__ out3) {
+__global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __restrict__ means, long long pitch,
+                                                               int nF, float* __restrict__ out3) {
     __shared__ __align__(16) float buf[2][3 * kCentPitch];
     const int t = threadIdx.x;
     const int nchunks = (nF + kCentChunk - 1) / kCentChunk;
-    centroid_fill(pts, tris, 0, nF < kCentChunk ? nF : kCentChunk, buf[0], t, kCentThreads);
+    centroid_fill(means, pitch, 0, nF < kCentChunk ? nF : kCentChunk, buf[0], t, kCentThreads);
     __syncthreads();
     float acc = 0.0f;
     for (int k = 0; k < nchunks; ++k) {
@@ -1952,7 +1968,7 @@ __global__ void __launch_bounds__(kCentThreads) centroid_seq_k(const float* __re
             const long long nb = base + kCentChunk;
             if (nb < nF) {
                 const int ncnt = (int)((nF - nb) < kCentChunk ? (nF - nb) : kCentChunk);
-                centroid_fill(pts, tris, nb, ncnt, buf[(k + 1) & 1], t - kWave, kCentThreads - kWave);
+                centroid_fill(means, pitch, nb, ncnt, buf[(k + 1) & 1], t - kWave, kCentThreads - kWave);
             }
         } else if (t < 3) {
             const float* row = buf[k & 1] + t * kCentPitch;

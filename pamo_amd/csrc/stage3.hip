@@ -117,6 +117,11 @@ struct Ctx {
     int cg_graph = 0;
     hipGraphExec_t cg_exec = nullptr;
     double last_energy = 0.0;
+    // vertex merge at the stage-2 -> stage-3 handoff: referenced flags, group
+    // first vertex, kept flags and their prefix sum (n + 1 each), hash table
+    int *mg_ref = nullptr, *mg_first = nullptr, *mg_keep = nullptr, *mg_pos = nullptr, *mg_tab = nullptr;
+    int* mg_err = nullptr;
+    long long cap_mg = 0, cap_mg_tab = 0;
 };
 
 enum { SC_E = 0, SC_EPREV = 1, SC_VAV = 2, SC_ZR = 3, SC_ZRLAST = 4, SC_N = 8 };
@@ -128,7 +133,8 @@ static void ctx_free(Ctx& c) {
                     c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
                     c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.sA, c.skey, c.sval, c.skey2,
                     c.vf_dst, c.vh_dst, c.cdv, c.cj,
-                    c.sval2, c.zr_part, c.vav_part};
+                    c.sval2, c.zr_part, c.vav_part,
+                    c.mg_ref, c.mg_first, c.mg_keep, c.mg_pos, c.mg_tab, c.mg_err};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcount) (void)hipHostFree(c.hcount);
@@ -1695,6 +1701,131 @@ __global__ void __launch_bounds__(kB) gt_stats_k(const float* __restrict__ v, in
     }
 }
 
+// ------------------------------------------------------------ handoff vertex merge
+// trimesh 4.4.0 builds the stage-3 input with Trimesh(vertices, faces) and its
+// default process=True (pamo/__init__.py:194), which merges vertices
+// (trimesh.grouping.merge_vertices): each vertex is keyed by its float64
+// coordinates times 10^8 (tol.merge = 1e-8) rounded half to even to int64;
+// among the vertices a face references (all of them when there are no
+// faces), vertices with equal keys become the first of them in index order
+// (unique_rows(..., keep_order=True)); the kept vertices stay in index
+// order, with their own coordinates, and faces are re-indexed.
+//
+// The groups come from an open-addressing table of vertex indices: a vertex
+// claims the first slot on its probe path that is empty or holds its key and
+// lowers it to its own index; slots never change key once claimed, so all
+// vertices of one key meet in one slot whatever the order of the inserts, and
+// the slot ends with the smallest index of the group.
+
+__device__ __forceinline__ bool merge_key(const float* __restrict__ v, int i, long long k[3]) {
+    bool finite = true;
+    for (int c = 0; c < 3; ++c) {
+        const float x = v[3 * (long long)i + c];
+        finite = finite && isfinite(x);
+        k[c] = finite ? (long long)rint((double)x * 1e8) : 0;
+    }
+    return finite;
+}
+
+__device__ __forceinline__ uint32_t merge_hash(const long long k[3]) {
+    uint64_t h = 0x243F6A8885A308D3ull;
+    for (int c = 0; c < 3; ++c) {
+        h = (h ^ (uint64_t)k[c]) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
+    }
+    h *= 0xBF58476D1CE4E5B9ull;
+    return (uint32_t)(h >> 32);
+}
+
+// err bit 0: a face index outside [0, n_v); bit 1: a referenced vertex is not
+// finite; bit 2: a probe path exceeded the table (cannot happen at load 1/2)
+__global__ void merge_mark_k(const int* __restrict__ f, long long n3, int n_v, int* __restrict__ ref,
+                             int* __restrict__ err) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const int x = f[i];
+    if (x < 0 || x >= n_v)
+        atomicOr(err, 1);
+    else
+        ref[x] = 1;
+}
+
+__global__ void merge_insert_k(const float* __restrict__ v, int n_v, const int* __restrict__ ref, int* tab,
+                               uint32_t mask, int* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_v || !ref[i]) return;
+    long long k[3];
+    if (!merge_key(v, i, k)) {
+        atomicOr(err, 2);
+        return;
+    }
+    uint32_t h = merge_hash(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int cur = atomicCAS(&tab[h], -1, i);
+        if (cur == -1) return;
+        long long q[3];
+        merge_key(v, cur, q);
+        if (q[0] == k[0] && q[1] == k[1] && q[2] == k[2]) {
+            atomicMin(&tab[h], i);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    atomicOr(err, 4);
+}
+
+// first[i]: the group's first vertex (-1 if i is not kept in any group);
+// keep[i] = 1 for the first vertex of each group; keep[n_v] = 0 for the scan
+__global__ void merge_first_k(const float* __restrict__ v, int n_v, const int* __restrict__ ref,
+                              const int* __restrict__ tab, uint32_t mask, int* __restrict__ first,
+                              int* __restrict__ keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_v) return;
+    if (i == n_v) {
+        keep[i] = 0;
+        return;
+    }
+    first[i] = -1;
+    keep[i] = 0;
+    long long k[3];
+    if (!ref[i] || !merge_key(v, i, k)) return;
+    uint32_t h = merge_hash(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int cur = tab[h];
+        if (cur < 0) return;
+        long long q[3];
+        merge_key(v, cur, q);
+        if (q[0] == k[0] && q[1] == k[1] && q[2] == k[2]) {
+            first[i] = cur;
+            keep[i] = cur == i;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+__global__ void merge_out_k(const float* __restrict__ v, int n_v, const int* __restrict__ keep,
+                            const int* __restrict__ pos, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_v || !keep[i]) return;
+    const long long o = 3LL * pos[i], s = 3LL * i;
+    out[o] = v[s];
+    out[o + 1] = v[s + 1];
+    out[o + 2] = v[s + 2];
+}
+
+// (indices the mark pass flagged, and vertices without a group, which the
+// error bits report, are left as they are)
+__global__ void merge_faces_k(int* __restrict__ f, long long n3, int n_v, const int* __restrict__ first,
+                              const int* __restrict__ pos) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const int x = f[i];
+    if (x < 0 || x >= n_v) return;
+    const int g = first[x];
+    if (g >= 0) f[i] = pos[g];
+}
+
 }  // namespace s3
 }  // namespace pamo
 
@@ -2164,6 +2295,66 @@ int pamo_s3_set_mesh_device(pamo_s3* s, const float* d_v, int n_v, const int* d_
     const int do_nothing = hp.min_angle_deg <= c.min_angle_thres ? 1 : 0;
     return pamo_s3_set_mesh(s, d_v, n_v, d_f, n_f, scale, t3, hp.edges.data(), n_e,
                             n_h ? hp.hinges.data() : nullptr, n_h, hp.vor.data(), do_nothing);
+}
+
+int pamo_s3_merge_vertices(pamo_s3* s, const float* d_v, int n_v, int* d_f, int n_f, float* d_v_out,
+                           int* out_n_v) {
+    pamo::clear_error();
+    PAMO_CHECK(s && out_n_v && n_v >= 0 && n_f >= 0, "pamo_s3_merge_vertices: bad argument");
+    PAMO_CHECK(n_v == 0 || (d_v && d_v_out), "pamo_s3_merge_vertices: null vertex array");
+    PAMO_CHECK(n_f == 0 || d_f, "pamo_s3_merge_vertices: null face array");
+    PAMO_CHECK(n_v == 0 || d_v_out != d_v, "pamo_s3_merge_vertices: the output must not alias the input");
+    Ctx& c = *reinterpret_cast<Ctx*>(s);
+    *out_n_v = 0;
+    if (n_v == 0) return 0;  // an empty mesh is left as it is
+    PAMO_HIP(hipSetDevice(c.device));
+    using pamo::s3::dalloc;
+    if (n_v + 1 > c.cap_mg) {
+        c.cap_mg = n_v + 1 + n_v / 4 + 64;
+        PAMO_TRY(dalloc(&c.mg_ref, (size_t)c.cap_mg, c.stream));
+        PAMO_TRY(dalloc(&c.mg_first, (size_t)c.cap_mg, c.stream));
+        PAMO_TRY(dalloc(&c.mg_keep, (size_t)c.cap_mg, c.stream));
+        PAMO_TRY(dalloc(&c.mg_pos, (size_t)c.cap_mg, c.stream));
+    }
+    long long tab = 256;
+    while (tab < 2LL * n_v) tab <<= 1;
+    if (tab > c.cap_mg_tab) {
+        c.cap_mg_tab = tab;
+        PAMO_TRY(dalloc(&c.mg_tab, (size_t)tab, c.stream));
+    }
+    if (!c.mg_err) PAMO_TRY(dalloc(&c.mg_err, 1, c.stream));
+    PAMO_TRY(pamo::s3::ensure_ws(c, (long long)n_v + 64));
+    const long long n3 = 3LL * n_f;
+    // without faces every vertex counts as referenced (merge_vertices'
+    // branch for geometry without faces)
+    PAMO_TRY(pamo::fill(c.mg_ref, n_f > 0 ? 0 : 1, n_v, c.stream));
+    PAMO_TRY(pamo::fill(c.mg_tab, -1, tab, c.stream));
+    PAMO_TRY(pamo::fill(c.mg_err, 0, 1, c.stream));
+    if (n3 > 0)
+        hipLaunchKernelGGL(pamo::s3::merge_mark_k, dim3(pamo::grid_for(n3, 256)), dim3(256), 0, c.stream, d_f, n3,
+                           n_v, c.mg_ref, c.mg_err);
+    const uint32_t mask = (uint32_t)(tab - 1);
+    hipLaunchKernelGGL(pamo::s3::merge_insert_k, dim3(pamo::grid_for(n_v, 256)), dim3(256), 0, c.stream, d_v, n_v,
+                       c.mg_ref, c.mg_tab, mask, c.mg_err);
+    hipLaunchKernelGGL(pamo::s3::merge_first_k, dim3(pamo::grid_for(n_v + 1LL, 256)), dim3(256), 0, c.stream, d_v,
+                       n_v, c.mg_ref, c.mg_tab, mask, c.mg_first, c.mg_keep);
+    PAMO_HIP(hipGetLastError());
+    PAMO_TRY(pamo::scan_exclusive(c.mg_keep, c.mg_pos, n_v + 1LL, c.ws, c.stream));
+    hipLaunchKernelGGL(pamo::s3::merge_out_k, dim3(pamo::grid_for(n_v, 256)), dim3(256), 0, c.stream, d_v, n_v,
+                       c.mg_keep, c.mg_pos, d_v_out);
+    if (n3 > 0)
+        hipLaunchKernelGGL(pamo::s3::merge_faces_k, dim3(pamo::grid_for(n3, 256)), dim3(256), 0, c.stream, d_f, n3,
+                           n_v, c.mg_first, c.mg_pos);
+    PAMO_HIP(hipGetLastError());
+    int h[2] = {0, 0};
+    PAMO_HIP(hipMemcpyAsync(&h[0], c.mg_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipMemcpyAsync(&h[1], c.mg_pos + n_v, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PAMO_HIP(hipStreamSynchronize(c.stream));
+    PAMO_CHECK(!(h[0] & 1), "pamo_s3_merge_vertices: a face index is out of range");
+    PAMO_CHECK(!(h[0] & 2), "pamo_s3_merge_vertices: a referenced vertex is not finite");
+    PAMO_CHECK(!(h[0] & 4), "pamo_s3_merge_vertices: the vertex table overflowed");
+    *out_n_v = h[1];
+    return 0;
 }
 
 int pamo_s3_set_option(pamo_s3* s, int option, long long value) {

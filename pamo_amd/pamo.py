@@ -266,10 +266,13 @@ class PaMO(torch.nn.Module):
         self.timings["stage2_s"] = t1 - t0 - self.timings.get("stage1_s", 0.0)
         if self.use_stage3:
             # pamo/__init__.py:191-203: the stage-2 result (+ centring offset,
-            # a float32 add) is projected back onto the input surface
+            # a float32 add) goes through trimesh.Trimesh(process=True)'s
+            # vertex merge and is projected back onto the input surface
             from . import stage3
             v3 = v2 + torch.from_numpy(tris_mean).to(v2.device)
-            verts_np, faces_np = stage3.process(self.gt_v_dev, self.gt_f_dev, v3, f2, 5, system=self.system,
+            v3, f3 = self.system.merge_vertices(v3, f2)
+            self.stage3_in = (v3, f3)
+            verts_np, faces_np = stage3.process(self.gt_v_dev, self.gt_f_dev, v3, f3, 5, system=self.system,
                                                 config=self.config, target=self.s3_target)
             self.timings["stage3_s"] = time.perf_counter() - t1
             return verts_np, faces_np.astype(np.int64)

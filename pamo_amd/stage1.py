@@ -4,8 +4,9 @@ device through pamo_s1_* of libpamo_hip.so.
 The input triangles are normalised into the unit cube as preprocess_mesh
 does, a narrow-band distance field on a resolution^3 grid is signed by a
 flood fill of the outside from the grid border, and the level 0.9 /
-resolution is extracted by dual marching cubes, each quad split by the
-reference's angle rule.  The vertices come back in the centred frame of
+resolution is extracted by marching cubes with one consistent rule for
+ambiguous faces (each inside corner cut off on its own), which keeps the
+surface closed, manifold and oriented.  The vertices come back in the centred frame of
 PaMO.run (input minus the numpy float32 face-centroid mean), the frame in
 which stage 2 runs.
 """

@@ -247,6 +247,23 @@ class Stage3System:
         self.do_nothing = st["do_nothing"]
         self.scale, self.t = float(scale), np.asarray(t, dtype=np.float64)
 
+    def merge_vertices(self, V, F):
+        """trimesh.Trimesh(vertices=V, faces=F) with its default
+        process=True, as PaMO.run builds the stage-3 input
+        (pamo/__init__.py:194; trimesh 4.4.0 merge_vertices): referenced
+        vertices whose coordinates times 1e8, rounded half to even, agree
+        merge into the first of them; unreferenced vertices are dropped.
+        Returns device (float32 vertices, int32 faces)."""
+        v = _dev_tensor(V, torch.float32)
+        f = _dev_tensor(F, torch.int32).clone()
+        out = torch.empty_like(v)
+        n = ctypes.c_int(0)
+        torch.cuda.current_stream().synchronize()
+        _hip.check(_hip.lib().pamo_s3_merge_vertices(self.ptr, v.data_ptr(), int(v.shape[0]), f.data_ptr(),
+                                                     int(f.shape[0]), out.data_ptr(), ctypes.byref(n)),
+                   "pamo_s3_merge_vertices")
+        return out[: n.value], f
+
     def set_mesh_prepared(self, V, F, E, H, vor, do_nothing, scale: float, t):
         """set_mesh with the connectivity given (tests)."""
         v = _dev_tensor(V, torch.float32)
