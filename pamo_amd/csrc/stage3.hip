@@ -15,9 +15,11 @@
 //
 // State is FP32 (as the reference); the total energy and every CG dot
 // product are reduced in FP64 in a fixed two-level order (per-block partial
-// sums, then one block), so energies, step sizes and line-search decisions
-// do not depend on scheduling.  Gradient / Hessian-vector scatters use FP32
-// atomics like the reference.
+// sums, then one block).  Contacts are listed in an order fixed by the
+// trees and positions, and the gradient, Jacobi diagonal and Hessian-vector
+// products are summed per vertex over per-element slots in a fixed order
+// (the reference scatters them with FP32 atomics), so a step's result does
+// not depend on scheduling.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -73,13 +75,15 @@ struct Ctx {
     // inverse incidence: position in the vertex lists of face slot 3 f + k /
     // hinge slot 4 h + a, so element contributions land in list order
     int *vf_dst = nullptr, *vh_dst = nullptr;
-    float4* cdv = nullptr;  // active-contact corner gradients in vertex-list order
-    int* cj = nullptr;      // their active-contact index
+    // gradient-diagonal terms per hinge corner and per active-contact corner
+    // (vertex-list order) for the fixed-order per-vertex sums of compute_diff
+    float4 *hdg = nullptr, *cdg = nullptr;
     float4 *fc = nullptr, *hc = nullptr;
     float* dn = nullptr;
     // active contacts (d < d_hat) of the current Newton iteration
     int *act = nullptr, *act_flag = nullptr, *act_pos = nullptr, *n_act = nullptr;
-    float* sA = nullptr;
+    float4* cslot = nullptr;  // per active-contact corner H dx terms, vertex-list order
+    int* c_dst = nullptr;     // list position of active-contact corner slot 4 j + a
     int nA = 0;
     uint32_t *skey = nullptr, *sval = nullptr, *skey2 = nullptr, *sval2 = nullptr;
     long long cap_sort = 0;
@@ -94,6 +98,13 @@ struct Ctx {
     int capC = 0;
     int4* cidx = nullptr;
     int *ctype1 = nullptr, *ctype2 = nullptr;
+    // detection order: second contact arrays (swapped with cidx / ctype1
+    // after ordering), per-contact (wave, rank) tags, per-wave counts and bases
+    int4* cidx2 = nullptr;
+    int* ctype1b = nullptr;
+    int2* ctag = nullptr;
+    int *wave_count = nullptr, *wave_base = nullptr;
+    int cap_wave = 0;
     float* cd = nullptr;
     float4* cdd = nullptr;  // 4 per contact
     int* ccount = nullptr;
@@ -131,10 +142,11 @@ static void ctx_free(Ctx& c) {
                     c.edge, c.hinge, c.area, c.rest_angle, c.rest_elen, c.inv_dm, c.hblock, c.gt_v, c.gt_f,
                     c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
                     c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
-                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.sA, c.skey, c.sval, c.skey2,
-                    c.vf_dst, c.vh_dst, c.cdv, c.cj,
+                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.cslot, c.c_dst, c.skey, c.sval, c.skey2,
+                    c.vf_dst, c.vh_dst, c.hdg, c.cdg,
                     c.sval2, c.zr_part, c.vav_part,
-                    c.mg_ref, c.mg_first, c.mg_keep, c.mg_pos, c.mg_tab, c.mg_err};
+                    c.mg_ref, c.mg_first, c.mg_keep, c.mg_pos, c.mg_tab, c.mg_err,
+                    c.cidx2, c.ctype1b, c.ctag, c.wave_count, c.wave_base};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c.hcount) (void)hipHostFree(c.hcount);
@@ -344,14 +356,56 @@ __device__ __forceinline__ bool det_pair(const float4* __restrict__ x, const int
     return ee_distance(x0, x1, x2, x3, ty) <= thr;
 }
 
+// Output of the detection: contacts go to raw slots reserved with one atomic
+// per emission step, each tagged with (wave, rank in the wave's emission
+// order); the wave's count goes to wave_count.  A wave's emission order
+// depends only on the trees and positions, so placing each contact at
+// (exclusive scan of the wave counts)[wave] + rank (contact_order_k) lists
+// the contacts in an order that does not depend on scheduling.
+struct DetOut {
+    int* counter;
+    int cap;
+    int4* cidx;
+    int* ctype1;
+    int2* ctag;
+    int* wave_count;
+    int wave0;  // global index of this launch's first wave
+};
+
+// The keeping lanes among the lanes active here emit in lane order; wcnt is
+// the wave's running count in LDS.
+__device__ __forceinline__ void det_emit(bool keep, int4 ids, int type, const DetOut& o, int wid, int* wcnt) {
+    const uint64_t km = __ballot(keep);
+    if (km == 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int leader = __ffsll((unsigned long long)km) - 1;
+    const int k = (int)__popcll(km);
+    int base = 0, before = 0;
+    if (lane == leader) {
+        base = atomicAdd(o.counter, k);
+        before = *wcnt;
+        *wcnt = before + k;
+    }
+    base = __shfl(base, leader);
+    before = __shfl(before, leader);
+    if (keep) {
+        const int r = lanes_below(km);
+        const int slot = base + r;
+        if (slot < o.cap) {
+            o.cidx[slot] = ids;
+            o.ctype1[slot] = type;
+            o.ctag[slot] = make_int2(wid, before + r);
+        }
+    }
+}
+
 // Stackless walk below internal node x (from its left child until x's
 // escape), classifying and appending in this lane.
 template <bool kPT>
 __device__ __forceinline__ void det_walk_subtree(const float4* __restrict__ x, const int4* __restrict__ tri,
                                               const int2* __restrict__ edge, const float4* __restrict__ node,
                                               int n_leaves, int cur, int stop, V3 lo, V3 hi, int q, float thr,
-                                              float ee_thres, int* counter, int cap, int4* __restrict__ cidx,
-                                              int* __restrict__ ctype1, int* err) {
+                                              float ee_thres, const DetOut& o, int wid, int* wcnt, int* err) {
     for (long long visits = 0; cur != stop; ++visits) {
         if (visits >= 2LL * n_leaves || cur < 0) {
             walk_overrun(err);
@@ -361,14 +415,9 @@ __device__ __forceinline__ void det_walk_subtree(const float4* __restrict__ x, c
         const int lw = __float_as_int(blo.w), esc = __float_as_int(bhi.w);
         if (box_hit(lo, hi, blo, bhi)) {
             if (cur >= n_leaves - 1) {
-                int4 ids;
-                if ((kPT || lw > q) && det_pair<kPT>(x, tri, edge, q, lw, thr, ee_thres, ids)) {
-                    const int slot = atomicAdd(counter, 1);
-                    if (slot < cap) {
-                        cidx[slot] = ids;
-                        ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
-                    }
-                }
+                int4 ids = make_int4(0, 0, 0, 0);
+                const bool keep = (kPT || lw > q) && det_pair<kPT>(x, tri, edge, q, lw, thr, ee_thres, ids);
+                det_emit(keep, ids, kPT ? BLOCK_PT : BLOCK_EE, o, wid, wcnt);
                 cur = esc;
             } else {
                 cur = lw;
@@ -394,15 +443,18 @@ template <bool kPT>
 __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int nq, const int4* __restrict__ tri,
                                                const int2* __restrict__ edge, const float4* __restrict__ node,
                                                const int* __restrict__ right, int n_leaves, float d_hat,
-                                               float radius, float ee_thres, int* counter, int cap,
-                                               int4* __restrict__ cidx, int* __restrict__ ctype1, int* err) {
+                                               float radius, float ee_thres, DetOut o, int* err) {
     constexpr int kW = kB / kWave;
     __shared__ int2 stk[kW][kDetStack];
     __shared__ int2 pq[kW][kDetQ];
     __shared__ float4 qbox[kW][kDetW][2];
+    __shared__ int wcnt[kW];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const long long q0 = ((long long)blockIdx.x * kW + w) * kDetW;
-    if (q0 >= nq || n_leaves <= 0) return;  // uniform per wave
+    const long long wl = (long long)blockIdx.x * kW + w;
+    const int wid = o.wave0 + (int)wl;
+    const long long q0 = wl * kDetW;
+    if (q0 >= nq || n_leaves <= 0) return;  // uniform per wave (such waves count 0: see contact_order)
+    if (lane == 0) wcnt[w] = 0;
     const int nl = nq - q0 < kDetW ? (int)(nq - q0) : kDetW;
     const float thr = 2.0f * radius + d_hat;
     if (lane < nl) {
@@ -431,23 +483,14 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
             const int2 e = pq[w][qn - take + lane];
             keep = det_pair<kPT>(x, tri, edge, (int)q0 + e.x, e.y, thr, ee_thres, ids);
         }
-        const uint64_t km = __ballot(keep);
-        int base = 0;
-        if (lane == 0 && km) base = atomicAdd(counter, (int)__popcll(km));
-        base = __shfl(base, 0);
-        if (keep) {
-            const int slot = base + lanes_below(km);
-            if (slot < cap) {
-                cidx[slot] = ids;
-                ctype1[slot] = kPT ? BLOCK_PT : BLOCK_EE;
-            }
-        }
+        det_emit(keep, ids, kPT ? BLOCK_PT : BLOCK_EE, o, wid, &wcnt[w]);
     };
     int size = nl, qn = 0;  // wave-uniform
     const long long max_steps = (long long)nl * (2LL * n_leaves - 1);
     for (long long steps = 0; size > 0; ++steps) {
         if (steps >= max_steps) {  // wave-uniform
             if (lane == 0) walk_overrun(err);
+            if (lane == 0) o.wave_count[wid] = 0;
             return;
         }
         const int take = size < kWave ? size : kWave;
@@ -490,7 +533,7 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
                 stk[w][size + 2 * below + 1] = make_int2(l, c1);
             } else {
                 det_walk_subtree<kPT>(x, tri, edge, node, n_leaves, c0, esc, lo, hi, (int)q0 + l, thr, ee_thres,
-                                      counter, cap, cidx, ctype1, err);
+                                      o, wid, &wcnt[w], err);
             }
         }
         size += 2 * (int)__popcll(__ballot(c0 >= 0 && fits));
@@ -502,6 +545,20 @@ __global__ void __launch_bounds__(kB) detect_k(const float4* __restrict__ x, int
         }
     }
     if (qn > 0) classify(qn, qn);
+    wave_sync();
+    if (lane == 0) o.wave_count[wid] = wcnt[w];
+}
+
+// Contact s moves to (wave base)[ctag.x] + ctag.y.
+__global__ void contact_order_k(int n, const int2* __restrict__ ctag, const int* __restrict__ wave_base,
+                                const int4* __restrict__ cidx, const int* __restrict__ ctype1,
+                                int4* __restrict__ cidx2, int* __restrict__ ctype1b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int2 t = ctag[s];
+    const int d = wave_base[t.x] + t.y;
+    cidx2[d] = cidx[s];
+    ctype1b[d] = ctype1[s];
 }
 
 // ------------------------------------------------------------ energies (B7)
@@ -592,11 +649,6 @@ __global__ void __launch_bounds__(kB) energy_k(const float4* __restrict__ x, Top
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__device__ __forceinline__ void atomic_add3(float4* a, int i, V3 v) {
-    atomicAdd(&a[i].x, v.x);
-    atomicAdd(&a[i].y, v.y);
-    atomicAdd(&a[i].z, v.z);
-}
 
 // Symmetric eigen-clamp of the 9x9 leading block (cyclic Jacobi), then rows /
 // columns of vertex 3 rebuilt by translation invariance (sum over a block
@@ -692,21 +744,27 @@ __device__ __forceinline__ M3 scl(M3 a, float s) { M3 r; for (int k = 0; k < 9; 
 __device__ __forceinline__ M3 sym(M3 a) { return add(a, tr(a)); }
 __device__ __forceinline__ void put(float* B, int a, int b, M3 x) { for (int k = 0; k < 9; ++k) B[9 * (4 * a + b) + k] = x.m[k]; }
 
-// Gradient (coefficient `coeff`), Jacobi diagonal and, for hinges, the
-// projected 12x12 Hessian blocks; for contacts the distance gradient.
-// Items as in energy_k.
+// Gradient (coefficient `coeff`) and Jacobi diagonal terms and, for hinges,
+// the projected 12x12 Hessian blocks; for contacts the distance gradient.
+// Items as in energy_k.  The M2GT terms are stored to grad / diag; the
+// terms of faces, hinges and active contacts go to their slots in the
+// vertex lists (f_dst, h_dst, c_dst), summed per vertex by diff_gather_k.
+struct DiffSlots {
+    const int *f_dst, *h_dst, *c_dst;
+    float4 *fc, *hc, *hdg, *cslot, *cdg;
+};
 __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF, int nH,
                                              int nC, float coeff, const float* __restrict__ cd,
                                              const int* __restrict__ ctype2, float4* __restrict__ grad,
                                              float4* __restrict__ diag, float* __restrict__ hblock,
                                              float* __restrict__ hsym, const int* __restrict__ act,
-                                             const int* __restrict__ n_act, float4* __restrict__ cdd) {
+                                             const int* __restrict__ n_act, float4* __restrict__ cdd, DiffSlots ds) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nV) {
         float w = tp.vor[i] * P.k_m2gt;
         V3 g = (v3(x[i]) - v3(tp.target[i])) * w;
-        atomic_add3(grad, (int)i, g * coeff);
-        atomic_add3(diag, (int)i, mk(w, w, w));
+        grad[i] = f4(g * coeff);
+        diag[i] = f4(mk(w, w, w));
     } else if ((i -= nV) < nF) {
         int4 f = tp.tri[i];
         Membrane m = membrane(v3(x[f.x]), v3(x[f.y]), v3(x[f.z]), tp.inv_dm[i]);
@@ -724,9 +782,9 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         V3 f1 = H[0] * m.b1 + H[2] * m.b2;
         V3 f2 = H[1] * m.b1 + H[3] * m.b2;
         V3 f0 = -f1 - f2;
-        atomic_add3(grad, f.x, f0);
-        atomic_add3(grad, f.y, f1);
-        atomic_add3(grad, f.z, f2);
+        ds.fc[ds.f_dst[3 * i]] = f4(f0);
+        ds.fc[ds.f_dst[3 * i + 1]] = f4(f1);
+        ds.fc[ds.f_dst[3 * i + 2]] = f4(f2);
     } else if ((i -= nF) < nH) {
         int4 k = tp.hinge[i];
         V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
@@ -749,10 +807,10 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         V3 g2 = n * (ca1 / h2) + n_ * (ca1_ / h2_);
         V3 g0 = n * (-1.0f / h0);
         V3 g3 = n_ * (-1.0f / h0_);
-        atomic_add3(grad, k.x, dE * g0 * coeff);
-        atomic_add3(grad, k.y, dE * g1 * coeff);
-        atomic_add3(grad, k.z, dE * g2 * coeff);
-        atomic_add3(grad, k.w, dE * g3 * coeff);
+        ds.hc[ds.h_dst[4 * i]] = f4(dE * g0 * coeff);
+        ds.hc[ds.h_dst[4 * i + 1]] = f4(dE * g1 * coeff);
+        ds.hc[ds.h_dst[4 * i + 2]] = f4(dE * g2 * coeff);
+        ds.hc[ds.h_dst[4 * i + 3]] = f4(dE * g3 * coeff);
         V3 m0 = normalize(cross(e0, n)), m1 = normalize(cross(e1, n)), m2 = normalize(cross(n, e2));
         V3 m0_ = normalize(cross(n_, e0)), m1_ = normalize(cross(n_, e1_)), m2_ = normalize(cross(e2_, n_));
         M3 M0 = outer(n, m0), M1 = outer(n, m1), M2 = outer(n, m2);
@@ -784,10 +842,9 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         put(B, 1, 2, b12); put(B, 2, 1, tr(b12));
         put(B, 0, 3, b03); put(B, 3, 0, tr(b03));
         // the Jacobi diagonal takes the blocks before projection (hinge_energy.py:255-256)
-        const int ids[4] = {k.x, k.y, k.z, k.w};
         for (int a = 0; a < 4; ++a) {
             const float* D = B + 9 * (4 * a + a);
-            atomic_add3(diag, ids[a], mk(D[0], D[4], D[8]));
+            ds.hdg[ds.h_dst[4 * i + a]] = f4(mk(D[0], D[4], D[8]));
         }
         spd_project_hinge(B, hsym + 45 * (long long)i);
     } else if ((i -= nH) < nC && i < n_act[0]) {
@@ -801,10 +858,15 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         V3 g[4];
         if (tp.ctype1[i] == BLOCK_PT) pt_pair_grad(x0, x1, x2, x3, ctype2[i], g);
         else ee_pair_grad(x0, x1, x2, x3, ctype2[i], g);
-        (void)d;
-        // the gradient and diagonal terms are gathered per vertex by
-        // contact_diff_gather_k once the contact incidence is built
-        for (int a = 0; a < 4; ++a) cdd[4 * j + a] = f4(g[a]);
+        // barrier' kappa coeff g and barrier'' kappa g*g per corner
+        const float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
+        const float c2 = barrier_d2(d, P.d_hat) * P.kappa;
+        for (int a = 0; a < 4; ++a) {
+            cdd[4 * j + a] = f4(g[a]);
+            const int e = ds.c_dst[4 * j + a];
+            ds.cslot[e] = f4(g[a] * c1);
+            ds.cdg[e] = f4(c2 * mk(g[a].x * g[a].x, g[a].y * g[a].y, g[a].z * g[a].z));
+        }
     }
 }
 
@@ -847,14 +909,16 @@ __device__ __forceinline__ V3 cg_dx(const CgVec& cv, float beta, int v) {
 
 // items: [0, nF) faces -> fc[f_dst[3f + k]]; [nF, +nH) hinges -> hc[h_dst[4h + a]]
 // (vertex-list order, so the gather reads them contiguously);
-// [.., +nA) active contacts -> sA[j] = c2 (g . dx); [.., +3 nV_dense) dense rows
+// [.., +nA) active contacts -> cslot[c_dst[4 j + a]] = g_a c2 (g . dx);
+// [.., +3 nV_dense) dense rows
 __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, Topo tp, Params P, int nF, int nH,
                                                  int nA_max, const int* __restrict__ n_act, int n_dense,
                                                  CgVec cv, const float* __restrict__ cd, const int* __restrict__ act,
                                                  const float* __restrict__ hsym, const float4* __restrict__ cdd,
                                                  const double* __restrict__ dA, const int* __restrict__ f_dst,
                                                  const int* __restrict__ h_dst, float4* __restrict__ fc,
-                                                 float4* __restrict__ hc, float* __restrict__ sA,
+                                                 float4* __restrict__ hc, const int* __restrict__ c_dst,
+                                                 float4* __restrict__ cslot,
                                                  float* __restrict__ dn) {
     __shared__ double red[kB];
     const float beta = cg_beta(cv, red);
@@ -926,9 +990,16 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
         const int c = act[i];
         const int4 k = tp.cidx[c];
         const float c2 = barrier_d2(cd[c], P.d_hat) * P.kappa;
-        float sdot = dot(v3(cdd[4 * i]), cg_dx(cv, beta, k.x)) + dot(v3(cdd[4 * i + 1]), cg_dx(cv, beta, k.y)) +
-                     dot(v3(cdd[4 * i + 2]), cg_dx(cv, beta, k.z)) + dot(v3(cdd[4 * i + 3]), cg_dx(cv, beta, k.w));
-        sA[i] = sdot * c2;
+        const V3 g0 = v3(cdd[4 * i]), g1 = v3(cdd[4 * i + 1]), g2 = v3(cdd[4 * i + 2]), g3 = v3(cdd[4 * i + 3]);
+        const float sdot = dot(g0, cg_dx(cv, beta, k.x)) + dot(g1, cg_dx(cv, beta, k.y)) +
+                           dot(g2, cg_dx(cv, beta, k.z)) + dot(g3, cg_dx(cv, beta, k.w));
+        // H_c dx = g (c2 g . dx): each corner's term goes to its slot in the
+        // vertex lists, so the gather reads it contiguously
+        const float sa = sdot * c2;
+        cslot[c_dst[4 * i]] = f4(g0 * sa);
+        cslot[c_dst[4 * i + 1]] = f4(g1 * sa);
+        cslot[c_dst[4 * i + 2]] = f4(g2 * sa);
+        cslot[c_dst[4 * i + 3]] = f4(g3 * sa);
     } else if ((i -= nA_max) < n_dense) {
         const int n3 = (int)n_dense;
         double acc = 0.0;
@@ -943,10 +1014,8 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
 
 struct Incid {
     const int* vf_off;  // vertex -> face corner entries of fc (list order)
-    const int* vh_off;  // vertex -> hinge corner entries of hc (list order)
-    const int* vc_off;  // vertex -> active contact corners: gradient cdv[e], contact cj[e]
-    const float4* cdv;
-    const int* cj;
+    const int* vh_off;  // vertex -> hinge corner entries of hc / hdg (list order)
+    const int* vc_off;  // vertex -> active contact corner entries of cslot / cdg (list order)
 };
 
 // Av = H dx per vertex; the direction dx is stored to v_out; block partials
@@ -956,25 +1025,33 @@ struct Incid {
 // whole wave.
 constexpr int kGather = 32;
 
-// Contact terms of the gradient (barrier' kappa coeff g) and of the Jacobi
-// diagonal (barrier'' kappa g*g) per vertex, summed over its active-contact
-// corners in the incidence order by kGather lanes and a fixed shuffle tree,
-// instead of float atomics per corner.  Runs after diff_k (same stream).
-__global__ void __launch_bounds__(kB) contact_diff_gather_k(int nV, Params P, float coeff, Incid ic,
-                                                            const float* __restrict__ cd, const int* __restrict__ act,
-                                                            float4* __restrict__ grad, float4* __restrict__ diag) {
+// Gradient and Jacobi diagonal per vertex: diff_k leaves the M2GT term in
+// grad / diag and every element's term in its slot of the vertex lists
+// (faces fc, hinges hc and hdg, active contacts cslot and cdg); kGather lanes
+// per vertex sum the slots in list order (faces, then hinges, then contacts,
+// strided over the group) and a fixed shuffle tree, so the result does not
+// depend on scheduling.
+__global__ void __launch_bounds__(kB) diff_gather_k(int nV, int use_elem, int use_hinge, int use_contact, Incid ic,
+                                                    const float4* __restrict__ fc, const float4* __restrict__ hc,
+                                                    const float4* __restrict__ hdg, const float4* __restrict__ cslot,
+                                                    const float4* __restrict__ cdg, float4* __restrict__ grad,
+                                                    float4* __restrict__ diag) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int v = (int)(t / kGather), g = (int)(t % kGather);
     V3 ga = mk(0.0f, 0.0f, 0.0f), da = ga;
     if (v < nV) {
-        for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
-            const float d = cd[act[ic.cj[e]]];
-            const float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
-            const float c2 = barrier_d2(d, P.d_hat) * P.kappa;
-            const V3 gg = v3(ic.cdv[e]);
-            ga = ga + gg * c1;
-            da = da + c2 * mk(gg.x * gg.x, gg.y * gg.y, gg.z * gg.z);
-        }
+        if (use_elem)
+            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) ga = ga + v3(fc[e]);
+        if (use_hinge)
+            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) {
+                ga = ga + v3(hc[e]);
+                da = da + v3(hdg[e]);
+            }
+        if (use_contact)
+            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
+                ga = ga + v3(cslot[e]);
+                da = da + v3(cdg[e]);
+            }
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
@@ -985,7 +1062,7 @@ __global__ void __launch_bounds__(kB) contact_diff_gather_k(int nV, Params P, fl
         da.y += __shfl_xor(da.y, m, kGather);
         da.z += __shfl_xor(da.z, m, kGather);
     }
-    if (v < nV && g == 0 && ic.vc_off[v + 1] > ic.vc_off[v]) {
+    if (v < nV && g == 0) {
         grad[v] = f4(v3(grad[v]) + ga);
         diag[v] = f4(v3(diag[v]) + da);
     }
@@ -994,7 +1071,7 @@ __global__ void __launch_bounds__(kB) contact_diff_gather_k(int nV, Params P, fl
 __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2gt, int use_elem, int use_hinge,
                                                    int use_contact, int use_dense, const float* __restrict__ vor,
                                                    CgVec cv, Incid ic, const float4* __restrict__ fc,
-                                                   const float4* __restrict__ hc, const float* __restrict__ sA,
+                                                   const float4* __restrict__ hc, const float4* __restrict__ cslot,
                                                    const float4* __restrict__ cdd, const float* __restrict__ dn,
                                                    float4* __restrict__ v_out, float4* __restrict__ Av,
                                                    double* __restrict__ vav) {
@@ -1009,7 +1086,7 @@ __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2g
         if (use_hinge)
             for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) acc = acc + v3(hc[e]);
         if (use_contact)
-            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) acc = acc + v3(ic.cdv[e]) * sA[ic.cj[e]];
+            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) acc = acc + v3(cslot[e]);
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
@@ -1083,15 +1160,6 @@ __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, 
 __global__ void invert_k(const int* __restrict__ ent, int n, int* __restrict__ dst) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) dst[ent[e]] = e;
-}
-// active-contact corner gradients and contact indices in vertex-list order
-__global__ void contact_permute_k(const int* __restrict__ ent, int n, const float4* __restrict__ cdd,
-                                  float4* __restrict__ cdv, int* __restrict__ cj) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int s = ent[e];
-    cdv[e] = cdd[s];
-    cj[e] = s >> 2;
 }
 // keys[s] = vertex of slot s of element s / width, vals[s] = s
 __global__ void slot_keys_k(const int* __restrict__ elem, int width, int stride, int n_slots,
@@ -1331,25 +1399,31 @@ static int compute_diff(Ctx& c, int nC, int mask) {
     } else {
         PAMO_TRY(fill(c.n_act, 0, 1, c.stream));
     }
-    Counts k = counts(c, c.nA, mask);
-    if (k.total() > 0)
-        L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
-           (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.hsym, (const int*)c.act, (const int*)c.n_act, c.cdd);
-    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
-        L3(dense_diff_k, 3 * c.nV, (const float4*)c.q, c.nV, (const double*)c.dense_A, (const double*)c.dense_b, -1.0f,
-           c.grad, c.diag);
     if (c.nA > 0) {
+        // vertex -> active-contact corner lists (list positions c_dst of the
+        // corner slots 4 j + a), before diff_k writes the slots
         const long long n = 4LL * c.nA;
         PAMO_TRY(ensure_sort(c, n));
         L3(contact_keys_k, n, (const int4*)c.cidx, (const int*)c.act, (const int*)c.n_act, c.skey, c.sval);
         PAMO_TRY(finish_incidence(c, n, c.vc_off, c.vc_ent));
-        L3(contact_permute_k, n, (const int*)c.vc_ent, (int)n, (const float4*)c.cdd, c.cdv, c.cj);
-        Incid ic{c.vf_off, c.vh_off, c.vc_off, (const float4*)c.cdv, (const int*)c.cj};
-        L3(contact_diff_gather_k, (long long)c.nV * kGather, c.nV, c.P, -1.0f, ic, (const float*)c.cd,
-           (const int*)c.act, c.grad, c.diag);
+        L3(invert_k, n, (const int*)c.vc_ent, (int)n, c.c_dst);
     } else {
         PAMO_TRY(fill(c.vc_off, 0, c.nV + 1, c.stream));
     }
+    Counts k = counts(c, c.nA, mask);
+    DiffSlots ds{c.vf_dst, c.vh_dst, c.c_dst, c.fc, c.hc, c.hdg, c.cslot, c.cdg};
+    if (k.total() > 0)
+        L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
+           (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.hsym, (const int*)c.act, (const int*)c.n_act, c.cdd, ds);
+    if (k.f + k.h + k.c > 0) {
+        Incid ic{c.vf_off, c.vh_off, c.vc_off};
+        L3(diff_gather_k, (long long)c.nV * kGather, c.nV, (int)(k.f > 0), (int)(k.h > 0), (int)(k.c > 0), ic,
+           (const float4*)c.fc, (const float4*)c.hc, (const float4*)c.hdg, (const float4*)c.cslot,
+           (const float4*)c.cdg, c.grad, c.diag);
+    }
+    if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
+        L3(dense_diff_k, 3 * c.nV, (const float4*)c.q, c.nV, (const double*)c.dense_A, (const double*)c.dense_b, -1.0f,
+           c.grad, c.diag);
     return 0;
 }
 
@@ -1364,11 +1438,11 @@ static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4*
     if (total > 0)
         L3(hvp_elem_k, total, (const float4*)c.q, topo(c), c.P, nF, nH, nA, (const int*)c.n_act, nd, cv,
            (const float*)c.cd, (const int*)c.act, (const float*)c.hsym, (const float4*)c.cdd,
-           (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, c.sA, c.dn);
-    Incid ic{c.vf_off, c.vh_off, c.vc_off, (const float4*)c.cdv, (const int*)c.cj};
+           (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, (const int*)c.c_dst, c.cslot, c.dn);
+    Incid ic{c.vf_off, c.vh_off, c.vc_off};
     L3(hvp_gather_k, (long long)c.nV * kGather, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
        (int)(nA > 0), (int)(nd > 0), (const float*)c.vor, cv, ic, (const float4*)c.fc, (const float4*)c.hc,
-       (const float*)c.sA, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
+       (const float4*)c.cslot, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
     return 0;
 }
 
@@ -1378,16 +1452,19 @@ static int ensure_contacts(Ctx& c, int n) {
     c.capC = a;
     PAMO_TRY(dalloc(&c.cidx, a, c.stream));
     PAMO_TRY(dalloc(&c.ctype1, a, c.stream));
+    PAMO_TRY(dalloc(&c.cidx2, a, c.stream));
+    PAMO_TRY(dalloc(&c.ctype1b, a, c.stream));
+    PAMO_TRY(dalloc(&c.ctag, a, c.stream));
     PAMO_TRY(dalloc(&c.ctype2, a, c.stream));
     PAMO_TRY(dalloc(&c.cd, a, c.stream));
     PAMO_TRY(dalloc(&c.cdd, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.act, a, c.stream));
     PAMO_TRY(dalloc(&c.act_flag, (size_t)a + 1, c.stream));
     PAMO_TRY(dalloc(&c.act_pos, (size_t)a + 1, c.stream));
-    PAMO_TRY(dalloc(&c.sA, a, c.stream));
+    PAMO_TRY(dalloc(&c.cslot, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.c_dst, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a, c.stream));
-    PAMO_TRY(dalloc(&c.cdv, 4 * (size_t)a, c.stream));
-    PAMO_TRY(dalloc(&c.cj, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.cdg, 4 * (size_t)a, c.stream));
     return 0;
 }
 
@@ -1424,14 +1501,24 @@ static int detect_contacts(Ctx& c, int& nC) {
         while (true) {
             PAMO_TRY(fill(c.ccount, 0, 1, c.stream));
             const int cap = std::min(c.capC, c.P.max_blocks);
-            const long long pt_threads = ((long long)c.nV + kDetW - 1) / kDetW * kWave;
-            const long long ee_threads = ((long long)c.nE + kDetW - 1) / kDetW * kWave;
-            L3(detect_k<true>, pt_threads, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
+            // launches of whole blocks: waves past the queries count 0
+            const long long pt_blocks = grid_for(((long long)c.nV + kDetW - 1) / kDetW * kWave, kB);
+            const long long ee_blocks = grid_for(((long long)c.nE + kDetW - 1) / kDetW * kWave, kB);
+            const int pt_waves = (int)(pt_blocks * (kB / kWave)), n_waves = (int)((pt_blocks + ee_blocks) * (kB / kWave));
+            if (n_waves + 1 > c.cap_wave) {
+                c.cap_wave = n_waves + 1 + n_waves / 4;
+                PAMO_TRY(dalloc(&c.wave_count, (size_t)c.cap_wave, c.stream));
+                PAMO_TRY(dalloc(&c.wave_base, (size_t)c.cap_wave, c.stream));
+            }
+            PAMO_TRY(fill(c.wave_count, 0, n_waves + 1, c.stream));
+            DetOut o{c.ccount, cap, c.cidx, c.ctype1, c.ctag, c.wave_count, 0};
+            L3(detect_k<true>, pt_blocks * kB, (const float4*)c.q, c.nV, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.tri_tree.box, (const int*)c.tri_tree.right, c.nF, c.P.d_hat, c.radius, c.P.ee_thres,
-               c.ccount, cap, c.cidx, c.ctype1, c.ccount + 1);
-            L3(detect_k<false>, ee_threads, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
+               o, c.ccount + 1);
+            o.wave0 = pt_waves;
+            L3(detect_k<false>, ee_blocks * kB, (const float4*)c.q, c.nE, (const int4*)c.tri, (const int2*)c.edge,
                (const float4*)c.edge_tree.box, (const int*)c.edge_tree.right, c.nE, c.P.d_hat, c.radius,
-               c.P.ee_thres, c.ccount, cap, c.cidx, c.ctype1, c.ccount + 1);
+               c.P.ee_thres, o, c.ccount + 1);
             PAMO_HIP(hipMemcpyAsync(c.hcount, c.ccount, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
             PAMO_HIP(hipStreamSynchronize(c.stream));
             PAMO_TRY(check_walks(c));
@@ -1446,6 +1533,15 @@ static int detect_contacts(Ctx& c, int& nC) {
                 continue;
             }
             nC = n;
+            // contacts in (wave, emission rank) order
+            if (nC > 0) {
+                PAMO_TRY(ensure_ws(c, (long long)n_waves + 64));
+                PAMO_TRY(scan_exclusive(c.wave_count, c.wave_base, n_waves + 1LL, c.ws, c.stream));
+                L3(contact_order_k, nC, nC, (const int2*)c.ctag, (const int*)c.wave_base, (const int4*)c.cidx,
+                   (const int*)c.ctype1, c.cidx2, c.ctype1b);
+                std::swap(c.cidx, c.cidx2);
+                std::swap(c.ctype1, c.ctype1b);
+            }
             break;
         }
     }
@@ -1679,7 +1775,84 @@ static void host_prepare(const float* V, int nV, const int* F, int nF, double sc
     hp.min_angle_deg = nF == 0 ? 180.0 : (any_nan ? NAN : min_ang * 180.0 / M_PI);
 }
 
-// normalization transform (processing.py:15-20) reduced on the device in FP64
+// numpy's V.mean(axis=0) of a float64 (n, 3) array adds the rows one after
+// another (from the first row) in FP64; the same sequential sum of the
+// float32 vertices widened to FP64, in one workgroup: waves 1..15 copy the
+// next chunk of coordinates to LDS while lanes 0..2 of wave 0 add up the
+// current chunk, one coordinate per lane, 32 values read ahead.
+constexpr int kSeqThreads = 1024;
+constexpr int kSeqChunk = 2048;            // multiple of 32
+constexpr int kSeqPitch = kSeqChunk + 4;
+
+__device__ inline void seq_fill(const float* __restrict__ v, long long base, int cnt, float* __restrict__ dst, int t,
+                                int nt) {
+    for (int j = t; j < 3 * cnt; j += nt) {
+        const int r = j / 3, d = j - 3 * r;  // coalesced reads of the interleaved rows
+        dst[d * kSeqPitch + r] = v[3 * base + j];
+    }
+}
+
+__device__ __forceinline__ double seq_chain(double acc, const float* __restrict__ row, int n) {
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+    const int full = n >> 5;
+    float4 cur[8], nxt[8];
+    if (full > 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cur[j] = r4[j];
+    }
+    for (int b = 0; b < full; ++b) {
+        if (b + 1 < full) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) nxt[j] = r4[8 * (b + 1) + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            acc = acc + (double)cur[j].x;
+            acc = acc + (double)cur[j].y;
+            acc = acc + (double)cur[j].z;
+            acc = acc + (double)cur[j].w;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
+    }
+    for (int i = full << 5; i < n; ++i) acc = acc + (double)row[i];
+    return acc;
+}
+
+__global__ void __launch_bounds__(kSeqThreads) seq_sum3_k(const float* __restrict__ v, int n,
+                                                         double* __restrict__ out3) {
+    __shared__ __align__(16) float buf[2][3 * kSeqPitch];
+    const int t = threadIdx.x;
+    const int nchunks = (n + kSeqChunk - 1) / kSeqChunk;
+    seq_fill(v, 0, n < kSeqChunk ? n : kSeqChunk, buf[0], t, kSeqThreads);
+    __syncthreads();
+    double acc = 0.0;
+    for (int k = 0; k < nchunks; ++k) {
+        const long long base = (long long)k * kSeqChunk;
+        const int cnt = (int)((n - base) < kSeqChunk ? (n - base) : kSeqChunk);
+        if (t >= kWave) {
+            const long long nb = base + kSeqChunk;
+            if (nb < n)
+                seq_fill(v, nb, (int)((n - nb) < kSeqChunk ? (n - nb) : kSeqChunk), buf[(k + 1) & 1], t - kWave,
+                         kSeqThreads - kWave);
+        } else if (t < 3) {
+            const float* row = buf[k & 1] + t * kSeqPitch;
+            if (k == 0) {
+                // from the first row, then one at a time up to 32 (aligned), then in batches
+                acc = (double)row[0];
+                const int head = cnt < 32 ? cnt : 32;
+                for (int i = 1; i < head; ++i) acc = acc + (double)row[i];
+                acc = seq_chain(acc, row + head, cnt - head);
+            } else {
+                acc = seq_chain(acc, row, cnt);
+            }
+        }
+        __syncthreads();
+    }
+    if (t < 3) out3[t] = acc;
+}
+
+// normalization transform (processing.py:15-20): bounds reduced on the device
 __global__ void __launch_bounds__(kB) gt_stats_k(const float* __restrict__ v, int n, double* __restrict__ part) {
     __shared__ double red[kB];
     __shared__ float rlo[6][kBlock];
@@ -1968,6 +2141,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
         PAMO_TRY(dalloc(&c.vh_ent, 4 * (size_t)c.capH, c.stream));
         PAMO_TRY(dalloc(&c.vh_dst, 4 * (size_t)c.capH, c.stream));
         PAMO_TRY(dalloc(&c.hc, 4 * (size_t)c.capH, c.stream));
+        PAMO_TRY(dalloc(&c.hdg, 4 * (size_t)c.capH, c.stream));
     }
     c.nV = n_v;
     c.nF = n_f;
@@ -2258,16 +2432,21 @@ int pamo_s3_normalization(pamo_s3* s, const float* d_v, int n_v, double* scale, 
     PAMO_TRY(pamo::s3::ensure_part(c, 9LL * nb * pamo::s3::kB));
     hipLaunchKernelGGL(pamo::s3::gt_stats_k, dim3(nb), dim3(pamo::s3::kB), 0, c.stream, d_v, n_v, c.part);
     PAMO_HIP(hipGetLastError());
-    std::vector<double> h(9 * (size_t)nb);
+    // the mean as numpy's sequential FP64 sum (one workgroup), after the
+    // bounds' partials
+    double* d_sum = c.part + 9LL * nb;
+    hipLaunchKernelGGL(pamo::s3::seq_sum3_k, dim3(1), dim3(pamo::s3::kSeqThreads), 0, c.stream, d_v, n_v, d_sum);
+    PAMO_HIP(hipGetLastError());
+    std::vector<double> h(9 * (size_t)nb + 3);
     PAMO_HIP(hipMemcpyAsync(h.data(), c.part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
     PAMO_HIP(hipStreamSynchronize(c.stream));
-    double sum[3] = {0, 0, 0}, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int b = 0; b < nb; ++b)
         for (int k = 0; k < 3; ++k) {
-            sum[k] += h[9 * (size_t)b + k];
             lo[k] = std::min(lo[k], h[9 * (size_t)b + 3 + k]);
             hi[k] = std::max(hi[k], h[9 * (size_t)b + 6 + k]);
         }
+    const double* sum = h.data() + 9 * (size_t)nb;
     const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
     *scale = 1.0 / ext;
     for (int k = 0; k < 3; ++k) t3[k] = -(sum[k] / n_v) * *scale;

@@ -357,8 +357,9 @@ class Stage3System:
 
 
 def normalization_transform_device(system: "Stage3System", V: torch.Tensor):
-    """normalization_transform of a device float32 (n, 3) tensor, reduced in
-    FP64 by the library."""
+    """normalization_transform of a device float32 (n, 3) tensor, in FP64 on
+    the device: the mean as numpy's sequential sum over the rows, so the
+    result equals numpy's on the same values bit for bit."""
     v = V.to(torch.float32).contiguous()
     sc = ctypes.c_double()
     t = (ctypes.c_double * 3)()

@@ -1,9 +1,10 @@
 """Batched and served forms (pamo_batch_run, pamo_serve_*) on the GPU.
 
 Every job is one whole PaMO.run and must give the result of that call made
-alone on a fresh object: stage-2 outputs bit for bit, stage-3 vertices
-within 2e-5 of the mesh extent (the stage-3 gradient is accumulated with
-float atomics, whose order varies between runs).
+alone on a fresh object, bit for bit: stage 3 lists contacts and sums every
+per-vertex term in a fixed order, and the engine normalises with numpy's
+sequential FP64 mean, so its float32 vertices equal the plain form's float64
+vertices rounded to float32.
 """
 import numpy as np
 import pytest
@@ -37,11 +38,7 @@ def _alone(V, F, ratio, stage3, exact_guard=False):
 def _same(a, b, V, stage3):
     (va, fa), (vb, fb) = a, b
     assert np.array_equal(fa.astype(np.int64), fb.astype(np.int64))
-    if stage3:
-        ext = float((V.max(0) - V.min(0)).max())
-        assert np.abs(va.astype(np.float64) - vb.astype(np.float64)).max() <= 2e-5 * ext
-    else:
-        assert np.array_equal(va.astype(np.float32).view(np.uint32), vb.astype(np.float32).view(np.uint32))
+    assert np.array_equal(va.astype(np.float32).view(np.uint32), vb.astype(np.float32).view(np.uint32))
 
 
 def test_context_reset_equals_fresh_context():

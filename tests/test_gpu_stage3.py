@@ -211,3 +211,34 @@ def test_process_end_to_end():
     n = len(V) // 2
     c = V[:n].mean(0)
     assert np.all(np.linalg.norm(V[n:] - c, axis=1) < np.linalg.norm(V[:n] - c, axis=1))
+
+
+def test_process_is_deterministic():
+    """Contacts in a fixed order and fixed-order per-vertex sums: two runs
+    on fresh systems, and a third on a reused one, give the same vertices
+    bit for bit (two sheets 0.1 % apart: thousands of active contacts)."""
+    s = scene("sheets", level=3)
+    args = [torch.from_numpy(s[k]).cuda() for k in ("gt_V", "gt_F", "V", "F")]
+    print("[stage3] process three times", flush=True)
+    V1, _ = stage3.process(*args, 5)
+    V2, _ = stage3.process(*args, 5)
+    sy = stage3.Stage3System(stage3.Stage3Config())
+    stage3.process(*args, 2, system=sy)
+    V3, _ = stage3.process(*args, 5, system=sy)
+    assert sy.stats()["contacts"] > 1000
+    assert np.array_equal(V1.view(np.uint64), V2.view(np.uint64))
+    assert np.array_equal(V1.view(np.uint64), V3.view(np.uint64))
+
+
+def test_device_normalization_equals_numpy():
+    """pamo_s3_normalization: numpy's sequential FP64 mean and the extents,
+    bit for bit, on an off-centre cloud of 300k points."""
+    rng = np.random.default_rng(3)
+    V = (rng.standard_normal((300_000, 3)) * np.array([1.0, 30.0, 1e-2]) + np.array([5.0, -300.0, 2e3]))
+    V = V.astype(np.float32)
+    sy = stage3.Stage3System(stage3.Stage3Config())
+    print("[stage3] device normalization, 300k points", flush=True)
+    sc, t = stage3.normalization_transform_device(sy, torch.from_numpy(V).cuda())
+    sc_ref, t_ref = stage3.normalization_transform(V.astype(np.float64))
+    assert sc == sc_ref
+    assert np.array_equal(np.asarray(t, np.float64), np.asarray(t_ref, np.float64))
