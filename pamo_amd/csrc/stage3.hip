@@ -83,6 +83,8 @@ struct Ctx {
     // active contacts (d < d_hat) of the current Newton iteration
     int *act = nullptr, *act_flag = nullptr, *act_pos = nullptr, *n_act = nullptr;
     float4* cslot = nullptr;  // per active-contact corner H dx terms, vertex-list order
+    int4* cvert = nullptr;    // per active contact: vertices
+    float* c2j = nullptr;     // per active contact: barrier'' kappa
     int* c_dst = nullptr;     // list position of active-contact corner slot 4 j + a
     int nA = 0;
     uint32_t *skey = nullptr, *sval = nullptr, *skey2 = nullptr, *sval2 = nullptr;
@@ -142,7 +144,7 @@ static void ctx_free(Ctx& c) {
                     c.edge, c.hinge, c.area, c.rest_angle, c.rest_elen, c.inv_dm, c.hblock, c.gt_v, c.gt_f,
                     c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
                     c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
-                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.cslot, c.c_dst, c.skey, c.sval, c.skey2,
+                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.cslot, c.c_dst, c.cvert, c.c2j, c.skey, c.sval, c.skey2,
                     c.vf_dst, c.vh_dst, c.hdg, c.cdg,
                     c.sval2, c.zr_part, c.vav_part,
                     c.mg_ref, c.mg_first, c.mg_keep, c.mg_pos, c.mg_tab, c.mg_err,
@@ -610,7 +612,7 @@ __device__ __forceinline__ Membrane membrane(V3 x0, V3 x1, V3 x2, float4 idm) {
 // [+nC) collision (also stores the distance and its classification).
 __global__ void __launch_bounds__(kB) energy_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF,
                                                int nH, int nC, float* __restrict__ cd, int* __restrict__ ctype2,
-                                               double* __restrict__ part) {
+                                               double* __restrict__ part, const float* __restrict__ reach) {
     __shared__ double red[kB];
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
@@ -630,20 +632,32 @@ __global__ void __launch_bounds__(kB) energy_k(const float4* __restrict__ x, Top
         float dt = th - tp.rest_angle[i];
         e = (double)(0.5f * P.k_hinge * dt * dt * tp.rest_elen[i]);
     } else if ((i -= nH) < nC) {
-        int4 k = tp.cidx[i];
-        V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
-        float d;
-        int t2;
-        if (tp.ctype1[i] == BLOCK_PT) {
-            t2 = pt_classify(x0, x1, x2, x3);
-            d = pt_distance(x0, x1, x2, x3, t2);
-        } else {
-            t2 = ee_classify(x0, x1, x2, x3, P.ee_thres);
-            d = ee_distance(x0, x1, x2, x3, t2);
+        // Line-search points (reach given): x = q_n + f p with f <= the CCD
+        // step s and |p_v| <= pmax, so a pair's distance differs from its
+        // distance cd at q_n by at most 2 s pmax; a pair farther than d_hat
+        // by that much (plus a tenth of d_hat for rounding) has energy 0 at
+        // every such point and is not evaluated.  cd / ctype2 keep the q_n
+        // values there (no later pass reads the line-search ones).
+        // (a NaN bound or distance compares false: the pair is evaluated)
+        const bool far = reach && cd[i] > P.d_hat + 2.0f * reach[0] * reach[1] + 0.1f * P.d_hat;
+        if (!far) {
+            int4 k = tp.cidx[i];
+            V3 x0 = v3(x[k.x]), x1 = v3(x[k.y]), x2 = v3(x[k.z]), x3 = v3(x[k.w]);
+            float d;
+            int t2;
+            if (tp.ctype1[i] == BLOCK_PT) {
+                t2 = pt_classify(x0, x1, x2, x3);
+                d = pt_distance(x0, x1, x2, x3, t2);
+            } else {
+                t2 = ee_classify(x0, x1, x2, x3, P.ee_thres);
+                d = ee_distance(x0, x1, x2, x3, t2);
+            }
+            if (!reach) {
+                ctype2[i] = t2;
+                cd[i] = d;
+            }
+            if (d < P.d_hat) e = (double)(barrier(d, P.d_hat) * P.kappa);
         }
-        ctype2[i] = t2;
-        cd[i] = d;
-        if (d < P.d_hat) e = (double)(barrier(d, P.d_hat) * P.kappa);
     }
     double s = block_sum(e, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -654,9 +668,10 @@ __global__ void __launch_bounds__(kB) energy_k(const float4* __restrict__ x, Top
 // columns of vertex 3 rebuilt by translation invariance (sum over a block
 // row / column vanishes), as block_spd_project_kernel (utils_kernels.py:21-39).
 // Also writes the projected 9x9 as a packed upper triangle (45 floats,
-// row-major), which is all the Hessian-vector product needs.
+// row-major, entry k at S45[k * stride]: the hinges' k-th entries are
+// contiguous), which is all the Hessian-vector product needs.
 __device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row-major 3x3 */,
-                                  float* __restrict__ S45) {
+                                  float* __restrict__ S45, long long stride) {
     float A[9][9], Vm[9][9];
 #pragma unroll
     for (int r = 0; r < 9; ++r)
@@ -716,7 +731,7 @@ __device__ void spd_project_hinge(float* B /* 144: block (a,b) at 9*(4*a+b), row
 #pragma unroll
             for (int k = 0; k < 9; ++k) s += Vm[r][k] * lam[k] * Vm[c][k];
             B[9 * (4 * (r / 3) + (c / 3)) + 3 * (r % 3) + (c % 3)] = s;
-            if (c >= r) S45[9 * r - (r * (r - 1)) / 2 + (c - r)] = s;
+            if (c >= r) S45[(9 * r - (r * (r - 1)) / 2 + (c - r)) * stride] = s;
         }
     // row 3: block[3][j] = -(block[0][j] + block[1][j] + block[2][j]) for j < 3
     for (int j = 0; j < 3; ++j)
@@ -752,6 +767,12 @@ __device__ __forceinline__ void put(float* B, int a, int b, M3 x) { for (int k =
 struct DiffSlots {
     const int *f_dst, *h_dst, *c_dst;
     float4 *fc, *hc, *hdg, *cslot, *cdg;
+    long long hstride;  // hsym: entry k of hinge i at k * hstride + i
+    // per active contact j for the Hessian-vector products: vertices, barrier''
+    // kappa, distance gradient of corner a at cdd[a * cstride + j]
+    int4* cvert;
+    float* c2j;
+    long long cstride;
 };
 __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo tp, Params P, int nV, int nF, int nH,
                                              int nC, float coeff, const float* __restrict__ cd,
@@ -846,7 +867,7 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
             const float* D = B + 9 * (4 * a + a);
             ds.hdg[ds.h_dst[4 * i + a]] = f4(mk(D[0], D[4], D[8]));
         }
-        spd_project_hinge(B, hsym + 45 * (long long)i);
+        spd_project_hinge(B, hsym + i, ds.hstride);
     } else if ((i -= nH) < nC && i < n_act[0]) {
         // active contacts only (d < d_hat); their distance gradients are kept
         // in list order for the Hessian-vector products
@@ -861,8 +882,10 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
         // barrier' kappa coeff g and barrier'' kappa g*g per corner
         const float c1 = barrier_d1(d, P.d_hat) * P.kappa * coeff;
         const float c2 = barrier_d2(d, P.d_hat) * P.kappa;
+        ds.cvert[j] = k;
+        ds.c2j[j] = c2;
         for (int a = 0; a < 4; ++a) {
-            cdd[4 * j + a] = f4(g[a]);
+            cdd[a * ds.cstride + j] = f4(g[a]);
             const int e = ds.c_dst[4 * j + a];
             ds.cslot[e] = f4(g[a] * c1);
             ds.cdg[e] = f4(c2 * mk(g[a].x * g[a].x, g[a].y * g[a].y, g[a].z * g[a].z));
@@ -913,8 +936,9 @@ __device__ __forceinline__ V3 cg_dx(const CgVec& cv, float beta, int v) {
 // [.., +3 nV_dense) dense rows
 __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, Topo tp, Params P, int nF, int nH,
                                                  int nA_max, const int* __restrict__ n_act, int n_dense,
-                                                 CgVec cv, const float* __restrict__ cd, const int* __restrict__ act,
-                                                 const float* __restrict__ hsym, const float4* __restrict__ cdd,
+                                                 CgVec cv, const int4* __restrict__ cvert, const float* __restrict__ c2j,
+                                                 const float* __restrict__ hsym, long long hstride,
+                                                 const float4* __restrict__ cdd, long long cstride,
                                                  const double* __restrict__ dA, const int* __restrict__ f_dst,
                                                  const int* __restrict__ h_dst, float4* __restrict__ fc,
                                                  float4* __restrict__ hc, const int* __restrict__ c_dst,
@@ -968,7 +992,7 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
         V3 d3 = cg_dx(cv, beta, k.w);
         V3 e[3] = {cg_dx(cv, beta, k.x) - d3, cg_dx(cv, beta, k.y) - d3, cg_dx(cv, beta, k.z) - d3};
         float u[9] = {e[0].x, e[0].y, e[0].z, e[1].x, e[1].y, e[1].z, e[2].x, e[2].y, e[2].z};
-        const float* S = hsym + 45 * i;
+        const float* S = hsym + i;
         float y[9];
 #pragma unroll
         for (int r = 0; r < 9; ++r) y[r] = 0.0f;
@@ -976,7 +1000,7 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
         for (int r = 0; r < 9; ++r)
 #pragma unroll
             for (int c = r; c < 9; ++c) {
-                const float a = S[9 * r - (r * (r - 1)) / 2 + (c - r)];
+                const float a = S[(9 * r - (r * (r - 1)) / 2 + (c - r)) * hstride];
                 y[r] += a * u[c];
                 if (c != r) y[c] += a * u[r];
             }
@@ -987,19 +1011,20 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
         hc[h_dst[4 * i + 3]] = f4(-(y0 + y1 + y2));
     } else if ((i -= nH) < nA_max) {
         if (i >= n_act[0]) return;
-        const int c = act[i];
-        const int4 k = tp.cidx[c];
-        const float c2 = barrier_d2(cd[c], P.d_hat) * P.kappa;
-        const V3 g0 = v3(cdd[4 * i]), g1 = v3(cdd[4 * i + 1]), g2 = v3(cdd[4 * i + 2]), g3 = v3(cdd[4 * i + 3]);
+        const int4 k = cvert[i];
+        const float c2 = c2j[i];
+        const V3 g0 = v3(cdd[i]), g1 = v3(cdd[cstride + i]), g2 = v3(cdd[2 * cstride + i]),
+                 g3 = v3(cdd[3 * cstride + i]);
         const float sdot = dot(g0, cg_dx(cv, beta, k.x)) + dot(g1, cg_dx(cv, beta, k.y)) +
                            dot(g2, cg_dx(cv, beta, k.z)) + dot(g3, cg_dx(cv, beta, k.w));
         // H_c dx = g (c2 g . dx): each corner's term goes to its slot in the
         // vertex lists, so the gather reads it contiguously
         const float sa = sdot * c2;
-        cslot[c_dst[4 * i]] = f4(g0 * sa);
-        cslot[c_dst[4 * i + 1]] = f4(g1 * sa);
-        cslot[c_dst[4 * i + 2]] = f4(g2 * sa);
-        cslot[c_dst[4 * i + 3]] = f4(g3 * sa);
+        const int4 dst = reinterpret_cast<const int4*>(c_dst)[i];
+        cslot[dst.x] = f4(g0 * sa);
+        cslot[dst.y] = f4(g1 * sa);
+        cslot[dst.z] = f4(g2 * sa);
+        cslot[dst.w] = f4(g3 * sa);
     } else if ((i -= nA_max) < n_dense) {
         const int n3 = (int)n_dense;
         double acc = 0.0;
@@ -1205,13 +1230,25 @@ __global__ void act_list_k(const int* __restrict__ flag, const int* __restrict__
 __global__ void copy_slot_k(double* sc, int from, int to) { sc[to] = sc[from]; }
 
 // ------------------------------------------------------------ clamp, CCD, line search (B9, B10)
+// also the largest |p| after the clamp into ccd[1] (integer max on the
+// non-negative float's bits, one atomic per wave)
 __global__ void clamp_k(float4* __restrict__ p, const float4* __restrict__ qn, const float4* __restrict__ qd, int n,
-                        float radius) {
+                        float radius, float* __restrict__ ccd) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 pi = v3(p[i]), qni = v3(qn[i]), qdi = v3(qd[i]);
-    V3 dq = qni + pi - qdi;
-    if (len(dq) > radius) p[i] = f4(normalize(dq) * radius + qdi - qni);
+    float m = 0.0f;
+    if (i < n) {
+        V3 pi = v3(p[i]), qni = v3(qn[i]), qdi = v3(qd[i]);
+        V3 dq = qni + pi - qdi;
+        if (len(dq) > radius) {
+            pi = normalize(dq) * radius + qdi - qni;
+            p[i] = f4(pi);
+        }
+        m = len(pi);
+        if (!(m >= 0.0f)) m = __builtin_huge_valf();  // a NaN step: no pair is skipped
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax((int*)ccd + 1, __float_as_int(m));
 }
 
 // Additive CCD (Li et al. 2021), ccd_kernels.py:8-110; min step by an
@@ -1330,12 +1367,13 @@ static Counts counts(const Ctx& c, int nC, int mask) {
                   (mask & PAMO_S3_E_HINGE) ? c.nH : 0, (mask & PAMO_S3_E_COLLISION) ? nC : 0};
 }
 
-static int compute_energy(Ctx& c, int nC, int slot, int mask) {
+// reach: null, or (CCD step, max |p|) at a line-search point (see energy_k)
+static int compute_energy(Ctx& c, int nC, int slot, int mask, const float* reach = nullptr) {
     const Counts k = counts(c, nC, mask);
     PAMO_TRY(ensure_part(c, k.total()));
     const unsigned nb = grid_for(k.total(), kB);
     hipLaunchKernelGGL(energy_k, dim3(nb), dim3(kB), 0, c.stream, (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c,
-                       c.cd, c.ctype2, c.part);
+                       c.cd, c.ctype2, c.part, reach);
     hipLaunchKernelGGL(finish_sum_k, dim3(1), dim3(kB), 0, c.stream, (const double*)c.part, (int)nb, c.sc, slot);
     if ((mask & PAMO_S3_E_DENSE) && c.dense_A)
         hipLaunchKernelGGL(dense_energy_k, dim3(1), dim3(kB), 0, c.stream, (const float4*)c.q, c.nV,
@@ -1411,7 +1449,8 @@ static int compute_diff(Ctx& c, int nC, int mask) {
         PAMO_TRY(fill(c.vc_off, 0, c.nV + 1, c.stream));
     }
     Counts k = counts(c, c.nA, mask);
-    DiffSlots ds{c.vf_dst, c.vh_dst, c.c_dst, c.fc, c.hc, c.hdg, c.cslot, c.cdg};
+    DiffSlots ds{c.vf_dst, c.vh_dst, c.c_dst, c.fc, c.hc, c.hdg, c.cslot, c.cdg, (long long)c.capH,
+                 c.cvert, c.c2j, (long long)c.capC};
     if (k.total() > 0)
         L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
            (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.hsym, (const int*)c.act, (const int*)c.n_act, c.cdd, ds);
@@ -1437,7 +1476,8 @@ static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4*
     const long long total = (long long)nF + nH + nA + nd;
     if (total > 0)
         L3(hvp_elem_k, total, (const float4*)c.q, topo(c), c.P, nF, nH, nA, (const int*)c.n_act, nd, cv,
-           (const float*)c.cd, (const int*)c.act, (const float*)c.hsym, (const float4*)c.cdd,
+           (const int4*)c.cvert, (const float*)c.c2j, (const float*)c.hsym, (long long)c.capH, (const float4*)c.cdd,
+           (long long)c.capC,
            (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, (const int*)c.c_dst, c.cslot, c.dn);
     Incid ic{c.vf_off, c.vh_off, c.vc_off};
     L3(hvp_gather_k, (long long)c.nV * kGather, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
@@ -1465,6 +1505,8 @@ static int ensure_contacts(Ctx& c, int n) {
     PAMO_TRY(dalloc(&c.c_dst, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.cdg, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.cvert, a, c.stream));
+    PAMO_TRY(dalloc(&c.c2j, a, c.stream));
     return 0;
 }
 
@@ -1611,7 +1653,8 @@ static int cg_solve(Ctx& c) {
 
 // _clamp_p + _ccd (system.py:354-391)
 static int clamp_and_ccd(Ctx& c, int nC) {
-    L3(clamp_k, c.nV, c.p, (const float4*)c.qn, (const float4*)c.qd, c.nV, c.P.radius0);
+    PAMO_TRY(fill(c.ccd_step + 1, 0.0f, 1, c.stream));
+    L3(clamp_k, c.nV, c.p, (const float4*)c.qn, (const float4*)c.qd, c.nV, c.P.radius0, c.ccd_step);
     PAMO_TRY(fill(c.ccd_step, 1.0f, 1, c.stream));
     if (nC > 0) L3(accd_k, nC, (const float4*)c.q, (const float4*)c.p, topo(c), c.P, nC, c.ccd_step);
     return 0;
@@ -1649,7 +1692,7 @@ static int step_once(Ctx& c, pamo_s3_newton_fn on_newton = nullptr, void* user =
             ++c.stat_ls;
             L3(line_search_k, nV, (const float4*)c.qn, (const float4*)c.p, (const float*)c.ccd_step, h,
                (const double*)c.sc, nV, c.q);
-            PAMO_TRY(compute_energy(c, nC, SC_E, mask));
+            PAMO_TRY(compute_energy(c, nC, SC_E, mask, c.ccd_step));
         }
         if (on_newton) {
             PAMO_HIP(hipStreamSynchronize(c.stream));
@@ -1792,6 +1835,7 @@ __device__ inline void seq_fill(const float* __restrict__ v, long long base, int
     }
 }
 
+// acc + row[0] + ... + row[n - 1] in FP64, left to right (row 16-byte aligned)
 __device__ __forceinline__ double seq_chain(double acc, const float* __restrict__ row, int n) {
     const float4* r4 = reinterpret_cast<const float4*>(row);
     const int full = n >> 5;
@@ -2041,7 +2085,8 @@ int pamo_s3_create(pamo_s3** out, int device, const pamo_s3_params* prm) {
     if (e == hipSuccess) e = hipMemsetAsync(c->n_act, 0, sizeof(int), c->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&c->sc, sizeof(double) * pamo::s3::SC_N);
     if (e == hipSuccess) e = hipMemsetAsync(c->sc, 0, sizeof(double) * pamo::s3::SC_N, c->stream);
-    if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, sizeof(float));
+    // ccd_step[0]: the CCD step; [1]: the largest |p| after the clamp
+    if (e == hipSuccess) e = hipMalloc((void**)&c->ccd_step, 2 * sizeof(float));
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcount, 2 * sizeof(int), 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hsc, sizeof(double) * pamo::s3::SC_N, 0);
     if (e != hipSuccess) {
