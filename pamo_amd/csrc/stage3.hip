@@ -179,14 +179,25 @@ static int ensure_part(Ctx& c, long long items) {
 }
 
 // ------------------------------------------------------------ deterministic FP64 reductions
+// Sum over the 64 lanes of a wave by an xor butterfly: each step adds a
+// lane's value and its partner's, which both lanes form alike, so every lane
+// ends with the same value whatever the scheduling.
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m);
+    return x;
+}
+
+// Sum over the block: wave sums, then the waves' sums in wave order; every
+// thread returns it.
 __device__ __forceinline__ double block_sum(double x, double* red) {
-    red[threadIdx.x] = x;
+    constexpr int kW = kB / kWave;
+    x = wave_sum(x);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
     __syncthreads();
-    for (int s = kB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
     double r = red[0];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) r += red[w];
     __syncthreads();
     return r;
 }
@@ -195,7 +206,15 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 __global__ void __launch_bounds__(kB) finish_sum_k(const double* __restrict__ part, int n, double* out, int slot) {
     __shared__ double red[kB];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kB) acc += part[i];
+    int i = threadIdx.x;
+    for (; i + 3 * kB < n; i += 4 * kB) {  // four loads in flight, added in order
+        const double a = part[i], b = part[i + kB], c = part[i + 2 * kB], d = part[i + 3 * kB];
+        acc += a;
+        acc += b;
+        acc += c;
+        acc += d;
+    }
+    for (; i < n; i += kB) acc += part[i];
     double s = block_sum(acc, red);
     if (threadIdx.x == 0) out[slot] = s;
 }
@@ -904,11 +923,21 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
 // three launches: elements, gather (+ v . Av partials), update (+ z . r
 // partials).
 
-// sum of part[0..n) in a fixed order; every thread of the block returns it
+// sum of part[0..n) in a fixed order, formed by every wave on its own (lane
+// strided sums, then the butterfly): no barrier; every thread returns it
 __device__ __forceinline__ double reduce_parts(const double* __restrict__ part, int n, double* red) {
+    (void)red;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kB) acc += part[i];
-    return block_sum(acc, red);
+    int i = threadIdx.x & (kWave - 1);
+    for (; i + 3 * kWave < n; i += 4 * kWave) {  // four loads in flight, added in order
+        const double a = part[i], b = part[i + kWave], c = part[i + 2 * kWave], d = part[i + 3 * kWave];
+        acc += a;
+        acc += b;
+        acc += c;
+        acc += d;
+    }
+    for (; i < n; i += kWave) acc += part[i];
+    return wave_sum(acc);
 }
 
 struct CgVec {
@@ -1037,6 +1066,24 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
     }
 }
 
+// lanes per vertex in the per-vertex gathers
+constexpr int kGather = 32;
+
+// acc + s[e] + s[e + kGather] + ... for e < end, in that order; four loads
+// are issued before their adds, so a long list (a vertex with hundreds of
+// contact corners) waits for one memory latency per four entries
+__device__ __forceinline__ V3 gather_add(V3 acc, const float4* __restrict__ s, int e, int end) {
+    for (; e + 3 * kGather < end; e += 4 * kGather) {
+        const float4 a = s[e], b = s[e + kGather], c = s[e + 2 * kGather], d = s[e + 3 * kGather];
+        acc = acc + v3(a);
+        acc = acc + v3(b);
+        acc = acc + v3(c);
+        acc = acc + v3(d);
+    }
+    for (; e < end; e += kGather) acc = acc + v3(s[e]);
+    return acc;
+}
+
 struct Incid {
     const int* vf_off;  // vertex -> face corner entries of fc (list order)
     const int* vh_off;  // vertex -> hinge corner entries of hc / hdg (list order)
@@ -1048,7 +1095,6 @@ struct Incid {
 // incidence lists are strided over the group and reduced with a fixed
 // butterfly), so a vertex with many incident contacts does not hold up a
 // whole wave.
-constexpr int kGather = 32;
 
 // Gradient and Jacobi diagonal per vertex: diff_k leaves the M2GT term in
 // grad / diag and every element's term in its slot of the vertex lists
@@ -1065,18 +1111,15 @@ __global__ void __launch_bounds__(kB) diff_gather_k(int nV, int use_elem, int us
     const int v = (int)(t / kGather), g = (int)(t % kGather);
     V3 ga = mk(0.0f, 0.0f, 0.0f), da = ga;
     if (v < nV) {
-        if (use_elem)
-            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) ga = ga + v3(fc[e]);
-        if (use_hinge)
-            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) {
-                ga = ga + v3(hc[e]);
-                da = da + v3(hdg[e]);
-            }
-        if (use_contact)
-            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) {
-                ga = ga + v3(cslot[e]);
-                da = da + v3(cdg[e]);
-            }
+        if (use_elem) ga = gather_add(ga, fc, ic.vf_off[v] + g, ic.vf_off[v + 1]);
+        if (use_hinge) {
+            ga = gather_add(ga, hc, ic.vh_off[v] + g, ic.vh_off[v + 1]);
+            da = gather_add(da, hdg, ic.vh_off[v] + g, ic.vh_off[v + 1]);
+        }
+        if (use_contact) {
+            ga = gather_add(ga, cslot, ic.vc_off[v] + g, ic.vc_off[v + 1]);
+            da = gather_add(da, cdg, ic.vc_off[v] + g, ic.vc_off[v + 1]);
+        }
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
@@ -1106,12 +1149,9 @@ __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2g
     const int v = (int)(t / kGather), g = (int)(t % kGather);
     V3 acc = mk(0.0f, 0.0f, 0.0f);
     if (v < nV) {
-        if (use_elem)
-            for (int e = ic.vf_off[v] + g; e < ic.vf_off[v + 1]; e += kGather) acc = acc + v3(fc[e]);
-        if (use_hinge)
-            for (int e = ic.vh_off[v] + g; e < ic.vh_off[v + 1]; e += kGather) acc = acc + v3(hc[e]);
-        if (use_contact)
-            for (int e = ic.vc_off[v] + g; e < ic.vc_off[v + 1]; e += kGather) acc = acc + v3(cslot[e]);
+        if (use_elem) acc = gather_add(acc, fc, ic.vf_off[v] + g, ic.vf_off[v + 1]);
+        if (use_hinge) acc = gather_add(acc, hc, ic.vh_off[v] + g, ic.vh_off[v + 1]);
+        if (use_contact) acc = gather_add(acc, cslot, ic.vc_off[v] + g, ic.vc_off[v + 1]);
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
