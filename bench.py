@@ -16,6 +16,8 @@ split on this mesh (results unchanged).  Prints one JSON line with:
 - roofline: the dominant kernel's algorithmic bytes over its HIP-event time,
   and its HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE,
   WRITE_SIZE) over one stage-2 run, scaled by a plain copy's counters;
+- path_roofline: the whole timed step's algorithmic bytes (the per-round and
+  per-Newton-iteration byte model of the hot path) over its wall time;
 - cpu_baseline: the CPU oracle on this box's cores -- the whole stage-2 run
   (also compared bit for bit with this run's stage 2: stage2_vs_oracle) and
   stage 3 timed per Newton iteration and scaled;
@@ -589,6 +591,37 @@ def main():
                 "avg_launch_us": round(1e3 * dom["total_ms"] / max(dom["launches"], 1), 2),
                 "launches": dom["launches"]}
 
+    # whole-path roofline with the byte model of the hot-path contract:
+    # stage 2 sum over rounds of 231 F_r + 128 V_r (+ 80 C_r, the guard's
+    # candidate pairs, not counted here: a lower bound), V_r = F_r / 2 on
+    # these closed meshes; stage 3 per Newton iteration 40 CG iterations of
+    # 160V + 32F + 24E + 68A, one full energy pass 28V + 32F + 24E + 20K and
+    # ten line-search passes 28V + 32F + 24E + 4K + 20A, the gradient pass
+    # 52V + 32F + 24E + 68A and the detection 12V + 24F + 24E + 16K, plus 24V
+    # per closest-point update, with the stage-3 mesh's sizes, the contacts K
+    # found at each detection and the active ones A (distance below d_hat):
+    # only those enter the gradient and the products, and the line-search
+    # passes read the others' distances only
+    Fr = np.concatenate([[nF0], hist[:-1]]).astype(np.float64)
+    B2 = float((231.0 * Fr + 128.0 * (Fr / 2.0)).sum())
+    B3 = 0.0
+    if use_s3:
+        st3 = pm.system.stats()
+        v3, f3 = pm.stage3_in
+        Vs, Fs = float(len(v3)), float(len(f3))
+        Es = 1.5 * Fs
+        n_nt, K, A = float(st3["newton"]), float(st3["contacts"]), float(pm.system.active_contacts())
+        per_newton = (40 * (160 * Vs + 32 * Fs + 24 * Es) + 11 * (28 * Vs + 32 * Fs + 24 * Es)
+                      + (52 * Vs + 32 * Fs + 24 * Es) + (12 * Vs + 24 * Fs + 24 * Es))
+        B3 = n_nt * per_newton + K * (20 + 10 * 4 + 16) + A * (40 * 68 + 10 * 20 + 68) + 5 * 24 * Vs
+    path_ach = (B2 + B3) / per_step / 1e9
+    path_roofline = {"bound": "hbm", "achieved": round(path_ach, 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(path_ach / 8000.0, 5), "stage2_bytes": round(B2), "stage3_bytes": round(B3),
+                     "model": "stage 2: sum_r 231 F_r + 128 V_r (V_r = F_r / 2; the 80 C_r guard-pair term not "
+                              "counted); stage 3 per Newton iteration: 40 CG passes over the active contacts A, "
+                              "one full and ten line-search energy passes, the gradient pass and the detection "
+                              "over the K contacts found"}
+
     # output quality, untimed: igl-semantics Hausdorff and sampled Chamfer
     # distances of the output to the input surface (GPU metrics)
     from pamo_amd import metrics
@@ -629,6 +662,7 @@ def main():
         "faces_per_s": round(nF0 / per_step, 1),
         "face_rounds": face_rounds,
         "roofline": roofline,
+        "path_roofline": path_roofline,
         "cpu_baseline": cpu,
         "stage2_vs_oracle": parity,
         "autotune": tune,

@@ -325,6 +325,11 @@ int pamo_s3_debug(pamo_s3* s3, int what, const double* h_in, long long n_in, dou
  * [3] CG iterations, [4] line-search evaluations, [5] current detection
  * radius, [6] do_nothing flag, [7] number of hinges. */
 int pamo_s3_stats(pamo_s3* s3, double* out8);
+/* Active contacts (distance below d_hat at the start of a Newton iteration)
+ * summed over the Newton iterations since the mesh was set: the pairs the
+ * gradient pass and the CG products touch (measurement; no counterpart in
+ * the reference). */
+int pamo_s3_active_contacts(pamo_s3* s3, long long* total);
 
 /* Run-time options (no counterpart in the reference's interface, which
  * fixes its CUDA-graph capture of the PCG loop in config.py:21).

@@ -112,6 +112,7 @@ SIGNATURES = [
     ("pamo_s3_set_dense", c_int, [c_vp, c_vp, c_vp, c_double, c_int]),
     ("pamo_s3_debug", c_int, [c_vp, c_int, c_vp, c_ll, c_vp, c_ll, ctypes.POINTER(c_ll)]),
     ("pamo_s3_stats", c_int, [c_vp, c_vp]),
+    ("pamo_s3_active_contacts", c_int, [c_vp, ctypes.POINTER(c_ll)]),
     ("pamo_s3_set_option", c_int, [c_vp, c_int, c_ll]),
     ("pamo_s3_prepare_host", c_int, [c_vp, c_int, c_vp, c_int, c_double, ctypes.POINTER(c_double), c_vp,
                                      ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_int), c_vp, ctypes.POINTER(c_double)]),

@@ -1890,9 +1890,7 @@ __global__ void store_tris_k(Mesh m, int n, int* __restrict__ aos) {
 // whole GPU (face_mean_k, rows of `pitch` floats per coordinate).  A float32
 // running sum cannot be split, so one workgroup adds them up: waves 1..15
 // copy the next chunk of means to LDS while lanes 0..2 of wave 0 add up the
-// current chunk, one coordinate per lane, reading 32 values ahead of the
-// additions so that the chain of dependent adds is not held up by LDS
-// latency.
+// current chunk, one coordinate per lane, 64 values per LDS batch.
 constexpr int kCentThreads = 1024;
 constexpr int kCentChunk = 2048;            // multiple of 32
 constexpr int kCentPitch = kCentChunk + 4;  // rows start in different LDS banks
@@ -1925,31 +1923,26 @@ __device__ inline void centroid_fill(const float* __restrict__ means, long long 
     }
 }
 
-// acc + row[0] + ... + row[n - 1], left to right; row is 16-byte aligned
+// acc + row[0] + ... + row[n - 1], left to right; row is 16-byte aligned.
+// Each batch of 64 values is loaded whole, then added: the adds wait for the
+// batch's first load only (a prefetch of the next batch between them made
+// the compiler wait on the prefetch before the adds).
 __device__ __forceinline__ float centroid_chain(float acc, const float* __restrict__ row, int n) {
     const float4* r4 = reinterpret_cast<const float4*>(row);
-    const int full = n >> 5;  // batches of 32
-    float4 cur[8], nxt[8];
-    if (full > 0) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cur[j] = r4[j];
-    }
+    const int full = n >> 6;  // batches of 64
     for (int b = 0; b < full; ++b) {
-        if (b + 1 < full) {
+        float4 v[16];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) nxt[j] = r4[8 * (b + 1) + j];
+        for (int j = 0; j < 16; ++j) v[j] = r4[16 * b + j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            acc = acc + v[j].x;
+            acc = acc + v[j].y;
+            acc = acc + v[j].z;
+            acc = acc + v[j].w;
         }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            acc = acc + cur[j].x;
-            acc = acc + cur[j].y;
-            acc = acc + cur[j].z;
-            acc = acc + cur[j].w;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
     }
-    for (int i = full << 5; i < n; ++i) acc = acc + row[i];
+    for (int i = full << 6; i < n; ++i) acc = acc + row[i];
     return acc;
 }
 

@@ -123,7 +123,7 @@ struct Ctx {
     int* ws = nullptr;
     long long cap_ws = 0;
     // statistics
-    long long stat_contacts = 0, stat_newton = 0, stat_cg = 0, stat_ls = 0;
+    long long stat_contacts = 0, stat_newton = 0, stat_cg = 0, stat_ls = 0, stat_active = 0;
     // the CG loop replayed from a captured graph (updated each solve); off by
     // default: on the 20k-face headline output five steps took 0.169 s with
     // the graph and 0.163 s launching the kernels one by one
@@ -1474,6 +1474,7 @@ static int compute_diff(Ctx& c, int nC, int mask) {
         PAMO_HIP(hipMemcpyAsync(c.hcount, c.n_act, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         PAMO_HIP(hipStreamSynchronize(c.stream));
         c.nA = c.hcount[0];
+        c.stat_active += c.nA;
     } else {
         PAMO_TRY(fill(c.n_act, 0, 1, c.stream));
     }
@@ -1862,7 +1863,7 @@ static void host_prepare(const float* V, int nV, const int* F, int nF, double sc
 // another (from the first row) in FP64; the same sequential sum of the
 // float32 vertices widened to FP64, in one workgroup: waves 1..15 copy the
 // next chunk of coordinates to LDS while lanes 0..2 of wave 0 add up the
-// current chunk, one coordinate per lane, 32 values read ahead.
+// current chunk, one coordinate per lane, 64 values per LDS batch.
 constexpr int kSeqThreads = 1024;
 constexpr int kSeqChunk = 2048;            // multiple of 32
 constexpr int kSeqPitch = kSeqChunk + 4;
@@ -1875,31 +1876,24 @@ __device__ inline void seq_fill(const float* __restrict__ v, long long base, int
     }
 }
 
-// acc + row[0] + ... + row[n - 1] in FP64, left to right (row 16-byte aligned)
+// acc + row[0] + ... + row[n - 1] in FP64, left to right (row 16-byte
+// aligned), 64 values per LDS batch as in centroid_chain
 __device__ __forceinline__ double seq_chain(double acc, const float* __restrict__ row, int n) {
     const float4* r4 = reinterpret_cast<const float4*>(row);
-    const int full = n >> 5;
-    float4 cur[8], nxt[8];
-    if (full > 0) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cur[j] = r4[j];
-    }
+    const int full = n >> 6;
     for (int b = 0; b < full; ++b) {
-        if (b + 1 < full) {
+        float4 v[16];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) nxt[j] = r4[8 * (b + 1) + j];
+        for (int j = 0; j < 16; ++j) v[j] = r4[16 * b + j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            acc = acc + (double)v[j].x;
+            acc = acc + (double)v[j].y;
+            acc = acc + (double)v[j].z;
+            acc = acc + (double)v[j].w;
         }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            acc = acc + (double)cur[j].x;
-            acc = acc + (double)cur[j].y;
-            acc = acc + (double)cur[j].z;
-            acc = acc + (double)cur[j].w;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
     }
-    for (int i = full << 5; i < n; ++i) acc = acc + (double)row[i];
+    for (int i = full << 6; i < n; ++i) acc = acc + (double)row[i];
     return acc;
 }
 
@@ -2235,7 +2229,7 @@ int pamo_s3_set_mesh(pamo_s3* s, const float* d_v, int n_v, const int* d_f, int 
     c.contact_trees_built = false;
     c.do_nothing = do_nothing != 0;
     c.radius = c.P.radius0;
-    c.stat_contacts = c.stat_newton = c.stat_cg = c.stat_ls = 0;
+    c.stat_contacts = c.stat_newton = c.stat_cg = c.stat_ls = c.stat_active = 0;
     PAMO_TRY(pamo::s3::ensure_ws(c, std::max(n_f, n_e) + 64));
     PAMO_TRY(pamo::s3::ensure_part(c, (long long)n_v + n_f + n_h + 1024));
     PAMO_TRY(pamo::s3::ensure_contacts(c, 4 * n_v + 1024));
@@ -2645,6 +2639,13 @@ int pamo_s3_stats(pamo_s3* s, double* out8) {
     out8[5] = (double)c.radius;
     out8[6] = c.do_nothing ? 1.0 : 0.0;
     out8[7] = (double)c.nH;
+    return 0;
+}
+
+int pamo_s3_active_contacts(pamo_s3* s, long long* total) {
+    pamo::clear_error();
+    PAMO_CHECK(s && total, "pamo_s3_active_contacts: null argument");
+    *total = reinterpret_cast<Ctx*>(s)->stat_active;
     return 0;
 }
 

@@ -325,6 +325,13 @@ class Stage3System:
         return {"energy": o[0], "contacts": int(o[1]), "newton": int(o[2]), "cg": int(o[3]), "line_search": int(o[4]),
                 "radius": o[5], "do_nothing": bool(o[6]), "n_hinges": int(o[7])}
 
+    def active_contacts(self) -> int:
+        """Active contacts summed over the Newton iterations since set_mesh
+        (measurement)."""
+        n = ctypes.c_longlong(0)
+        _hip.check(_hip.lib().pamo_s3_active_contacts(self.ptr, ctypes.byref(n)), "pamo_s3_active_contacts")
+        return int(n.value)
+
     def set_cg_graph(self, on: bool):
         """Replay the PCG loop from a captured HIP graph (default) or launch
         its kernels one by one; results do not change."""
