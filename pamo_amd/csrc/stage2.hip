@@ -593,10 +593,11 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_TRY(topology_and_quadrics(c, m, nP, nT));
     const int nE = c.n_edges;
     CostParams cp{edge_s, tres, is_stuck ? 1 : 0, c.inv_epoch};
-    const double cost_bytes = 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE;
+    // (+ 16 B per vertex: the minimum keys the fused propagation lowers)
+    const double cost_bytes = 12.0 * nE + 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP + 1.0 * nP + 4.0 * nE + 16.0 * nP;
 #define PAMO_EDGE_COST(K, lanes)                                                                              \
     LAUNCHP(K_EDGE_COST, cost_bytes, K, (long long)nE * (lanes), m, c.ring_off, c.rv, c.eu, c.ev, c.eord, nE,     \
-            c.invalid, c.Q, cp, c.edge_cost)
+            c.invalid, c.Q, cp, c.edge_cost, (unsigned long long*)c.vmin)
     if (c.edge_lanes == 8) {
         if (c.placement) PAMO_EDGE_COST((edge_cost_group_k<true, 8>), 8);
         else PAMO_EDGE_COST((edge_cost_group_k<false, 8>), 8);
@@ -608,9 +609,8 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
         else PAMO_EDGE_COST(edge_cost_k<false>, 1);
     }
 #undef PAMO_EDGE_COST
-    // independent set + collapse
-    LAUNCHP(K_PROPAGATE, 12.0 * nE + 16.0 * nP, propagate_k, nE, c.eu, c.ev, nE, c.edge_cost,
-            (unsigned long long*)c.vmin);
+    // independent set + collapse (the cost kernels have lowered the
+    // per-vertex minimum keys)
     const double sel_bytes = 16.0 * nE + 4.0 * nP + 48.0 * nT + 8.0 * nP;
 #define PAMO_SELECT(K, lanes)                                                                                 \
     LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, K, (long long)(nE + 1) * (lanes), m, orig_mesh(c), c.occ, c.ring_off,   \

@@ -439,8 +439,21 @@ __global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __
     if (e < nE) order[atomicAdd(&cursor[eu[e]], 1)] = e;
 }
 
+__device__ __forceinline__ uint64_t edge_key(uint32_t c, int e) { return ((uint64_t)c << 32) | (uint32_t)e; }
+
+// propagate_edge_cost_kernel (:587-610) fused into the cost kernels: the
+// edge's key (cost << 32 | edge) lowers the per-vertex minimum key of both
+// endpoints (see propagate_k).  An edge whose cost the round leaves as it was
+// (stuck round, link count below 2) propagates the stored value, as the
+// separate pass read it.
+__device__ __forceinline__ void prop_key(unsigned long long* __restrict__ vmin, int u, int v, uint32_t c, int e) {
+    const unsigned long long k = edge_key(c, e);
+    atomicMin(&vmin[u], k);
+    atomicMin(&vmin[v], k);
+}
+
 // compute_edge_cost_kernel (cusimp_free.cu:382-585), one lane per edge; lane
-// t takes edge order[t] and writes cost[order[t]].
+// t takes edge order[t] and writes cost[order[t]], then propagates its key.
 
 template <bool kOptimal>
 __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restrict__ off,
@@ -448,23 +461,32 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
                                                       const int* __restrict__ ev, const int* __restrict__ order,
                                                       int nE, const uint32_t* __restrict__ invalid,
                                                       const float4* __restrict__ Q, CostParams cp,
-                                                      uint32_t* __restrict__ cost) {
+                                                      uint32_t* __restrict__ cost,
+                                                      unsigned long long* __restrict__ vmin) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nE) return;
     const int e = order[t];
     const int u = eu[e], v = ev[e];
     if (invalid[u] == cp.inv_epoch && invalid[v] == cp.inv_epoch) {
         cost[e] = 0xFFFFFFFFu;
+        prop_key(vmin, u, v, 0xFFFFFFFFu, e);
         return;
     }
-    if (cp.is_stuck) return;
+    if (cp.is_stuck) {
+        prop_key(vmin, u, v, cost[e], e);
+        return;
+    }
     const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
     const int dup = link_dup(rv, u, ub, ue, v, vb, ve);
     if (dup > 2) {
         cost[e] = 0xFFFFFFFFu;
+        prop_key(vmin, u, v, 0xFFFFFFFFu, e);
         return;
     }
-    if (dup != 2) return;
+    if (dup != 2) {
+        prop_key(vmin, u, v, cost[e], e);
+        return;
+    }
     const F3 p0 = ld(m, u), p1 = ld(m, v);
     float M[10];
     edge_quadric(Q, u, v, M);
@@ -491,6 +513,7 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
             nn = f3divs(nn, f3norm(nn));
             if (f3dot(on, nn) < 0) {
                 cost[e] = 0xFFFFFFFFu;
+                prop_key(vmin, u, v, 0xFFFFFFFFu, e);
                 return;
             }
             float l01 = f3norm(f3sub(n0, n1)), l12 = f3norm(f3sub(n1, n2)), l20 = f3norm(f3sub(n2, n0));
@@ -512,7 +535,9 @@ __global__ void __launch_bounds__(kBlock) edge_cost_k(Mesh m, const int* __restr
     float r3 = M3 * x0 + M6 * x1 + M8 * x2 + M9 * x3;
     float qc = (r0 * x0 + r1 * x1 + r2 * x2 + r3 * x3) / (cp.edge_s * cp.edge_s);
     float tot = qc + elen + qa / (float)ntri * cp.tres;
-    cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    const uint32_t ce = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    cost[e] = ce;
+    prop_key(vmin, u, v, ce, e);
 }
 
 // link_dup's count over the outer-ring faces g, g + kG, ... (one lane's
@@ -554,7 +579,8 @@ __global__ void __launch_bounds__(kBlock) edge_cost_group_k(Mesh m, const int* _
                                                             const int* __restrict__ ev, const int* __restrict__ order,
                                                             int nE, const uint32_t* __restrict__ invalid,
                                                             const float4* __restrict__ Q, CostParams cp,
-                                                            uint32_t* __restrict__ cost) {
+                                                            uint32_t* __restrict__ cost,
+                                                            unsigned long long* __restrict__ vmin) {
     static_assert(kWave % kG == 0, "a group must not straddle waves");
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = (int)(threadIdx.x & (kG - 1));
@@ -563,19 +589,31 @@ __global__ void __launch_bounds__(kBlock) edge_cost_group_k(Mesh m, const int* _
     const int e = order[t];
     const int u = eu[e], v = ev[e];
     if (invalid[u] == cp.inv_epoch && invalid[v] == cp.inv_epoch) {
-        if (g == 0) cost[e] = 0xFFFFFFFFu;
+        if (g == 0) {
+            cost[e] = 0xFFFFFFFFu;
+            prop_key(vmin, u, v, 0xFFFFFFFFu, e);
+        }
         return;
     }
-    if (cp.is_stuck) return;
+    if (cp.is_stuck) {
+        if (g == 0) prop_key(vmin, u, v, cost[e], e);
+        return;
+    }
     const int ub = off[u], ue = off[u + 1], vb = off[v], ve = off[v + 1];
     int dup = link_dup_share<kG>(rv, u, ub, ue, v, vb, ve, g);
 #pragma unroll
     for (int d = kG / 2; d > 0; d >>= 1) dup += __shfl_xor(dup, d, kG);
     if (dup > 2) {
-        if (g == 0) cost[e] = 0xFFFFFFFFu;
+        if (g == 0) {
+            cost[e] = 0xFFFFFFFFu;
+            prop_key(vmin, u, v, 0xFFFFFFFFu, e);
+        }
         return;
     }
-    if (dup != 2) return;
+    if (dup != 2) {
+        if (g == 0) prop_key(vmin, u, v, cost[e], e);
+        return;
+    }
     const F3 p0 = ld(m, u), p1 = ld(m, v);
     float M[10];
     edge_quadric(Q, u, v, M);
@@ -616,7 +654,10 @@ __global__ void __launch_bounds__(kBlock) edge_cost_group_k(Mesh m, const int* _
             }
         }
         if (__ballot(flip) & gmask) {
-            if (g == 0) cost[e] = 0xFFFFFFFFu;
+            if (g == 0) {
+                cost[e] = 0xFFFFFFFFu;
+                prop_key(vmin, u, v, 0xFFFFFFFFu, e);
+            }
             return;
         }
 #pragma unroll
@@ -643,11 +684,12 @@ __global__ void __launch_bounds__(kBlock) edge_cost_group_k(Mesh m, const int* _
     float r3 = M3 * x0 + M6 * x1 + M8 * x2 + M9 * x3;
     float qc = (r0 * x0 + r1 * x1 + r2 * x2 + r3 * x3) / (cp.edge_s * cp.edge_s);
     float tot = qc + elen + qa / (float)ntri * cp.tres;
-    cost[e] = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    const uint32_t ce = sat_u32(tclamp(tot, 0.0f, 10.0f) / 10.0f * 4294967296.0f);
+    cost[e] = ce;
+    prop_key(vmin, u, v, ce, e);
 }
 
 // ---------------------------------------------------------------- independent set + collapse (A8, A9)
-__device__ __forceinline__ uint64_t edge_key(uint32_t c, int e) { return ((uint64_t)c << 32) | (uint32_t)e; }
 
 // propagate_edge_cost_kernel (:587-610) keeps, per face t, the minimum key
 // over edges whose endpoint one-rings contain t.  A face lies in ring(x)
