@@ -615,7 +615,7 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
 #define PAMO_SELECT(K, lanes)                                                                                 \
     LAUNCHP(K_SELECT_COLLAPSE, sel_bytes, K, (long long)(nE + 1) * (lanes), m, orig_mesh(c), c.occ, c.ring_off,   \
             c.ring, (const int4*)c.rv, c.eu, c.ev, nE, c.edge_cost, (const unsigned long long*)c.vmin, collapse_t,  \
-            c.esel, (const float4*)c.Q)
+            c.esel, (const float4*)c.Q, c.changed)
     if (c.ring_lanes == 8) {
         if (c.placement) PAMO_SELECT((select_collapse_group_k<true, 8>), 8);
         else PAMO_SELECT((select_collapse_group_k<false, 8>), 8);
@@ -632,10 +632,8 @@ static int round_device(Ctx& c, int nP, int nT, float scale, float thr, bool is_
     PAMO_TRY(scan_op(CleanupOp{m, c.eu, c.ev, nE, nT, c.recs, c.dcnt}, (long long)std::max(nE, nT), c.lb, c.stream,
                      K_LINE_FLAG, 8.0 * nE + 16.0 * nP + 4.0 * nE + 12.0 * nT));
     hipLaunchKernelGGL(line_apply_k, dim3(1), dim3(1), 0, c.stream, m, c.occ, c.ring_off, c.ring, c.eu, c.ev,
-                       (const int*)c.recs, c.dcnt);
+                       (const int*)c.recs, c.dcnt, c.changed);
     PAMO_HIP(hipGetLastError());
-    LAUNCHP(K_MARK_CHANGED, 1.0 * nT, mark_changed_k, nE, c.recs, c.dcnt, c.ring_off, c.ring, c.eu, c.ev,
-            c.changed);
     // guard + undo loop (:1014-1040).  The guard is queued before the record
     // counts are read back, so the read waits on the guard's kernels instead
     // of stalling the stream; a round without undo candidates ignores its

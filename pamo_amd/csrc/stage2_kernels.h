@@ -769,7 +769,8 @@ __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int*
                                                             int nE, const uint32_t* __restrict__ cost,
                                                             const unsigned long long* __restrict__ vmin,
                                                             uint32_t collapse_t, int* __restrict__ sel,
-                                                            const float4* __restrict__ Q) {
+                                                            const float4* __restrict__ Q,
+                                                            uint8_t* __restrict__ changed) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > nE) return;
     if (e == nE) {
@@ -795,6 +796,9 @@ __global__ void __launch_bounds__(kBlock) select_collapse_k(Mesh m, Orig o, int*
         }
         if (ok) {
             s = 1;
+            // the record's one-ring faces are the guard's queries
+            for (int a = off[u]; a < off[u + 1]; ++a) changed[ring[a]] = 1;
+            for (int a = off[v]; a < off[v + 1]; ++a) changed[ring[a]] = 1;
             F3 vm;
             if (kOptimal) {
                 float M[10];
@@ -840,7 +844,8 @@ __global__ void __launch_bounds__(kBlock) select_collapse_group_k(Mesh m, Orig o
                                                                   const uint32_t* __restrict__ cost,
                                                                   const unsigned long long* __restrict__ vmin,
                                                                   uint32_t collapse_t, int* __restrict__ sel,
-                                                                  const float4* __restrict__ Q) {
+                                                                  const float4* __restrict__ Q,
+                                                                  uint8_t* __restrict__ changed) {
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = (int)(threadIdx.x & (kG - 1));
     const long long ee = gt / kG;
@@ -873,6 +878,8 @@ __global__ void __launch_bounds__(kBlock) select_collapse_group_k(Mesh m, Orig o
         if (g == 0) sel[e] = 0;
         return;
     }
+    // the record's one-ring faces are the guard's queries
+    for (int i = g; i < nf; i += kG) changed[ring[i < nu ? ub + i : vb + (i - nu)]] = 1;
     F3 vm;
     if (kOptimal) {
         float M[10];
@@ -1038,29 +1045,21 @@ __global__ void __launch_bounds__(kBlock) pasp_collapse_k(Mesh m, int* occ, cons
 // lane walks the list CleanupOp built, not the whole edge array.
 __global__ void line_apply_k(Mesh m, int* occ, const int* __restrict__ off, const int* __restrict__ ring,
                              const int* __restrict__ eu, const int* __restrict__ ev, const int* __restrict__ recs,
-                             const int* cnt) {
+                             const int* cnt, uint8_t* __restrict__ changed) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int base = cnt[C_NREG];
     const int nl = cnt[C_LINE];
     for (int w = 0; w < nl; ++w) {
         const int e = recs[base + w];
         const int u = eu[e], v = ev[e];
+        // the record's one-ring faces are the guard's queries (the regular
+        // records' are marked by the selection kernel)
+        for (int a = off[u]; a < off[u + 1]; ++a) changed[ring[a]] = 1;
+        for (int a = off[v]; a < off[v + 1]; ++a) changed[ring[a]] = 1;
         st(m, v, F3{0.0f, 0.0f, 0.0f});
         occ[v] = 0;
         rewrite_rings(m, off, ring, u, v);
     }
-}
-
-__global__ void mark_changed_k(const int* __restrict__ recs, const int* __restrict__ cnt,
-                               const int* __restrict__ off, const int* __restrict__ ring,
-                               const int* __restrict__ eu, const int* __restrict__ ev,
-                               uint8_t* __restrict__ changed) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= cnt[C_NREG] + cnt[C_LINE]) return;
-    int e = recs[r];
-    int u = eu[e], v = ev[e];
-    for (int a = off[u]; a < off[u + 1]; ++a) changed[ring[a]] = 1;
-    for (int a = off[v]; a < off[v + 1]; ++a) changed[ring[a]] = 1;
 }
 
 // ---------------------------------------------------------------- LBVH (A11)
