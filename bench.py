@@ -577,7 +577,9 @@ def main():
     achieved = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9 if dom["total_ms"] > 0 else 0.0
     traffic, tdetail = (None, "not measured (--no-traffic)")
     if not args.no_traffic:
-        traffic, tdetail = measure_traffic(dom["name"] + "_k")
+        # the profiler's name is a fragment of every variant's kernel name
+        # (guard_query_k, guard_query_w4_k, edge_cost_group_k, ...)
+        traffic, tdetail = measure_traffic(dom["name"])
         print(f"[bench] traffic of {dom['name']}: {traffic} bytes per launch ({tdetail})", file=sys.stderr,
               flush=True)
     roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": 8000.0,

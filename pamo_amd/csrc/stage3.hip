@@ -923,10 +923,24 @@ __global__ void __launch_bounds__(kB) diff_k(const float4* __restrict__ x, Topo 
 // three launches: elements, gather (+ v . Av partials), update (+ z . r
 // partials).
 
-// sum of part[0..n) in a fixed order, formed by every wave on its own (lane
-// strided sums, then the butterfly): no barrier; every thread returns it
+// sum of part[0..n) in a fixed order; every thread returns it.  Up to 4
+// partials per lane every wave forms it on its own (lane-strided sums, then
+// the butterfly), with no barrier; more partials are spread over the whole
+// block (all its threads must call).
 __device__ __forceinline__ double reduce_parts(const double* __restrict__ part, int n, double* red) {
-    (void)red;
+    if (n > 4 * kWave) {
+        double acc = 0.0;
+        int i = threadIdx.x;
+        for (; i + 3 * kB < n; i += 4 * kB) {
+            const double a = part[i], b = part[i + kB], c = part[i + 2 * kB], d = part[i + 3 * kB];
+            acc += a;
+            acc += b;
+            acc += c;
+            acc += d;
+        }
+        for (; i < n; i += kB) acc += part[i];
+        return block_sum(acc, red);
+    }
     double acc = 0.0;
     int i = threadIdx.x & (kWave - 1);
     for (; i + 3 * kWave < n; i += 4 * kWave) {  // four loads in flight, added in order
