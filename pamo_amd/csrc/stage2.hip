@@ -130,6 +130,7 @@ struct Ctx {
     uint32_t* edge_cost = nullptr;
     int *esel = nullptr, *recs = nullptr, *pflag = nullptr, *pushed = nullptr;
     int* eord = nullptr;  // edge ids grouped by smaller endpoint (edge-cost launch order)
+    int* erank = nullptr;  // each edge's rank among the edges of its smaller endpoint
     int* shuf = nullptr;
     int* ws = nullptr;
     LookbackScan lb;  // single-launch scans of the round loop
@@ -262,6 +263,7 @@ static int alloc_edges(Ctx& c, size_t ae) {
     PAMO_TRY(dalloc(&c.pushed, ae + 1, c.stream));
     PAMO_TRY(dalloc(&c.edge_cost, ae + 1, c.stream));
     PAMO_TRY(dalloc(&c.eord, ae + 1, c.stream));
+    PAMO_TRY(dalloc(&c.erank, ae + 1, c.stream));
     return 0;
 }
 
@@ -339,7 +341,7 @@ static void free_all(Ctx& c) {
                     c.undo_prev, c.invalid, c.ti, c.tj, c.tk, c.oi, c.oj, c.ok, c.ni, c.nj, c.nk, c.ring, c.ring_raw, c.ring_own, c.rv,
                     c.edge_off, c.vmin, c.changed, c.flagged, c.face_scan, c.qlist, c.bkey, c.bval, c.bkey2,
                     c.bval2, c.bleft, c.bright, c.bparent, c.brange, c.bbox, c.fbox, c.bpre, c.bsuf, c.blk, c.blkpre,
-                    c.blksuf, c.sblk, c.blktab, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord,
+                    c.blksuf, c.sblk, c.blktab, c.box_part, c.eu, c.ev, c.edge_cost, c.esel, c.recs, c.pflag, c.pushed, c.eord, c.erank,
                     c.shuf, c.ws, c.dcnt, c.cent_out, c.ulist, c.cent_means};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -522,8 +524,8 @@ static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     // read then waits on them instead of stalling the stream)
     PAMO_TRY(scan_exclusive_lb(c.edge_off, c.edge_off, nT + 1, c.lb, c.stream));
     PAMO_CHECK((size_t)2 * nT <= c.alloc_edges, "edge buffers smaller than two edges per face");
-    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
-            c.eord_cnt);
+    LAUNCHP(K_EDGE_FILL, 16.0 * nT + 12.0 * 1.5 * nT, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
+            c.eord_cnt, c.erank);
     // quadrics
     const double quad_bytes = 4.0 * nP + 48.0 * nT + 16.0 * nP + 48.0 * nP;
     if (c.ring_lanes == 8)
@@ -539,13 +541,13 @@ static int topology_and_quadrics(Ctx& c, Mesh m, int nP, int nT) {
     PAMO_TRY(grow_edges(c, (size_t)nE, &realloc));
     if (realloc) {  // fresh edge buffers: fill and count them again
         PAMO_HIP(hipMemsetAsync(c.eord_cnt, 0, sizeof(int) * (nP + 1), c.stream));
-        LAUNCHP(K_EDGE_FILL, 16.0 * nT + 8.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
-                c.eord_cnt);
+        LAUNCHP(K_EDGE_FILL, 16.0 * nT + 12.0 * nE, edge_fill_k, nT, m, nT, c.edge_off, c.eu, c.ev, c.dcnt,
+                c.eord_cnt, c.erank);
     }
     // edge-cost launch order: edges grouped by smaller endpoint (counted by
     // edge_fill_k)
     PAMO_TRY(scan_exclusive_lb(c.eord_cnt, c.eord_cnt, nP + 1, c.lb, c.stream));
-    LAUNCH(edge_order_scatter_k, nE, c.eu, nE, c.eord_cnt, c.eord);
+    LAUNCH(edge_order_scatter_k, nE, c.eu, (const int*)c.erank, nE, (const int*)c.eord_cnt, c.eord);
     return 0;
 }
 

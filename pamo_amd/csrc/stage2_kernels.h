@@ -225,17 +225,19 @@ __global__ void topo_count_k(Mesh m, int nT, int* __restrict__ ring_cnt, int* __
 // Also stores the edge count off[nT] to cnt[C_NEDGE].
 // Also counts the edges per smaller endpoint into order_cnt (zeroed by
 // round_init_k), the histogram of the edge-cost launch order.
+// also counts the edges per smaller endpoint (order_cnt) and keeps each
+// edge's rank among them (rank), for the edge-cost launch order
 __global__ void edge_fill_k(Mesh m, int nT, const int* __restrict__ off, int* __restrict__ eu, int* __restrict__ ev,
-                            int* __restrict__ cnt, int* __restrict__ order_cnt) {
+                            int* __restrict__ cnt, int* __restrict__ order_cnt, int* __restrict__ rank) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f == 0) cnt[C_NEDGE] = off[nT];
     if (f >= nT) return;
     int a = m.i[f], b = m.j[f], c = m.k[f];
     if (face_dead(a, b, c)) return;
     int q = off[f];
-    if (a > b) { eu[q] = b; ev[q] = a; ++q; atomicAdd(&order_cnt[b], 1); }
-    if (b > c) { eu[q] = c; ev[q] = b; ++q; atomicAdd(&order_cnt[c], 1); }
-    if (c > a) { eu[q] = a; ev[q] = c; ++q; atomicAdd(&order_cnt[a], 1); }
+    if (a > b) { eu[q] = b; ev[q] = a; rank[q] = atomicAdd(&order_cnt[b], 1); ++q; }
+    if (b > c) { eu[q] = c; ev[q] = b; rank[q] = atomicAdd(&order_cnt[c], 1); ++q; }
+    if (c > a) { eu[q] = a; ev[q] = c; rank[q] = atomicAdd(&order_cnt[a], 1); ++q; }
 }
 
 // ---------------------------------------------------------------- quadrics and costs (A6, A7)
@@ -428,15 +430,16 @@ __device__ __forceinline__ void edge_quadric(const float4* __restrict__ Q, int u
 }
 
 // Launch order of the edge-cost lanes: edge ids grouped by their smaller
-// endpoint (counting sort: histogram, exclusive scan, scatter).  Vertex ids of
+// endpoint (counting sort: histogram with ranks, exclusive scan, scatter).  Vertex ids of
 // a mesh are spatially coherent while edge ids follow the shuffled face
 // order, so in this order the lanes of a wave read neighbouring rings,
-// positions and quadrics.  The order inside one group depends on atomic
-// timing; it changes only which lane computes an edge, never a result.
-__global__ void edge_order_scatter_k(const int* __restrict__ eu, int nE, int* __restrict__ cursor,
-                                     int* __restrict__ order) {
+// positions and quadrics.  The order inside one group (the ranks edge_fill_k
+// took) depends on atomic timing; it changes only which lane computes an
+// edge, never a result.
+__global__ void edge_order_scatter_k(const int* __restrict__ eu, const int* __restrict__ rank, int nE,
+                                     const int* __restrict__ start, int* __restrict__ order) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nE) order[atomicAdd(&cursor[eu[e]], 1)] = e;
+    if (e < nE) order[start[eu[e]] + rank[e]] = e;
 }
 
 __device__ __forceinline__ uint64_t edge_key(uint32_t c, int e) { return ((uint64_t)c << 32) | (uint32_t)e; }
