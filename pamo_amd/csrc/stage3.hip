@@ -84,6 +84,11 @@ struct Ctx {
     int *act = nullptr, *act_flag = nullptr, *act_pos = nullptr, *n_act = nullptr;
     float4* cslot = nullptr;  // per active-contact corner H dx terms, vertex-list order
     int4* cvert = nullptr;    // per active contact: vertices
+    // per active-contact corner in vertex-list order: its distance gradient
+    // and its contact; per active contact the CG scalar c2 (g . dx)
+    float4* cdv = nullptr;
+    int* cj = nullptr;
+    float* sA = nullptr;
     float* c2j = nullptr;     // per active contact: barrier'' kappa
     int* c_dst = nullptr;     // list position of active-contact corner slot 4 j + a
     int nA = 0;
@@ -144,7 +149,7 @@ static void ctx_free(Ctx& c) {
                     c.edge, c.hinge, c.area, c.rest_angle, c.rest_elen, c.inv_dm, c.hblock, c.gt_v, c.gt_f,
                     c.cidx, c.ctype1, c.ctype2, c.cd, c.cdd, c.ccount, c.sc, c.ccd_step, c.part, c.ws,
                     c.dense_A, c.dense_b, c.hsym, c.vf_off, c.vf_ent, c.vh_off, c.vh_ent, c.vc_off, c.vc_ent,
-                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.cslot, c.c_dst, c.cvert, c.c2j, c.skey, c.sval, c.skey2,
+                    c.fc, c.hc, c.dn, c.act, c.act_flag, c.act_pos, c.n_act, c.cslot, c.c_dst, c.cvert, c.c2j, c.cdv, c.cj, c.sA, c.skey, c.sval, c.skey2,
                     c.vf_dst, c.vh_dst, c.hdg, c.cdg,
                     c.sval2, c.zr_part, c.vav_part,
                     c.mg_ref, c.mg_first, c.mg_keep, c.mg_pos, c.mg_tab, c.mg_err,
@@ -975,7 +980,7 @@ __device__ __forceinline__ V3 cg_dx(const CgVec& cv, float beta, int v) {
 
 // items: [0, nF) faces -> fc[f_dst[3f + k]]; [nF, +nH) hinges -> hc[h_dst[4h + a]]
 // (vertex-list order, so the gather reads them contiguously);
-// [.., +nA) active contacts -> cslot[c_dst[4 j + a]] = g_a c2 (g . dx);
+// [.., +nA) active contacts -> sA[j] = c2 (g . dx);
 // [.., +3 nV_dense) dense rows
 __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, Topo tp, Params P, int nF, int nH,
                                                  int nA_max, const int* __restrict__ n_act, int n_dense,
@@ -984,8 +989,7 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
                                                  const float4* __restrict__ cdd, long long cstride,
                                                  const double* __restrict__ dA, const int* __restrict__ f_dst,
                                                  const int* __restrict__ h_dst, float4* __restrict__ fc,
-                                                 float4* __restrict__ hc, const int* __restrict__ c_dst,
-                                                 float4* __restrict__ cslot,
+                                                 float4* __restrict__ hc, float* __restrict__ sA,
                                                  float* __restrict__ dn) {
     __shared__ double red[kB];
     const float beta = cg_beta(cv, red);
@@ -1060,14 +1064,9 @@ __global__ void __launch_bounds__(kB) hvp_elem_k(const float4* __restrict__ x, T
                  g3 = v3(cdd[3 * cstride + i]);
         const float sdot = dot(g0, cg_dx(cv, beta, k.x)) + dot(g1, cg_dx(cv, beta, k.y)) +
                            dot(g2, cg_dx(cv, beta, k.z)) + dot(g3, cg_dx(cv, beta, k.w));
-        // H_c dx = g (c2 g . dx): each corner's term goes to its slot in the
-        // vertex lists, so the gather reads it contiguously
-        const float sa = sdot * c2;
-        const int4 dst = reinterpret_cast<const int4*>(c_dst)[i];
-        cslot[dst.x] = f4(g0 * sa);
-        cslot[dst.y] = f4(g1 * sa);
-        cslot[dst.z] = f4(g2 * sa);
-        cslot[dst.w] = f4(g3 * sa);
+        // H_c dx = g (c2 g . dx): the scalar per contact, contiguous; the
+        // gather multiplies each corner's gradient by it
+        sA[i] = sdot * c2;
     } else if ((i -= nA_max) < n_dense) {
         const int n3 = (int)n_dense;
         double acc = 0.0;
@@ -1095,6 +1094,23 @@ __device__ __forceinline__ V3 gather_add(V3 acc, const float4* __restrict__ s, i
         acc = acc + v3(d);
     }
     for (; e < end; e += kGather) acc = acc + v3(s[e]);
+    return acc;
+}
+
+// acc + g[e] sA[cj[e]] + ... for e < end (stride kGather), in that order,
+// four entries' loads issued before their adds
+__device__ __forceinline__ V3 gather_contacts(V3 acc, const float4* __restrict__ g, const int* __restrict__ cj,
+                                              const float* __restrict__ sA, int e, int end) {
+    for (; e + 3 * kGather < end; e += 4 * kGather) {
+        const int j0 = cj[e], j1 = cj[e + kGather], j2 = cj[e + 2 * kGather], j3 = cj[e + 3 * kGather];
+        const float4 g0 = g[e], g1 = g[e + kGather], g2 = g[e + 2 * kGather], g3 = g[e + 3 * kGather];
+        const float s0 = sA[j0], s1 = sA[j1], s2 = sA[j2], s3 = sA[j3];
+        acc = acc + v3(g0) * s0;
+        acc = acc + v3(g1) * s1;
+        acc = acc + v3(g2) * s2;
+        acc = acc + v3(g3) * s3;
+    }
+    for (; e < end; e += kGather) acc = acc + v3(g[e]) * sA[cj[e]];
     return acc;
 }
 
@@ -1153,7 +1169,8 @@ __global__ void __launch_bounds__(kB) diff_gather_k(int nV, int use_elem, int us
 __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2gt, int use_elem, int use_hinge,
                                                    int use_contact, int use_dense, const float* __restrict__ vor,
                                                    CgVec cv, Incid ic, const float4* __restrict__ fc,
-                                                   const float4* __restrict__ hc, const float4* __restrict__ cslot,
+                                                   const float4* __restrict__ hc, const float4* __restrict__ cdv,
+                                                   const int* __restrict__ cj, const float* __restrict__ sA,
                                                    const float4* __restrict__ cdd, const float* __restrict__ dn,
                                                    float4* __restrict__ v_out, float4* __restrict__ Av,
                                                    double* __restrict__ vav) {
@@ -1165,7 +1182,7 @@ __global__ void __launch_bounds__(kB) hvp_gather_k(int nV, Params P, int use_m2g
     if (v < nV) {
         if (use_elem) acc = gather_add(acc, fc, ic.vf_off[v] + g, ic.vf_off[v + 1]);
         if (use_hinge) acc = gather_add(acc, hc, ic.vh_off[v] + g, ic.vh_off[v + 1]);
-        if (use_contact) acc = gather_add(acc, cslot, ic.vc_off[v] + g, ic.vc_off[v + 1]);
+        if (use_contact) acc = gather_contacts(acc, cdv, cj, sA, ic.vc_off[v] + g, ic.vc_off[v + 1]);
     }
 #pragma unroll
     for (int m = kGather / 2; m > 0; m >>= 1) {
@@ -1239,6 +1256,16 @@ __global__ void __launch_bounds__(kB) cg_update_k(const float4* __restrict__ v, 
 __global__ void invert_k(const int* __restrict__ ent, int n, int* __restrict__ dst) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) dst[ent[e]] = e;
+}
+// the active-contact corner of list entry e (slot s = 4 j + a): its distance
+// gradient (corner-major cdd) and its contact j
+__global__ void contact_lists_k(const int* __restrict__ ent, int n, const float4* __restrict__ cdd, long long cstride,
+                                float4* __restrict__ cdv, int* __restrict__ cj) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int s = ent[e], j = s >> 2, a = s & 3;
+    cdv[e] = cdd[a * cstride + j];
+    cj[e] = j;
 }
 // keys[s] = vertex of slot s of element s / width, vals[s] = s
 __global__ void slot_keys_k(const int* __restrict__ elem, int width, int stride, int n_slots,
@@ -1509,6 +1536,9 @@ static int compute_diff(Ctx& c, int nC, int mask) {
     if (k.total() > 0)
         L3(diff_k, k.total(), (const float4*)c.q, topo(c), c.P, k.v, k.f, k.h, k.c, -1.0f, (const float*)c.cd,
            (const int*)c.ctype2, c.grad, c.diag, c.hblock, c.hsym, (const int*)c.act, (const int*)c.n_act, c.cdd, ds);
+    if (c.nA > 0)
+        L3(contact_lists_k, 4LL * c.nA, (const int*)c.vc_ent, 4 * c.nA, (const float4*)c.cdd, (long long)c.capC, c.cdv,
+           c.cj);
     if (k.f + k.h + k.c > 0) {
         Incid ic{c.vf_off, c.vh_off, c.vc_off};
         L3(diff_gather_k, (long long)c.nV * kGather, c.nV, (int)(k.f > 0), (int)(k.h > 0), (int)(k.c > 0), ic,
@@ -1533,11 +1563,12 @@ static int compute_hvp(Ctx& c, int mask, const CgVec& cv, float4* v_out, float4*
         L3(hvp_elem_k, total, (const float4*)c.q, topo(c), c.P, nF, nH, nA, (const int*)c.n_act, nd, cv,
            (const int4*)c.cvert, (const float*)c.c2j, (const float*)c.hsym, (long long)c.capH, (const float4*)c.cdd,
            (long long)c.capC,
-           (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, (const int*)c.c_dst, c.cslot, c.dn);
+           (const double*)c.dense_A, (const int*)c.vf_dst, (const int*)c.vh_dst, c.fc, c.hc, c.sA, c.dn);
     Incid ic{c.vf_off, c.vh_off, c.vc_off};
     L3(hvp_gather_k, (long long)c.nV * kGather, c.nV, c.P, (int)((mask & PAMO_S3_E_M2GT) != 0), (int)(nF > 0), (int)(nH > 0),
        (int)(nA > 0), (int)(nd > 0), (const float*)c.vor, cv, ic, (const float4*)c.fc, (const float4*)c.hc,
-       (const float4*)c.cslot, (const float4*)c.cdd, (const float*)c.dn, v_out, Av, vav);
+       (const float4*)c.cdv, (const int*)c.cj, (const float*)c.sA, (const float4*)c.cdd, (const float*)c.dn, v_out,
+       Av, vav);
     return 0;
 }
 
@@ -1561,6 +1592,9 @@ static int ensure_contacts(Ctx& c, int n) {
     PAMO_TRY(dalloc(&c.vc_ent, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.cdg, 4 * (size_t)a, c.stream));
     PAMO_TRY(dalloc(&c.cvert, a, c.stream));
+    PAMO_TRY(dalloc(&c.cdv, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.cj, 4 * (size_t)a, c.stream));
+    PAMO_TRY(dalloc(&c.sA, a, c.stream));
     PAMO_TRY(dalloc(&c.c2j, a, c.stream));
     return 0;
 }
